@@ -1,0 +1,912 @@
+// Kernel engine of libgml: GmlCompileKernel (argument classification, HIP
+// source generation around the user's OpenCL-C body, hipRTC for gfx950,
+// code-object cache) and GmlLaunchKernel (launch-shape tuner, profiling).
+//
+// Reference behaviour followed (paths in the GMlib tree):
+//   GmlCompileKernel argument classification  sources/gmlib3.c:2217-2499
+//   generated-kernel contract (names, locals,
+//   counters, reads, body, writes)            sources/gmlib3.c:2502-2938
+//   high-degree twin kernel                   sources/gmlib3.c:2530-2577
+//   NewOclKrn compile options                 sources/gmlib3.c:2971-3057
+//   GmlLaunchKernel / RunOclKrn               sources/gmlib3.c:3064-3222
+//   GmlGetKernelRunTime                       sources/gmlib3.c:4343-4366
+//
+// What differs by design: uplink rows are read from a SELL-64 layout (one
+// coalesced 256-byte load per ball slot and wave) instead of per-lane int16
+// vectors; uplink data of several items per line are fetched lazily, only
+// when the body reads them; launches are asynchronous on the handle's
+// stream; the work-group size (and XCD block remap) is tuned with HIP events
+// without serialising the queue.
+
+#include "gml_internal.h"
+
+#include <hip/hiprtc.h>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_map>
+
+namespace gml {
+
+static const char *kPrelude =
+#include "gml_prelude.inc"
+;
+
+// ---------------------------------------------------------------------------
+// -cl-single-precision-constant: unsuffixed floating literals of user code
+// become float literals.  Comments, strings and character literals are
+// skipped; hexadecimal literals are left alone.
+// ---------------------------------------------------------------------------
+static std::string SinglePrecisionLiterals(const char *s)
+{
+   std::string out;
+   if(!s) return out;
+   size_t n = strlen(s), i = 0;
+   out.reserve(n + n / 16);
+   auto isid = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
+
+   while(i < n)
+   {
+      char c = s[i];
+      if(c == '/' && i + 1 < n && s[i + 1] == '/')
+      {
+         size_t j = i;
+         while(j < n && s[j] != '\n') j++;
+         out.append(s + i, j - i); i = j; continue;
+      }
+      if(c == '/' && i + 1 < n && s[i + 1] == '*')
+      {
+         size_t j = i + 2;
+         while(j + 1 < n && !(s[j] == '*' && s[j + 1] == '/')) j++;
+         j = j + 2 <= n ? j + 2 : n;
+         out.append(s + i, j - i); i = j; continue;
+      }
+      if(c == '"' || c == '\'')
+      {
+         size_t j = i + 1;
+         while(j < n && s[j] != c) { if(s[j] == '\\') j++; j++; }
+         j = j < n ? j + 1 : n;
+         out.append(s + i, j - i); i = j; continue;
+      }
+      if(isid(c) && !isdigit((unsigned char)c))
+      {
+         size_t j = i;
+         while(j < n && isid(s[j])) j++;
+         out.append(s + i, j - i); i = j; continue;
+      }
+      if(isdigit((unsigned char)c) || (c == '.' && i + 1 < n && isdigit((unsigned char)s[i + 1])))
+      {
+         // pp-number: digits, letters, '.', and exponent signs
+         size_t j = i;
+         bool hex = (c == '0' && i + 1 < n && (s[i + 1] == 'x' || s[i + 1] == 'X'));
+         while(j < n)
+         {
+            char d = s[j];
+            if(isalnum((unsigned char)d) || d == '.' || d == '_') { j++; continue; }
+            if((d == '+' || d == '-') && j > i && (s[j - 1] == 'e' || s[j - 1] == 'E' || (hex && (s[j - 1] == 'p' || s[j - 1] == 'P'))))
+            { j++; continue; }
+            break;
+         }
+         std::string tok(s + i, j - i);
+         bool flt = false;
+         if(!hex)
+            for(char d : tok) if(d == '.' || d == 'e' || d == 'E') { flt = true; break; }
+         if(flt)
+         {
+            char last = tok.back();
+            if(last == 'l' || last == 'L') tok.pop_back();
+            last = tok.back();
+            if(last != 'f' && last != 'F') tok.push_back('f');
+         }
+         out += tok; i = j; continue;
+      }
+      out.push_back(c); i++;
+   }
+   return out;
+}
+
+// ---------------------------------------------------------------------------
+// Code-object cache shared by every handle of the process (hipRTC output for
+// gfx950 keyed by a 64-bit FNV-1a hash of source and options).
+// ---------------------------------------------------------------------------
+static std::mutex g_cache_mtx;
+static std::unordered_map<uint64_t, std::string> g_cache;
+
+static uint64_t Fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull)
+{
+   for(unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+   return h;
+}
+
+static bool RtcCompile(const std::string &src, const std::string &name, const std::string &cflags,
+                       std::string *code, std::string *log)
+{
+   std::vector<std::string> opts = {
+      "--gpu-architecture=gfx950", "-O3", "-ffp-contract=fast", "-std=c++17",
+      "-Wno-c++11-narrowing", "-Wno-unused-value", "-Wno-unused-variable",
+      "-Wno-macro-redefined", "-Wno-unused-but-set-variable",
+      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+   // user options, split on blanks (GmlSetCompilerOptions)
+   {
+      size_t i = 0;
+      while(i < cflags.size())
+      {
+         while(i < cflags.size() && isspace((unsigned char)cflags[i])) i++;
+         size_t j = i;
+         while(j < cflags.size() && !isspace((unsigned char)cflags[j])) j++;
+         if(j > i)
+         {
+            std::string o = cflags.substr(i, j - i);
+            if(o.rfind("-cl-", 0) != 0)          // OpenCL-only options have no meaning here
+               opts.push_back(o);
+         }
+         i = j;
+      }
+   }
+   std::string key = src;
+   for(auto &o : opts) { key += '\x01'; key += o; }
+   uint64_t h = Fnv1a(key);
+   {
+      std::lock_guard<std::mutex> lk(g_cache_mtx);
+      auto it = g_cache.find(h);
+      if(it != g_cache.end()) { *code = it->second; return true; }
+   }
+   hiprtcProgram prog;
+   if(hiprtcCreateProgram(&prog, src.c_str(), (name + ".hip").c_str(), 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+   {
+      *log = "hiprtcCreateProgram failed";
+      return false;
+   }
+   std::vector<const char *> o;
+   for(auto &x : opts) o.push_back(x.c_str());
+   hiprtcResult r = hiprtcCompileProgram(prog, (int)o.size(), o.data());
+   size_t ls = 0;
+   hiprtcGetProgramLogSize(prog, &ls);
+   if(ls > 1)
+   {
+      std::string l(ls, '\0');
+      hiprtcGetProgramLog(prog, &l[0]);
+      *log = l;
+   }
+   if(r != HIPRTC_SUCCESS)
+   {
+      hiprtcDestroyProgram(&prog);
+      return false;
+   }
+   size_t cs = 0;
+   hiprtcGetCodeSize(prog, &cs);
+   code->assign(cs, '\0');
+   hiprtcGetCode(prog, &(*code)[0]);
+   hiprtcDestroyProgram(&prog);
+   std::lock_guard<std::mutex> lk(g_cache_mtx);
+   g_cache[h] = *code;
+   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Argument descriptors (the reference's ArgSct, gmlib3.c:86-91)
+// ---------------------------------------------------------------------------
+enum ArgKind { AkData = 0, AkBall, AkDeg, AkDown, AkTrans };
+
+struct Arg {
+   int kind = AkData;
+   int MshTyp = 0, DatIdx = 0, LnkDir = 0, LnkIdx = -1, CntIdx = -1;
+   int LnkDeg = -1, MaxDeg = -1, NmbItm = 1, ItmLen = 1, ItmTyp = GmlInt, FlgTab = 0;
+   std::string nam, VoyNam;
+   int SrcTyp = 0, DstTyp = 0;   // for AkBall: which uplink
+};
+
+static std::string Fmt(const char *fmt, ...)
+{
+   char buf[2048];
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(buf, sizeof(buf), fmt, ap);
+   va_end(ap);
+   return buf;
+}
+
+static const char *HexDig = "0123456789abcdef";
+
+// Pointer type of an argument in the kernel signature.
+static std::string ArgDecl(const Arg &a)
+{
+   if(a.kind == AkBall)
+      return Fmt("const int *__restrict__ %sTab, const unsigned int *__restrict__ %sOff", a.nam.c_str(), a.nam.c_str());
+   if(a.NmbItm <= 1)
+      return Fmt("%s *%sTab", OclTypStr[a.ItmTyp], a.nam.c_str());
+   return Fmt("%s (*%sTab)[%d]", OclTypStr[a.ItmTyp], a.nam.c_str(), a.NmbItm);
+}
+
+// Emit the generated kernel.  lazy = 1 makes multi-item uplink data lazy
+// accessors instead of eagerly filled local arrays.
+static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam, int MshTyp,
+                              const std::vector<Arg> &args, const char *ParSrc, bool lazy)
+{
+   std::string s;
+   s.reserve(16384);
+   s += "// GMlib MESHING AND GEOMETRICAL TOOLKIT\n";
+   s += kPrelude;
+   if(lazy)
+      s += "#define GML_LAZY_UPLINKS 1\n"
+           "template<class T, int NI> struct gml_uprow {\n"
+           "   T (*tab)[NI]; int idx; bool ok; T nul;\n"
+           "   __device__ __forceinline__ T operator[](int j) const { return ok ? tab[idx][j] : nul; }\n"
+           "};\n"
+           "template<class T, int NI, int W> struct gml_upn {\n"
+           "   T (*tab)[NI]; const int *idx; int deg; T nul;\n"
+           "   __device__ __forceinline__ gml_uprow<T, NI> operator[](int k) const { return gml_uprow<T, NI>{tab, idx[k], k < deg, nul}; }\n"
+           "};\n";
+   if(g->UsrTlk)
+   {
+      s += "// USER'S TOOLKIT\n";
+      s += SinglePrecisionLiterals(g->UsrTlk);
+      s += "\n";
+   }
+   s += "// USER'S ARGUMENTS STRUCTURE\n";
+   if(ParSrc)
+      s += SinglePrecisionLiterals(ParSrc);
+   else
+      s += "typedef struct { int GmlNoParameters; } GmlParSct;";
+   s += "\n";
+
+   // --- header
+   s += "// KERNEL HEADER\n";
+   s += Fmt("extern \"C\" __global__ void __launch_bounds__(1024) %s(", PrcNam);
+   for(auto &a : args)
+      s += "\n   " + ArgDecl(a) + ",";
+   s += "\n   GmlParSct *GmlPar,\n   const int2 count,\n   const int gml_remap)\n{\n";
+
+   // --- locals (gmlib3.c:2670-2732)
+   s += "// KERNEL VARIABLES DEFINITION\n";
+   for(size_t i = 0; i < args.size(); i++)
+   {
+      const Arg &a = args[i];
+      if(a.FlgTab & GmlManual) continue;
+      const Arg *L = a.LnkIdx >= 0 ? &args[a.LnkIdx] : nullptr;
+      const Arg *C = a.CntIdx >= 0 ? &args[a.CntIdx] : nullptr;
+      const char *T = OclTypStr[a.ItmTyp];
+      bool lz = lazy && C && a.NmbItm > 1;
+      if(a.kind == AkBall)
+      {
+         s += Fmt("   int gml_%s[%d];\n", a.nam.c_str(), a.NmbItm * a.ItmLen);  // shifted entries, one per slot
+         if(a.NmbItm > 1) s += Fmt("   int16 %s[%d];\n", a.nam.c_str(), a.NmbItm);
+         else             s += Fmt("   %s %s;\n", OclTypStr[a.ItmTyp], a.nam.c_str());
+      }
+      else if(lz)
+         ;  // declared after the reads as an accessor
+      else if(L && L->MaxDeg > 1)
+      {
+         if(a.NmbItm > 1) s += Fmt("   %s %s[%d][%d];\n", T, a.nam.c_str(), L->MaxDeg, a.NmbItm);
+         else             s += Fmt("   %s %s[%d];\n", T, a.nam.c_str(), L->MaxDeg);
+      }
+      else
+      {
+         if(a.NmbItm > 1) s += Fmt("   %s %s[%d];\n", T, a.nam.c_str(), a.NmbItm);
+         else             s += Fmt("   %s %s;\n", T, a.nam.c_str());
+      }
+      if(C)
+      {
+         s += Fmt("   %s %sDeg;\n", OclTypStr[C->ItmTyp], a.nam.c_str());
+         s += Fmt("   %s %sNul;\n", T, a.nam.c_str());
+         s += Fmt("   #define   %sDegMax %d\n", a.nam.c_str(), L->MaxDeg);
+      }
+      else if(L && L->LnkDir == 0)
+         s += Fmt("   %s %sNul;\n", T, a.nam.c_str());
+      if(a.FlgTab & GmlVoyeurs)
+         s += Fmt("   char      %s[%d];\n", a.VoyNam.c_str(), a.NmbItm * a.ItmLen);
+   }
+
+   // --- counters (gmlib3.c:2739-2748)
+   s += "// KERNEL COUNTERS\n";
+   s += "   int       cnt = gml_gid(gml_remap);\n";
+   s += Fmt("   int       %sIdx = cnt + count.s1;\n\n", BalTypStr[MshTyp]);
+   s += "   if(cnt >= count.s0)\n      return;\n\n";
+
+   // --- reads (gmlib3.c:2755-2885)
+   s += "// KERNEL MEMORY READINGS\n";
+   for(size_t i = 0; i < args.size(); i++)
+   {
+      const Arg &a = args[i];
+      if(!(a.FlgTab & GmlReadMode) || (a.FlgTab & GmlManual)) continue;
+      const Arg *L = a.LnkIdx >= 0 ? &args[a.LnkIdx] : nullptr;
+      const Arg *C = a.CntIdx >= 0 ? &args[a.CntIdx] : nullptr;
+      const char *n = a.nam.c_str();
+
+      if(a.kind == AkBall)
+      {
+         // SELL-64 row of this entity; slots past the degree read as 0 as in
+         // the reference's zero-padded rows.
+         int w = a.NmbItm * a.ItmLen;
+         const Arg *D = nullptr;
+         for(auto &b : args) if(b.kind == AkDeg && b.SrcTyp == a.SrcTyp && b.DstTyp == a.DstTyp) D = &b;
+         s += Fmt("   {\n      const int gml_e = cnt + count.s1;\n"
+                  "      const int *gml_row = %sTab + (size_t)%sOff[gml_e >> 6] * 64 + (gml_e & 63);\n"
+                  "      const int gml_d = %sTab[gml_e];\n", n, n, D ? D->nam.c_str() : "");
+         s += Fmt("#pragma unroll\n      for(int k = 0; k < %d; k++)\n      {\n"
+                  "         int v = k < gml_d ? gml_row[k * 64] : 0;\n", w);
+         if(a.FlgTab & GmlVoyeurs)
+            s += Fmt("         %s[k] = (char)(v & 7);\n", a.VoyNam.c_str());
+         s += Fmt("         gml_%s[k] = v >> 4;\n      }\n   }\n", n);
+         for(int v = 0; v < a.NmbItm; v++)
+         {
+            if(a.NmbItm > 1) s += Fmt("   %s[%d] = (int16){", n, v);
+            else             s += Fmt("   %s = (%s){", n, OclTypStr[a.ItmTyp]);
+            for(int l = 0; l < a.ItmLen; l++)
+               s += Fmt("%sgml_%s[%d]", l ? "," : "", n, v * a.ItmLen + l);
+            s += "};\n";
+         }
+         s += "\n";
+         continue;
+      }
+      if(a.kind == AkDeg)
+      {
+         s += Fmt("   %s = %sTab[ cnt + count.s1 ];\n\n", n, n);
+         continue;
+      }
+
+      if(C)
+      {
+         s += Fmt("   %sDeg = %s;\n", n, C->nam.c_str());
+         s += Fmt("   %sNul = %s;\n", n, OclNulVec[a.ItmTyp]);
+         if(lazy && a.NmbItm > 1)
+         {
+            s += Fmt("   gml_upn<%s, %d, %d> %s{%sTab, gml_%s, %sDeg, %sNul};\n\n",
+                     OclTypStr[a.ItmTyp], a.NmbItm, L->MaxDeg, n, n, L->nam.c_str(), n, n);
+            continue;
+         }
+      }
+      else if(L && L->LnkDir == 0)
+         s += Fmt("   %sNul = %s;\n", n, OclNulVec[a.ItmTyp]);
+
+      int siz = (L && L->NmbItm * L->ItmLen > 1) ? L->NmbItm * L->ItmLen : 1;
+      if(L && L->kind == AkBall) siz = L->MaxDeg;
+      for(int j = 0; j < a.NmbItm; j++)
+      {
+         std::string td1 = a.NmbItm > 1 ? Fmt("[%d]", j) : "";
+         for(int k = 0; k < siz; k++)
+         {
+            std::string td2 = siz > 1 ? Fmt("[%d]", k) : "";
+            std::string lnk;
+            if(!L)
+               lnk = "cnt + count.s1";
+            else if(L->kind == AkBall)
+               lnk = Fmt("gml_%s[%d]", L->nam.c_str(), k);
+            else
+            {
+               std::string l1 = L->NmbItm > 1 ? Fmt("[%d]", k / 16) : "";
+               std::string l2 = L->ItmLen > 1 ? Fmt(".s%c", HexDig[k & 15]) : "";
+               lnk = L->nam + l1 + l2;
+            }
+            if(C)
+               s += Fmt("   %s%s%s = (%sDeg >= %d) ? %sTab[ %s ]%s : %sNul;\n",
+                        n, td2.c_str(), td1.c_str(), n, k + 1, n, lnk.c_str(), td1.c_str(), n);
+            else if(L && L->LnkDir == 0)
+               s += Fmt("   %s%s%s = %s ? %sTab[ %s ]%s : %sNul;\n",
+                        n, td2.c_str(), td1.c_str(), lnk.c_str(), n, lnk.c_str(), td1.c_str(), n);
+            else
+               s += Fmt("   %s%s%s = %sTab[ %s ]%s;\n", n, td2.c_str(), td1.c_str(), n, lnk.c_str(), td1.c_str());
+         }
+      }
+      s += "\n";
+   }
+
+   // --- body
+   s += "// USER'S KERNEL CODE\n";
+   s += SinglePrecisionLiterals(KrnSrc);
+   s += "\n";
+
+   // --- writes (gmlib3.c:2903-2938): only the loop entity's own line
+   s += "\n// KERNEL MEMORY WRITINGS\n";
+   for(auto &a : args)
+   {
+      if(!(a.FlgTab & GmlWriteMode) || (a.FlgTab & GmlManual)) continue;
+      if(a.NmbItm == 1)
+         s += Fmt("   %sTab[ cnt + count.s1 ] = %s;\n", a.nam.c_str(), a.nam.c_str());
+      else
+         for(int c = 0; c < a.NmbItm; c++)
+            s += Fmt("   %sTab[ cnt + count.s1 ][%d] = %s[%d];\n", a.nam.c_str(), c, a.nam.c_str(), c);
+   }
+   s += "}\n";
+   return s;
+}
+
+// Compile the emitted source into a new kernel slot.
+static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std::string *log)
+{
+   int idx = 0;
+   for(int i = 1; i <= GmlMaxKrn; i++)
+      if(!g->krn[i].use) { idx = i; break; }
+   if(!idx)
+   {
+      fprintf(stderr, "gml: all %d kernel slots are in use\n", GmlMaxKrn);
+      return 0;
+   }
+   std::string code;
+   if(!RtcCompile(src, PrcNam, g->cflags, &code, log))
+      return 0;
+   Kernel *k = &g->krn[idx];
+   *k = Kernel();
+   if(g->host_only)
+   {
+      k->use = true;
+      k->name = PrcNam;
+      k->src = src;
+      if(idx > g->NmbKrn) g->NmbKrn = idx;
+      return idx;
+   }
+   if(hipModuleLoadData(&k->mod, code.data()) != hipSuccess
+   || hipModuleGetFunction(&k->fn, k->mod, PrcNam) != hipSuccess)
+   {
+      (void)hipGetLastError();
+      fprintf(stderr, "gml: loading the code object of kernel %s failed\n", PrcNam);
+      if(k->mod) (void)hipModuleUnload(k->mod);
+      *k = Kernel();
+      return 0;
+   }
+   k->use = true;
+   k->name = PrcNam;
+   k->src = src;
+   if(idx > g->NmbKrn) g->NmbKrn = idx;
+   return idx;
+}
+
+// ---------------------------------------------------------------------------
+// Launch
+// ---------------------------------------------------------------------------
+static const int kShapes[][2] = {{256, 1}, {128, 1}, {512, 1}, {64, 1}, {1024, 1}, {256, 0}, {256, 1}};
+static const int kNmbShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
+{
+   size_t keep = 0;
+   for(size_t i = 0; i < ev.size(); i++)
+   {
+      if(hipEventQuery(ev[i].end) == hipSuccess)
+      {
+         float ms = 0.f;
+         if(hipEventElapsedTime(&ms, ev[i].beg, ev[i].end) == hipSuccess)
+            *acc += ms * 1e-3;
+         (void)hipEventDestroy(ev[i].beg);
+         (void)hipEventDestroy(ev[i].end);
+      }
+      else
+         ev[keep++] = ev[i];
+   }
+   (void)hipGetLastError();
+   ev.resize(keep);
+}
+
+static bool NewRec(LaunchRec *r)
+{
+   if(hipEventCreate(&r->beg) != hipSuccess) return false;
+   if(hipEventCreate(&r->end) != hipSuccess) { (void)hipEventDestroy(r->beg); return false; }
+   return true;
+}
+
+int RunKernel(GmlCtx *g, Kernel *k)
+{
+   // argument block: one device pointer per slot (a ball slot also passes its
+   // chunk offsets), then GmlPar, count and the remap flag
+   std::vector<void *> ptrs;
+   ptrs.reserve(k->args.size() * 2 + 1);
+   for(int di : k->args)
+   {
+      if(di < 1 || di > GmlMaxDat || !g->dat[di].use || !g->dat[di].dev)
+      {
+         fprintf(stderr, "gml: invalid argument data %d of kernel %s\n", di, k->name.c_str());
+         return -1;
+      }
+      ptrs.push_back(g->dat[di].dev);
+      if(g->dat[di].kind == DkBal)
+         ptrs.push_back(g->dat[g->dat[di].OffIdx].dev);
+   }
+   void *par = g->ParIdx && g->dat[g->ParIdx].dev ? g->dat[g->ParIdx].dev : g->dummy_par;
+   long long n = k->NmbLin[0];
+   if(n <= 0) return 1;
+   int count[2] = {(int)k->NmbLin[0], (int)k->NmbLin[1]};
+
+   int grp = k->grp, remap = k->remap, tuning = 0;
+   if(!k->tuned)
+   {
+      grp = kShapes[k->tune_step][0];
+      remap = kShapes[k->tune_step][1];
+      tuning = 1;
+   }
+   int remap_arg = remap;
+   std::vector<void *> kp;
+   for(auto &p : ptrs) kp.push_back((void *)&p);
+   kp.push_back((void *)&par);
+   kp.push_back((void *)count);
+   kp.push_back((void *)&remap_arg);
+
+   LaunchRec rec, tr;
+   bool have = NewRec(&rec);
+   bool tun = tuning && NewRec(&tr);
+   if(have) (void)hipEventRecord(rec.beg, g->stream);
+   if(tun) (void)hipEventRecord(tr.beg, g->stream);
+   unsigned int nb = (unsigned int)((n + grp - 1) / grp);
+   hipError_t e = hipModuleLaunchKernel(k->fn, nb, 1, 1, grp, 1, 1, 0, g->stream, kp.data(), nullptr);
+   if(e != hipSuccess)
+   {
+      (void)hipGetLastError();
+      fprintf(stderr, "gml: launching kernel %s failed: %s\n", k->name.c_str(), hipGetErrorString(e));
+      if(have) { (void)hipEventDestroy(rec.beg); (void)hipEventDestroy(rec.end); }
+      if(tun) { (void)hipEventDestroy(tr.beg); (void)hipEventDestroy(tr.end); }
+      return -6;
+   }
+   if(tun) (void)hipEventRecord(tr.end, g->stream);
+   if(have)
+   {
+      (void)hipEventRecord(rec.end, g->stream);
+      k->evts.push_back(rec);
+      if(k->evts.size() > 256) HarvestEvents(k->evts, &k->run_time);
+   }
+
+   if(tuning)
+   {
+      if(tun) k->tune_ev.push_back(tr);
+      k->tune_step++;
+      if(k->tune_step == kNmbShapes)
+      {
+         // every shape launched once: keep the fastest.  The first shape is
+         // launched again last, once code and caches are warm, and only that
+         // second timing of it counts.
+         float best = 1e30f;
+         int bi = kNmbShapes - 1;
+         if(!k->tune_ev.empty()) (void)hipEventSynchronize(k->tune_ev.back().end);
+         for(int i = 1; i < (int)k->tune_ev.size() && i < kNmbShapes; i++)
+         {
+            float ms = 0.f;
+            if(hipEventElapsedTime(&ms, k->tune_ev[i].beg, k->tune_ev[i].end) == hipSuccess && ms < best)
+            { best = ms; bi = i; }
+         }
+         (void)hipGetLastError();
+         for(auto &t : k->tune_ev) { (void)hipEventDestroy(t.beg); (void)hipEventDestroy(t.end); }
+         k->tune_ev.clear();
+         k->grp = kShapes[bi][0];
+         k->remap = kShapes[bi][1];
+         k->tuned = 1;
+      }
+   }
+
+   // data written by the kernel: host mirrors are stale
+   for(int di : k->wargs)
+      g->dat[di].host_valid = false;
+   return 1;
+}
+
+} // namespace gml
+
+using namespace gml;
+
+// ---------------------------------------------------------------------------
+// GmlCompileKernel: classify arguments exactly as gmlib3.c:2238-2499, build
+// missing links, emit and compile; add the high-degree twin when the uplink
+// of the kernel has one.
+// ---------------------------------------------------------------------------
+static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, int MshTyp,
+                            int NmbTyp, const int *IdxTab, const int *FlgIn, const int *LnkIn)
+{
+   if(!KrnSrc || !PrcNam || MshTyp < 0 || MshTyp >= GmlMaxEleTyp || NmbTyp < 0 || NmbTyp > GmlMaxDat)
+      return 0;
+   std::vector<int> FlgTab(FlgIn, FlgIn + NmbTyp), LnkTab(LnkIn, LnkIn + NmbTyp), CntTab(NmbTyp, 0);
+   for(int i = 0; i < NmbTyp; i++)
+      if(IdxTab[i] < 1 || IdxTab[i] > GmlMaxDat || !g->dat[IdxTab[i]].use)
+      {
+         fprintf(stderr, "gml: kernel %s: invalid data index %d\n", PrcNam, IdxTab[i]);
+         return 0;
+      }
+   const char *ParSrc = g->ParIdx ? g->dat[g->ParIdx].src : nullptr;
+
+   // 1. links between the kernel's entity kind and each argument's kind
+   std::vector<int> UpSrc(NmbTyp, -1), UpDst(NmbTyp, -1);
+   for(int i = 0; i < NmbTyp; i++)
+   {
+      Data *d = &g->dat[IdxTab[i]];
+      if(LnkTab[i] || d->MshTyp == MshTyp) continue;
+      int Dst = d->MshTyp;
+      if(MshTypDim[MshTyp] > MshTypDim[Dst])
+      {
+         if(!g->LnkMat[MshTyp][Dst] && !BuildDownlink(g, MshTyp, Dst)) return 0;
+         LnkTab[i] = g->LnkMat[MshTyp][Dst];
+      }
+      else if(MshTypDim[MshTyp] < MshTypDim[Dst])
+      {
+         if(!g->up[MshTyp][Dst].BalIdx && !BuildUplink(g, MshTyp, Dst)) return 0;
+         LnkTab[i] = g->up[MshTyp][Dst].BalIdx;
+         CntTab[i] = g->up[MshTyp][Dst].DegIdx;
+         UpSrc[i] = MshTyp; UpDst[i] = Dst;
+      }
+      else
+      {
+         fprintf(stderr, "gml: kernel %s: data %d of an entity kind of the same dimension needs an explicit link\n",
+                 PrcNam, IdxTab[i]);
+         return 0;
+      }
+   }
+
+   // 2. context arguments (links first; an uplink is followed by its degree)
+   std::vector<Arg> args;
+   int HghSrc = -1, HghDst = -1;
+   for(int i = 0; i < NmbTyp; i++)
+   {
+      if(!LnkTab[i]) continue;
+      bool seen = false;
+      for(auto &a : args) if(a.DatIdx == LnkTab[i]) seen = true;
+      if(seen) continue;
+      Data *d = &g->dat[IdxTab[i]];
+      int Dst = d->MshTyp;
+      Data *ld = &g->dat[LnkTab[i]];
+      Arg a;
+      a.MshTyp = Dst;
+      a.DatIdx = LnkTab[i];
+      a.FlgTab = GmlReadMode;
+      a.nam = ld->nam;
+      if(UpSrc[i] >= 0)
+      {
+         Uplink *u = &g->up[MshTyp][Dst];
+         a.kind = AkBall;
+         a.LnkDir = 1;
+         a.MaxDeg = u->BasWid;
+         int cnt, vec, typ;
+         GetCntVec(u->BasWid, &cnt, &vec, &typ);
+         a.NmbItm = cnt; a.ItmLen = vec; a.ItmTyp = typ;
+         a.SrcTyp = MshTyp; a.DstTyp = Dst;
+         if(u->HghWid)
+         {
+            if(HghSrc >= 0 && (HghSrc != MshTyp || HghDst != Dst))
+            {
+               fprintf(stderr, "gml: kernel %s mixes two uplinks with high-degree rows\n", PrcNam);
+               return 0;
+            }
+            HghSrc = MshTyp; HghDst = Dst;
+         }
+         if(FlgTab[i] & GmlVoyeurs)
+         {
+            a.VoyNam = ld->VoyNam;
+            a.FlgTab |= GmlVoyeurs;
+            FlgTab[i] &= ~GmlVoyeurs;
+         }
+         args.push_back(a);
+         Arg c;
+         c.kind = AkDeg;
+         c.MshTyp = Dst;
+         c.DatIdx = u->DegIdx;
+         c.FlgTab = GmlReadMode;
+         c.ItmTyp = GmlInt;
+         c.LnkDeg = c.MaxDeg = 1;
+         c.nam = g->dat[u->DegIdx].nam;
+         c.SrcTyp = MshTyp; c.DstTyp = Dst;
+         args.push_back(c);
+      }
+      else
+      {
+         // downlink (element table or built link) or an explicit user link
+         a.kind = (MshTypDim[MshTyp] == MshTypDim[Dst]) ? AkTrans : AkDown;
+         a.LnkDir = (a.kind == AkTrans) ? 0 : -1;
+         a.NmbItm = ld->NmbItm;
+         a.ItmLen = ld->ItmLen;
+         a.ItmTyp = ld->ItmTyp;
+         a.LnkDeg = a.MaxDeg = ld->NmbItm * ld->ItmLen;
+         args.push_back(a);
+      }
+   }
+
+   // 3. data arguments, each optionally followed by its ref table
+   for(int i = 0; i < NmbTyp; i++)
+   {
+      Data *d = &g->dat[IdxTab[i]];
+      int LnkPos = -1, CptPos = -1;
+      for(int j = 0; j < (int)args.size(); j++)
+      {
+         if(LnkTab[i] && args[j].DatIdx == LnkTab[i]) LnkPos = j;
+         if(CntTab[i] && args[j].DatIdx == CntTab[i]) CptPos = j;
+      }
+      Arg a;
+      a.MshTyp = d->MshTyp;
+      a.DatIdx = IdxTab[i];
+      a.LnkDir = 0;
+      a.LnkIdx = LnkPos;
+      a.CntIdx = CptPos;
+      if(LnkPos >= 0) { a.MaxDeg = args[LnkPos].MaxDeg; a.LnkDeg = CptPos >= 0 ? -1 : args[LnkPos].LnkDeg; }
+      bool RefFlg = (FlgTab[i] & GmlRefFlag) != 0;
+      FlgTab[i] &= ~GmlRefFlag;
+      a.NmbItm = d->NmbItm;
+      a.ItmLen = d->ItmLen;
+      a.ItmTyp = d->ItmTyp;
+      a.FlgTab = FlgTab[i];
+      a.nam = d->nam;
+      args.push_back(a);
+      if(!RefFlg || CptPos >= 0) continue;
+      int RefIdx = g->RefIdx[d->MshTyp];
+      if(!RefIdx) continue;
+      Arg r;
+      r.MshTyp = d->MshTyp;
+      r.DatIdx = RefIdx;
+      r.LnkIdx = LnkPos;
+      r.CntIdx = -1;
+      r.MaxDeg = LnkPos >= 0 ? args[LnkPos].MaxDeg : -1;
+      r.NmbItm = 1; r.ItmLen = 0; r.ItmTyp = GmlInt;
+      r.FlgTab = FlgTab[i] & ~GmlWriteMode;
+      r.nam = g->dat[RefIdx].nam;
+      args.push_back(r);
+   }
+
+   // 4. emit and compile: lazy uplink accessors first, eager arrays if the
+   //    body needs them as lvalues
+   bool any_lazy = false;
+   for(auto &a : args) if(a.CntIdx >= 0 && a.NmbItm > 1) any_lazy = true;
+   std::string log, src;
+   int KrnIdx = 0;
+   if(any_lazy)
+   {
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true);
+      KrnIdx = NewKernel(g, src, PrcNam, &log);
+   }
+   if(!KrnIdx)
+   {
+      log.clear();
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false);
+      KrnIdx = NewKernel(g, src, PrcNam, &log);
+   }
+   if(!KrnIdx)
+   {
+      printf("Compiling the kernel %s failed\n%s\n", PrcNam, log.c_str());
+      fflush(stdout);
+      if(g->DbgFlg) puts(src.c_str());
+      return 0;
+   }
+   DbgPrint(g, "Generated source for kernel=%s, index=%2d\n%s\n", PrcNam, KrnIdx, g->DbgFlg ? src.c_str() : "");
+
+   Kernel *k = &g->krn[KrnIdx];
+   for(auto &a : args)
+   {
+      k->args.push_back(a.DatIdx);
+      if(a.FlgTab & (GmlWriteMode | GmlManual)) k->wargs.push_back(a.DatIdx);
+   }
+   k->NmbLin[1] = 0;
+   if(HghSrc < 0)
+      k->NmbLin[0] = g->NmbEle[MshTyp];
+   else
+      k->NmbLin[0] = g->up[HghSrc][HghDst].MaxPos;
+   if(HghSrc < 0) return KrnIdx;
+
+   // 5. high-degree twin: same arguments, wider ball, rows MaxPos..N-1
+   Uplink *u = &g->up[HghSrc][HghDst];
+   for(auto &a : args)
+      if(a.kind == AkBall && a.SrcTyp == HghSrc && a.DstTyp == HghDst)
+      {
+         int cnt, vec, typ;
+         GetCntVec(u->HghWid, &cnt, &vec, &typ);
+         a.MaxDeg = u->HghWid; a.NmbItm = cnt; a.ItmLen = vec; a.ItmTyp = typ;
+      }
+   for(auto &a : args)
+      if(a.LnkIdx >= 0 && args[a.LnkIdx].kind == AkBall)
+         a.MaxDeg = args[a.LnkIdx].MaxDeg;
+   int HghIdx = 0;
+   if(any_lazy)
+   {
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true);
+      HghIdx = NewKernel(g, src, PrcNam, &log);
+   }
+   if(!HghIdx)
+   {
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false);
+      HghIdx = NewKernel(g, src, PrcNam, &log);
+   }
+   if(!HghIdx)
+   {
+      printf("Compiling the high-degree kernel %s failed\n%s\n", PrcNam, log.c_str());
+      return 0;
+   }
+   Kernel *h = &g->krn[HghIdx];
+   h->args = g->krn[KrnIdx].args;
+   h->wargs = g->krn[KrnIdx].wargs;
+   h->NmbLin[0] = g->NmbEle[MshTyp] - u->MaxPos;
+   h->NmbLin[1] = u->MaxPos;
+   g->krn[KrnIdx].HghIdx = HghIdx;
+   return KrnIdx;
+}
+
+extern "C" {
+
+int GmlCompileKernel(size_t GmlIdx, char *KrnSrc, char *PrcNam, int MshTyp, int NmbTyp, ...)
+{
+   GML_CTX(g, GmlIdx);
+   if(NmbTyp < 0 || NmbTyp > GmlMaxDat) return 0;
+   std::vector<int> idx(NmbTyp), flg(NmbTyp), lnk(NmbTyp);
+   va_list ap;
+   va_start(ap, NmbTyp);
+   for(int i = 0; i < NmbTyp; i++)
+   {
+      idx[i] = va_arg(ap, int);
+      flg[i] = va_arg(ap, int);
+      lnk[i] = va_arg(ap, int);
+   }
+   va_end(ap);
+   return CompileKernelTab(g, KrnSrc, PrcNam, MshTyp, NmbTyp, idx.data(), flg.data(), lnk.data());
+}
+
+int GmlxCompileKernelv(size_t GmlIdx, const char *KrnSrc, const char *PrcNam, int MshTyp,
+                       int NmbTyp, const int *tri)
+{
+   GML_CTX(g, GmlIdx);
+   if(NmbTyp < 0 || NmbTyp > GmlMaxDat || (NmbTyp && !tri)) return 0;
+   std::vector<int> idx(NmbTyp), flg(NmbTyp), lnk(NmbTyp);
+   for(int i = 0; i < NmbTyp; i++)
+   {
+      idx[i] = tri[3 * i];
+      flg[i] = tri[3 * i + 1];
+      lnk[i] = tri[3 * i + 2];
+   }
+   return CompileKernelTab(g, KrnSrc, PrcNam, MshTyp, NmbTyp, idx.data(), flg.data(), lnk.data());
+}
+
+int GmlLaunchKernel(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use || !g->krn[idx].fn)
+      return -1;
+   int res = RunKernel(g, &g->krn[idx]);
+   if(res != 1) return res;
+   if(g->krn[idx].HghIdx)
+      res = RunKernel(g, &g->krn[g->krn[idx].HghIdx]);
+   return res;
+}
+
+// Sum of the device durations of every launch of a kernel (and of its
+// high-degree twin), in seconds.
+double GmlGetKernelRunTime(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use || g->host_only)
+      return -1.;
+   HIPCHK(hipStreamSynchronize(g->stream));
+   Kernel *k = &g->krn[idx];
+   HarvestEvents(k->evts, &k->run_time);
+   double t = k->run_time;
+   if(k->HghIdx)
+   {
+      Kernel *h = &g->krn[k->HghIdx];
+      HarvestEvents(h->evts, &h->run_time);
+      t += h->run_time;
+   }
+   return t;
+}
+
+const char *GmlxGetKernelSource(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return nullptr;
+   return g->krn[idx].src.c_str();
+}
+
+int GmlxGetKernelShape(size_t GmlIdx, int idx, int *GrpSiz, int *Remap)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
+   Kernel *k = &g->krn[idx];
+   *GrpSiz = k->tuned ? k->grp : 0;
+   *Remap = k->tuned ? k->remap : 0;
+   return 1;
+}
+
+int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
+   if(GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
+   for(int i : {idx, g->krn[idx].HghIdx})
+   {
+      if(!i) continue;
+      Kernel *k = &g->krn[i];
+      k->grp = GrpSiz;
+      k->remap = Remap ? 1 : 0;
+      k->tuned = 1;
+      k->tune_ev.clear();
+   }
+   return 1;
+}
+
+} // extern "C"

@@ -1,0 +1,156 @@
+// Internal structures of the MI355X-native GMlib engine (libgml.so).
+//
+// One GmlCtx per GmlInit handle.  Every data item lives in HBM for the life
+// of the handle; the host mirror is allocated on first line/block access.
+// All device work of a handle is issued on its own HIP stream in order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../include/gmlib3.h"
+#include "../../include/gmlx.h"
+
+namespace gml {
+
+// Data kinds (the reference's data_type enum, gmlib3.c:71-72)
+enum DatKind { DkArg = 0, DkRaw, DkLnk, DkEle, DkRef, DkMat, DkVec, DkBal, DkOff };
+
+// Tables whose values are part of the interface (gmlib3.c:170-365):
+// sizes of the OpenCL types, element node counts, link widths.
+extern const int OclTypSiz[GmlMaxOclTyp];
+extern const int TypVecSiz[GmlMaxOclTyp];
+extern const char *OclTypStr[GmlMaxOclTyp];
+extern const char *OclNulVec[GmlMaxOclTyp];
+extern const int MshItmTyp[GmlMaxEleTyp];
+extern const int EleNmbNod[GmlMaxEleTyp];
+extern const int MshTypDim[GmlMaxEleTyp];
+extern const char *BalTypStr[GmlMaxEleTyp];
+extern const char *MshTypStr[GmlMaxEleTyp];
+extern const char *MshRefStr[GmlMaxEleTyp];
+extern const int LenMatBas[GmlMaxEleTyp][GmlMaxEleTyp];
+extern const int LenMatMax[GmlMaxEleTyp][GmlMaxEleTyp];
+extern const int NmbTpoLnk[GmlMaxEleTyp][GmlMaxEleTyp];
+extern const int ItmEdgNod[8][12][2];
+extern const int ItmFacDeg[8][6];
+extern const int ItmFacNod[8][6][4];
+
+struct Data {
+   bool     use = false;
+   int      kind = 0, MshTyp = 0, LnkTyp = 0, ItmTyp = 0;
+   int      NmbItm = 0, ItmSiz = 0, ItmLen = 0;
+   long long NmbLin = 0;
+   size_t   LinSiz = 0, MemSiz = 0;
+   void    *dev = nullptr;
+   void    *host = nullptr;
+   std::string nam, VoyNam;
+   const char *src = nullptr;        // parameter struct source (caller-owned, kept by pointer)
+   int      RedIdx = 0;
+   int      OffIdx = 0;              // DkBal: slot of the chunk offset table
+   bool     host_valid = false;      // mirror holds the device contents
+};
+
+// Vertex/entity -> element uplink in SELL-64 layout: rows are grouped in
+// chunks of 64 consecutive source entities; chunk c stores its entries
+// slab-major at ent[(off[c] + k) * 64 + lane], k < width of the chunk.
+struct Uplink {
+   int   DegIdx = 0, BalIdx = 0, OffIdx = 0;
+   int   BasWid = 0, HghWid = 0, MaxPos = 0, MaxDeg = 0;
+   long long nslab = 0;
+};
+
+struct LaunchRec { hipEvent_t beg, end; };
+
+struct Kernel {
+   bool          use = false;
+   std::string   name, src;
+   hipModule_t   mod = nullptr;
+   hipFunction_t fn = nullptr;
+   std::vector<int> args;            // data slot per pointer argument (DkBal expands to 2)
+   std::vector<int> wargs;           // slots the kernel writes
+   long long     NmbLin[2] = {0, 0}; // {count, offset}
+   int           HghIdx = 0;
+   bool          has_par = true;
+   // launch shape autotuning (group size x XCD remap)
+   int           grp = 256, remap = 1, tuned = 0, tune_step = 0;
+   std::vector<LaunchRec> tune_ev;
+   // profiling
+   std::vector<LaunchRec> evts;
+   double        run_time = 0.;
+};
+
+struct Matrix;
+struct Vector;
+
+struct GmlCtx {
+   int          dev = 0;
+   hipStream_t  stream = nullptr;
+   int          NmbKrn = 0, ParIdx = 0, DbgFlg = 0, DblExt = 1;
+   int          TypIdx[GmlMaxEleTyp] = {0};
+   int          RefIdx[GmlMaxEleTyp] = {0};
+   long long    NmbEle[GmlMaxEleTyp] = {0};
+   int          LnkMat[GmlMaxEleTyp][GmlMaxEleTyp] = {{0}};
+   Uplink       up[GmlMaxEleTyp][GmlMaxEleTyp];
+   int          RedKrn[GmlMaxRed] = {0};
+   const char  *UsrTlk = nullptr;
+   std::string  cflags;
+   size_t       MemSiz = 0, MovSiz = 0;
+   double       MemAcc = 0., FltOpp = 0.;
+   Data         dat[GmlMaxDat + 1];
+   Kernel       krn[GmlMaxKrn + 1];
+   Matrix      *mat[GmlMaxMat + 1] = {nullptr};
+   Vector      *vec[GmlMaxVec + 1] = {nullptr};
+   // reduction scratch (device): partials, ticket, result
+   void        *red_scratch = nullptr;
+   double      *red_host = nullptr;  // pinned result slot
+   std::vector<LaunchRec> red_evts[GmlMaxRed];
+   double       red_time[GmlMaxRed] = {0.};
+   void        *dummy_par = nullptr; // parameter buffer when none was declared
+   // device budget (0 = unlimited): used by the serving admission path
+   size_t       budget = 0;
+   // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
+   // generated and compiled but never launched
+   bool         host_only = false;
+};
+
+// --- helpers ---------------------------------------------------------------
+#define GML_CTX(g, h) gml::GmlCtx *g = reinterpret_cast<gml::GmlCtx *>(h)
+#define HIPCHK(x) do { hipError_t e_ = (x); if(e_ != hipSuccess) { \
+   fprintf(stderr, "gml: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), __FILE__, __LINE__, #x); } } while(0)
+
+int  NewDatIdx(GmlCtx *g);
+bool AllocData(GmlCtx *g, Data *d, bool zero = true);
+void FreeDataSlot(GmlCtx *g, int idx);
+bool EnsureHost(Data *d);
+size_t Upload(GmlCtx *g, int idx);
+size_t Download(GmlCtx *g, int idx);
+int  GetCntVec(int siz, int *cnt, int *vec, int *typ);
+void *DevAlloc(GmlCtx *g, size_t bytes);
+void DevFree(GmlCtx *g, void *p);
+bool DevCopy(GmlCtx *g, void *dst, const void *src, size_t bytes, int dir);  // dir 1: H->D, 2: D->H, 3: D->D
+void DbgPrint(GmlCtx *g, const char *fmt, ...);
+
+// topology (gml_topology.cpp / gml_topology_kernels.hip)
+int  BuildUplink(GmlCtx *g, int SrcTyp, int DstTyp);
+int  BuildDownlink(GmlCtx *g, int SrcTyp, int DstTyp);
+int  gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long long nele,
+                     long long nver, int *deg, unsigned int **off_out, int **ent_out,
+                     long long *nslab_out, int *maxdeg_out, int *firsthigh_out, int baswid);
+
+// reductions (gml_reduce.hip)
+int  ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed);
+size_t ReduceScratchBytes();
+
+// block algebra (gml_algebra.hip)
+void FreeAlgebra(GmlCtx *g);
+
+// launcher (gml_codegen.cpp)
+int  RunKernel(GmlCtx *g, Kernel *k);
+void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
+
+} // namespace gml

@@ -1,0 +1,222 @@
+// GmlReduceVector: single-pass wave64 reductions over a float vector.
+//
+// Operation meanings follow the reference (sources/gmlib3.c:3229-3332,
+// sources/reduce.cl:6-137): min, max, sum, L0 = number of nonzero entries,
+// L1 = sum |x|, L2 = sum x^2 (no square root), Linf = max |x|.  Unlike the
+// reference, which leaves one partial per work-group for a float loop on the
+// host and drops the last partial group, the whole vector is reduced on the
+// GPU: each thread streams float4 loads, min/max stay exact in FP32 and the
+// sums accumulate in FP64; waves combine with cross-lane shuffles, work-groups
+// through LDS, and the last work-group to finish (atomic ticket) combines the
+// per-group partials and re-arms the ticket for the next call.  Only the
+// final double crosses PCIe.
+
+#include "gml_internal.h"
+
+namespace gml {
+
+static const int kRedBlock = 256;
+static const int kRedMaxGrid = 2048;
+
+struct RedScratch {
+   double       part[kRedMaxGrid];
+   double       result;
+   unsigned int ticket;
+};
+
+size_t ReduceScratchBytes() { return sizeof(RedScratch) + 256; }
+
+template<int OP> __device__ __forceinline__ double red_init()
+{
+   return OP == GmlMin ? 3.402823466e+38 : (OP == GmlMax ? -3.402823466e+38 : 0.);
+}
+
+template<int OP> __device__ __forceinline__ double red_comb(double a, double b)
+{
+   if(OP == GmlMin) return fmin(a, b);
+   if(OP == GmlMax || OP == GmlLinf) return fmax(a, b);
+   return a + b;
+}
+
+template<int OP> __device__ __forceinline__ double red_elem(float x)
+{
+   if(OP == GmlL0) return x != 0.f ? 1. : 0.;
+   if(OP == GmlL1) return fabs((double)x);
+   if(OP == GmlL2) return (double)x * (double)x;
+   if(OP == GmlLinf) return (double)fabsf(x);
+   return (double)x;
+}
+
+template<int OP> __device__ __forceinline__ double wave_reduce(double v)
+{
+   for(int o = 32; o > 0; o >>= 1)
+      v = red_comb<OP>(v, __shfl_xor(v, o, 64));
+   return v;
+}
+
+template<int OP> __device__ __forceinline__ double block_reduce(double v, double *lds)
+{
+   v = wave_reduce<OP>(v);
+   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+   if(lane == 0) lds[wid] = v;
+   __syncthreads();
+   v = threadIdx.x < (unsigned)nw ? lds[threadIdx.x] : red_init<OP>();
+   if(wid == 0) v = wave_reduce<OP>(v);
+   return v;
+}
+
+// min/max accumulate in float registers (exact); sums in double
+template<int OP>
+__global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ x, long long n, RedScratch *__restrict__ s)
+{
+   __shared__ double lds[kRedBlock / 64];
+   __shared__ bool last;
+   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   const long long nth = (long long)gridDim.x * blockDim.x;
+   double acc = red_init<OP>();
+   const long long n4 = n >> 2;
+   const float4 *x4 = reinterpret_cast<const float4 *>(x);
+
+   if(OP == GmlMin || OP == GmlMax || OP == GmlLinf)
+   {
+      float a = (float)red_init<OP>();
+      for(long long i = tid; i < n4; i += nth)
+      {
+         float4 v = x4[i];
+         if(OP == GmlMin)      a = fminf(a, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+         else if(OP == GmlMax) a = fmaxf(a, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+         else                  a = fmaxf(a, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+      }
+      for(long long i = n4 * 4 + tid; i < n; i += nth)
+      {
+         float v = x[i];
+         if(OP == GmlMin)      a = fminf(a, v);
+         else if(OP == GmlMax) a = fmaxf(a, v);
+         else                  a = fmaxf(a, fabsf(v));
+      }
+      acc = (double)a;
+   }
+   else
+   {
+      double a0 = 0., a1 = 0.;
+      for(long long i = tid; i < n4; i += nth)
+      {
+         float4 v = x4[i];
+         a0 += red_elem<OP>(v.x) + red_elem<OP>(v.y);
+         a1 += red_elem<OP>(v.z) + red_elem<OP>(v.w);
+      }
+      for(long long i = n4 * 4 + tid; i < n; i += nth)
+         a0 += red_elem<OP>(x[i]);
+      acc = a0 + a1;
+   }
+
+   acc = block_reduce<OP>(acc, lds);
+   if(threadIdx.x == 0)
+   {
+      s->part[blockIdx.x] = acc;
+      __threadfence();
+      unsigned int t = atomicAdd(&s->ticket, 1u);
+      last = (t == gridDim.x - 1);
+   }
+   __syncthreads();
+   if(!last) return;
+
+   // last work-group: combine the partials, publish, re-arm the ticket
+   __threadfence();
+   double v = red_init<OP>();
+   for(unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+      v = red_comb<OP>(v, ((volatile double *)s->part)[i]);
+   __syncthreads();
+   v = block_reduce<OP>(v, lds);
+   if(threadIdx.x == 0)
+   {
+      s->result = v;
+      atomicExch(&s->ticket, 0u);
+   }
+}
+
+static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s)
+{
+   long long want = (n / 4 + kRedBlock * 4 - 1) / (kRedBlock * 4);
+   int grid = (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
+   switch(op)
+   {
+      case GmlMin:  hipLaunchKernelGGL(k_reduce<GmlMin>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlMax:  hipLaunchKernelGGL(k_reduce<GmlMax>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlSum:  hipLaunchKernelGGL(k_reduce<GmlSum>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlL0:   hipLaunchKernelGGL(k_reduce<GmlL0>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlL1:   hipLaunchKernelGGL(k_reduce<GmlL1>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlL2:   hipLaunchKernelGGL(k_reduce<GmlL2>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlLinf: hipLaunchKernelGGL(k_reduce<GmlLinf>, dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      default: return -3;
+   }
+   return hipGetLastError() == hipSuccess ? 1 : -6;
+}
+
+// Reduce n floats at device address x; *res receives the result.
+int ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed)
+{
+   if(g->host_only) return -6;
+   RedScratch *s = (RedScratch *)g->red_scratch;
+   LaunchRec r;
+   bool ev = timed && hipEventCreate(&r.beg) == hipSuccess && hipEventCreate(&r.end) == hipSuccess;
+   if(ev) (void)hipEventRecord(r.beg, g->stream);
+   int rc = ReduceLaunch(g->stream, x, n, op, s);
+   if(rc != 1) return rc;
+   if(ev)
+   {
+      (void)hipEventRecord(r.end, g->stream);
+      g->red_evts[op].push_back(r);
+   }
+   if(hipMemcpyAsync(g->red_host, &s->result, sizeof(double), hipMemcpyDeviceToHost, g->stream) != hipSuccess
+   || hipStreamSynchronize(g->stream) != hipSuccess)
+      return -6;
+   *res = *g->red_host;
+   if(g->red_evts[op].size() > 256) HarvestEvents(g->red_evts[op], &g->red_time[op]);
+   return 1;
+}
+
+} // namespace gml
+
+using namespace gml;
+
+extern "C" {
+
+int GmlReduceVector(size_t GmlIdx, int DatIdx, int RedOpp, double *nrm)
+{
+   GML_CTX(g, GmlIdx);
+   if(DatIdx < 1 || DatIdx > GmlMaxDat || !g->dat[DatIdx].use)
+   {
+      printf("Invalid data index: %d\n", DatIdx);
+      return -1;
+   }
+   Data *d = &g->dat[DatIdx];
+   if(d->ItmTyp != GmlFlt || d->NmbItm != 1 || d->ItmLen != 1)
+   {
+      printf("Invalid data structure: count %d, type %s, length %d\n", d->NmbItm, OclTypStr[d->ItmTyp], d->ItmLen);
+      return -2;
+   }
+   if(RedOpp < 0 || RedOpp >= GmlMaxRed)
+   {
+      printf("Invalid operation code %d\n", RedOpp);
+      return -3;
+   }
+   if(!nrm) return -4;
+   return ReduceDevice(g, (const float *)d->dev, d->NmbLin, RedOpp, nrm, true);
+}
+
+double GmlGetReduceRunTime(size_t GmlIdx, int RedOpp)
+{
+   GML_CTX(g, GmlIdx);
+   if(RedOpp < 0 || RedOpp >= GmlMaxRed)
+   {
+      printf("Invalid operation code %d\n", RedOpp);
+      return -1.;
+   }
+   if(g->host_only) return -1.;
+   HIPCHK(hipStreamSynchronize(g->stream));
+   HarvestEvents(g->red_evts[RedOpp], &g->red_time[RedOpp]);
+   return g->red_time[RedOpp];
+}
+
+} // extern "C"

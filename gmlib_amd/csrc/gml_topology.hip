@@ -1,0 +1,510 @@
+// Link builders of libgml.
+//
+// Uplinks (ball of a vertex, shell of an edge or face) follow the ordering
+// contract of the reference's hash builder (sources/gmlib3.c:743-1168,
+// AddHsh/GetHsh 1645-1721): the entries of a row are the elements containing
+// the source entity in DESCENDING element index, encoded (element << 4) |
+// local index of the entity in that element.  Rows up to LenMatBas wide are
+// "base" rows; if some row is wider, rows from the first such entity on are
+// "high" rows and get a second kernel (gmlib3.c:911-964, 2530-2577), sized
+// here from the true maximum degree.
+//
+// Storage is SELL-64: chunk c of 64 consecutive source entities keeps its
+// entries slab-major, ent[(off[c] + k) * 64 + lane], k < width(c) = the
+// chunk's largest degree, so a wave reads slot k of 64 rows as one 256-byte
+// coalesced load.  Vertex balls are built on the GPU (degree count, chunk
+// widths, offset scan, atomic fill, per-row descending sort); edge and face
+// shells and downlinks are built on the host.
+
+#include "gml_internal.h"
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+
+namespace gml {
+
+// ---------------------------------------------------------------------------
+// GPU vertex-ball builder
+// ---------------------------------------------------------------------------
+__global__ void k_ball_count(const int *__restrict__ ele, int stride, int nnod, long long nele,
+                             long long nver, int *__restrict__ deg, int *__restrict__ bad)
+{
+   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(e >= nele) return;
+   for(int j = 0; j < nnod; j++)
+   {
+      int v = ele[e * stride + j];
+      if(v < 0 || v >= nver) { atomicOr(bad, 1); continue; }
+      atomicAdd(&deg[v], 1);
+   }
+}
+
+// width of each 64-row chunk = its largest degree; global max degree and the
+// first row wider than the base width
+__global__ void k_ball_chunks(const int *__restrict__ deg, long long nver, unsigned int *__restrict__ w,
+                              int *__restrict__ stats, int baswid)
+{
+   long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   int d = v < nver ? deg[v] : 0;
+   int m = d;
+   for(int o = 32; o > 0; o >>= 1)
+      m = max(m, __shfl_xor(m, o, 64));
+   if((threadIdx.x & 63) == 0 && v < nver + 63)
+   {
+      long long c = v >> 6;
+      if(c * 64 < nver) w[c] = (unsigned int)m;
+      atomicMax(&stats[0], m);
+   }
+   if(v < nver && d > baswid)
+      atomicMin(&stats[1], (int)v);
+}
+
+// exclusive scan of the chunk widths (one block; nchunk is at most a few
+// million, each thread scans a contiguous run)
+__global__ void k_ball_scan(const unsigned int *__restrict__ w, long long nchunk, unsigned int *__restrict__ off)
+{
+   __shared__ unsigned long long part[1024];
+   const int t = threadIdx.x, nt = blockDim.x;
+   long long per = (nchunk + nt - 1) / nt;
+   long long b = per * t, e = min(b + per, nchunk);
+   unsigned long long s = 0;
+   for(long long i = b; i < e; i++) s += w[i];
+   part[t] = s;
+   __syncthreads();
+   if(t == 0)
+   {
+      unsigned long long acc = 0;
+      for(int i = 0; i < nt; i++) { unsigned long long x = part[i]; part[i] = acc; acc += x; }
+      off[nchunk] = (unsigned int)acc;
+   }
+   __syncthreads();
+   unsigned long long acc = part[t];
+   for(long long i = b; i < e; i++) { off[i] = (unsigned int)acc; acc += w[i]; }
+}
+
+__global__ void k_ball_fill(const int *__restrict__ ele, int stride, int nnod, long long nele, long long nver,
+                            const unsigned int *__restrict__ off, int *__restrict__ fill, int *__restrict__ ent)
+{
+   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(e >= nele) return;
+   for(int j = 0; j < nnod; j++)
+   {
+      int v = ele[e * stride + j];
+      if(v < 0 || v >= nver) continue;
+      int p = atomicAdd(&fill[v], 1);
+      ent[((size_t)off[v >> 6] + p) * 64 + (v & 63)] = (int)((e << 4) | j);
+   }
+}
+
+// descending sort of each row (the reference's chain order)
+__global__ void k_ball_sort(const int *__restrict__ deg, long long nver, const unsigned int *__restrict__ off,
+                            int *__restrict__ ent)
+{
+   long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(v >= nver) return;
+   int d = deg[v];
+   int *row = ent + (size_t)off[v >> 6] * 64 + (v & 63);
+   if(d <= 64)
+   {
+      int r[64];
+      for(int k = 0; k < d; k++) r[k] = row[(size_t)k * 64];
+      for(int k = 1; k < d; k++)
+      {
+         int x = r[k], i = k - 1;
+         while(i >= 0 && r[i] < x) { r[i + 1] = r[i]; i--; }
+         r[i + 1] = x;
+      }
+      for(int k = 0; k < d; k++) row[(size_t)k * 64] = r[k];
+   }
+   else
+   {
+      for(int k = 1; k < d; k++)
+      {
+         int x = row[(size_t)k * 64], i = k - 1;
+         while(i >= 0 && row[(size_t)i * 64] < x) { row[(size_t)(i + 1) * 64] = row[(size_t)i * 64]; i--; }
+         row[(size_t)(i + 1) * 64] = x;
+      }
+   }
+}
+
+// Returns 1 on success.  deg must hold nver ints (zeroed here); *off_out and
+// *ent_out are new device allocations owned by the caller.
+int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long long nele,
+                    long long nver, int *deg, unsigned int **off_out, int **ent_out,
+                    long long *nslab_out, int *maxdeg_out, int *firsthigh_out, int baswid)
+{
+   long long nchunk = (nver + 63) / 64;
+   unsigned int *w = nullptr, *off = nullptr;
+   int *stats = nullptr, *fill = nullptr, *ent = nullptr;
+   int hs[3] = {0, 0, 0};
+   const int B = 256;
+   unsigned int nbe = (unsigned int)((nele + B - 1) / B), nbv = (unsigned int)((nver + 63 + B - 1) / B);
+   *off_out = nullptr; *ent_out = nullptr;
+
+   if(hipMalloc(&w, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
+   || hipMalloc(&off, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
+   || hipMalloc(&stats, 4 * sizeof(int)) != hipSuccess)
+      goto fail;
+   HIPCHK(hipMemsetAsync(deg, 0, nver * sizeof(int), s));
+   HIPCHK(hipMemsetAsync(w, 0, (nchunk + 1) * sizeof(unsigned int), s));
+   {
+      int init[4] = {0, (int)0x7fffffff, 0, 0};
+      HIPCHK(hipMemcpyAsync(stats, init, sizeof(init), hipMemcpyHostToDevice, s));
+   }
+   if(nele > 0)
+      hipLaunchKernelGGL(k_ball_count, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, deg, stats + 2);
+   hipLaunchKernelGGL(k_ball_chunks, dim3(nbv), dim3(B), 0, s, deg, nver, w, stats, baswid);
+   hipLaunchKernelGGL(k_ball_scan, dim3(1), dim3(1024), 0, s, w, nchunk, off);
+   {
+      unsigned int nslab = 0;
+      HIPCHK(hipMemcpyAsync(&nslab, off + nchunk, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(hs, stats, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+      if(hipStreamSynchronize(s) != hipSuccess) goto fail;
+      if(hs[2])
+      {
+         fprintf(stderr, "gml: element table references vertices out of range\n");
+         goto fail;
+      }
+      *nslab_out = nslab;
+      if(hipMalloc(&ent, ((size_t)nslab * 64 + 64) * sizeof(int)) != hipSuccess
+      || hipMalloc(&fill, nver * sizeof(int) + 4) != hipSuccess)
+         goto fail;
+      HIPCHK(hipMemsetAsync(ent, 0, ((size_t)nslab * 64 + 64) * sizeof(int), s));
+      HIPCHK(hipMemsetAsync(fill, 0, nver * sizeof(int) + 4, s));
+   }
+   if(nele > 0)
+      hipLaunchKernelGGL(k_ball_fill, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, off, fill, ent);
+   hipLaunchKernelGGL(k_ball_sort, dim3((unsigned int)((nver + B - 1) / B)), dim3(B), 0, s, deg, nver, off, ent);
+   if(hipStreamSynchronize(s) != hipSuccess) goto fail;
+   (void)hipFree(w);
+   (void)hipFree(stats);
+   (void)hipFree(fill);
+   *maxdeg_out = hs[0];
+   *firsthigh_out = hs[1] == 0x7fffffff ? -1 : hs[1];
+   *off_out = off;
+   *ent_out = ent;
+   return 1;
+fail:
+   (void)hipGetLastError();
+   (void)hipFree(w); (void)hipFree(off); (void)hipFree(stats); (void)hipFree(fill); (void)hipFree(ent);
+   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+typedef std::array<int, 4> Key;
+
+// sorted node tuple of local item j (an edge or face) of element e
+static int ItemKey(int EleTyp, const int *nod, int ItmTyp, int j, Key *k)
+{
+   Key t = {{-1, -1, -1, -1}};
+   int n = 0;
+   if(ItmTyp == GmlVertices) { t[0] = nod[j]; n = 1; }
+   else if(ItmTyp == GmlEdges) { t[0] = nod[ItmEdgNod[EleTyp][j][0]]; t[1] = nod[ItmEdgNod[EleTyp][j][1]]; n = 2; }
+   else
+   {
+      n = ItmFacDeg[EleTyp][j];
+      for(int i = 0; i < n; i++) t[i] = nod[ItmFacNod[EleTyp][j][i]];
+   }
+   std::sort(t.begin(), t.begin() + n);
+   *k = t;
+   return n;
+}
+
+// number of local items of kind ItmTyp in an element of kind EleTyp
+static int NmbItems(int EleTyp, int ItmTyp)
+{
+   if(ItmTyp == GmlVertices) return EleNmbNod[EleTyp];
+   if(ItmTyp == GmlEdges) return NmbTpoLnk[EleTyp][GmlEdges];
+   int n = 0, want = ItmTyp == GmlTriangles ? 3 : 4;
+   for(int j = 0; j < 6; j++) if(ItmFacDeg[EleTyp][j]) n = j + 1;
+   (void)want;
+   return n;
+}
+
+static int FaceNodes(int ItmTyp) { return ItmTyp == GmlTriangles ? 3 : 4; }
+
+static bool HostTable(GmlCtx *g, int typ, std::vector<int> *tab, int *stride)
+{
+   int idx = g->TypIdx[typ];
+   if(!idx) return false;
+   Data *d = &g->dat[idx];
+   if(!d->host_valid && !Download(g, idx)) return false;
+   *stride = d->ItmLen;
+   tab->assign((const int *)d->host, (const int *)d->host + (size_t)d->NmbLin * d->ItmLen);
+   return true;
+}
+
+static int NewLinkSlot(GmlCtx *g, int kind, int Src, int Dst, const std::string &nam, size_t bytes)
+{
+   int idx = NewDatIdx(g);
+   if(!idx) return 0;
+   Data *d = &g->dat[idx];
+   d->kind = kind;
+   d->MshTyp = Src;
+   d->LnkTyp = Dst;
+   d->ItmTyp = GmlInt;
+   d->NmbItm = 1; d->ItmSiz = 4; d->ItmLen = 1;
+   d->NmbLin = (long long)(bytes / 4);
+   d->LinSiz = 4;
+   d->MemSiz = bytes;
+   d->nam = nam;
+   return idx;
+}
+
+// Register device SELL tables (already allocated) as data slots of uplink Src->Dst.
+static int RegisterUplink(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int *off, int *ent,
+                          long long nslab, int maxdeg, int firsthigh)
+{
+   Uplink *u = &g->up[Src][Dst];
+   std::string b = std::string(BalTypStr[Src]) + BalTypStr[Dst];
+   long long nsrc = g->NmbEle[Src], nchunk = (nsrc + 63) / 64;
+
+   int di = NewLinkSlot(g, DkLnk, Src, Dst, b + "Deg", (size_t)nsrc * 4);
+   int bi = NewLinkSlot(g, DkBal, Src, Dst, b + "Bal", ((size_t)nslab * 64 + 64) * 4);
+   int oi = NewLinkSlot(g, DkOff, Src, Dst, b + "BalOff", (size_t)(nchunk + 1) * 4);
+   if(!di || !bi || !oi) return 0;
+   g->dat[di].dev = degdev; g->dat[di].NmbLin = nsrc;
+   g->dat[bi].dev = ent; g->dat[bi].VoyNam = b + "Voy"; g->dat[bi].OffIdx = oi;
+   g->dat[oi].dev = off;
+   g->MemSiz += g->dat[di].MemSiz + g->dat[bi].MemSiz + g->dat[oi].MemSiz;
+
+   u->DegIdx = di; u->BalIdx = bi; u->OffIdx = oi;
+   u->BasWid = LenMatBas[Src][Dst];
+   u->MaxDeg = maxdeg;
+   u->nslab = nslab;
+   if(firsthigh >= 0 && maxdeg > u->BasWid)
+   {
+      int w = 1;
+      while(w < maxdeg) w <<= 1;
+      u->HghWid = w;
+      u->MaxPos = firsthigh;
+   }
+   else
+   {
+      u->HghWid = 0;
+      u->MaxPos = (int)nsrc;
+   }
+   DbgPrint(g, "Ball generation: type %s -> %s, max degree %d, base rows %d x %d, high rows %lld x %d, %lld slabs\n",
+            BalTypStr[Src], BalTypStr[Dst], maxdeg, u->MaxPos, u->BasWid,
+            u->HghWid ? nsrc - u->MaxPos : 0LL, u->HghWid, nslab);
+   return 1;
+}
+
+int BuildUplink(GmlCtx *g, int Src, int Dst)
+{
+   if(g->up[Src][Dst].BalIdx) return 1;
+   int DstIdx = g->TypIdx[Dst];
+   long long nsrc = g->NmbEle[Src], ndst = g->NmbEle[Dst];
+   if(!DstIdx || (Src != GmlVertices && !g->TypIdx[Src]))
+   {
+      fprintf(stderr, "gml: uplink %s -> %s needs both mesh tables\n", BalTypStr[Src], BalTypStr[Dst]);
+      return 0;
+   }
+   if(ndst >= (1LL << 27))
+   {
+      fprintf(stderr, "gml: uplink entries encode element << 4 in 32 bits: %lld elements is too many\n", ndst);
+      return 0;
+   }
+   int *degdev = (int *)DevAlloc(g, (size_t)nsrc * 4 + 4);
+   if(!degdev) return 0;
+
+   if(Src == GmlVertices && !g->host_only)
+   {
+      Data *e = &g->dat[DstIdx];
+      unsigned int *off = nullptr;
+      int *ent = nullptr, maxdeg = 0, first = -1;
+      long long nslab = 0;
+      if(!gpu_vertex_ball(g->stream, (const int *)e->dev, e->ItmLen, EleNmbNod[Dst], ndst, nsrc,
+                          degdev, &off, &ent, &nslab, &maxdeg, &first, LenMatBas[Src][Dst]))
+      {
+         DevFree(g, degdev);
+         return 0;
+      }
+      return RegisterUplink(g, Src, Dst, degdev, off, ent, nslab, maxdeg, first);
+   }
+
+   // host builder: rows in descending element order
+   std::vector<int> dtab, stab;
+   int dst = 0, sst = 0;
+   if(!HostTable(g, Dst, &dtab, &dst) || (Src != GmlVertices && !HostTable(g, Src, &stab, &sst)))
+   {
+      DevFree(g, degdev);
+      return 0;
+   }
+   std::vector<int> deg((size_t)nsrc, 0);
+   std::vector<std::vector<int>> rows((size_t)nsrc);
+   if(Src == GmlVertices)
+   {
+      int nn = EleNmbNod[Dst];
+      for(long long e = ndst - 1; e >= 0; e--)
+         for(int j = 0; j < nn; j++)
+         {
+            int v = dtab[(size_t)e * dst + j];
+            if(v >= 0 && v < nsrc) rows[v].push_back((int)((e << 4) | j));
+         }
+   }
+   else
+   {
+      // (key, element, local item) sorted by key then element desc
+      int ni = NmbItems(Dst, Src);
+      struct Ent { Key k; int e, j; };
+      std::vector<Ent> all;
+      all.reserve((size_t)ndst * ni);
+      for(long long e = 0; e < ndst; e++)
+         for(int j = 0; j < ni; j++)
+         {
+            Key k;
+            int n = ItemKey(Dst, &dtab[(size_t)e * dst], Src, j, &k);
+            if(Src >= GmlTriangles && n != FaceNodes(Src)) continue;
+            all.push_back({k, (int)e, j});
+         }
+      std::sort(all.begin(), all.end(), [](const Ent &a, const Ent &b) {
+         return a.k != b.k ? a.k < b.k : a.e > b.e; });
+      for(long long s = 0; s < nsrc; s++)
+      {
+         Key k = {{-1, -1, -1, -1}};
+         int n = Src == GmlEdges ? 2 : FaceNodes(Src);
+         for(int i = 0; i < n; i++) k[i] = stab[(size_t)s * sst + i];
+         std::sort(k.begin(), k.begin() + n);
+         auto lo = std::lower_bound(all.begin(), all.end(), k, [](const Ent &a, const Key &b) { return a.k < b; });
+         for(auto it = lo; it != all.end() && it->k == k; ++it)
+            rows[s].push_back((it->e << 4) | it->j);
+      }
+   }
+   int maxdeg = 0, first = -1, bw = LenMatBas[Src][Dst];
+   for(long long s = 0; s < nsrc; s++)
+   {
+      deg[s] = (int)rows[s].size();
+      maxdeg = std::max(maxdeg, deg[s]);
+      if(first < 0 && deg[s] > bw) first = (int)s;
+   }
+   // pack SELL-64
+   long long nchunk = (nsrc + 63) / 64;
+   std::vector<unsigned int> off((size_t)nchunk + 1, 0);
+   for(long long c = 0; c < nchunk; c++)
+   {
+      int w = 0;
+      for(long long v = c * 64; v < std::min(nsrc, c * 64 + 64); v++) w = std::max(w, deg[v]);
+      off[c + 1] = off[c] + w;
+   }
+   long long nslab = off[nchunk];
+   std::vector<int> ent(((size_t)nslab * 64 + 64), 0);
+   for(long long v = 0; v < nsrc; v++)
+      for(int k = 0; k < deg[v]; k++)
+         ent[((size_t)off[v >> 6] + k) * 64 + (v & 63)] = rows[v][k];
+   unsigned int *doff = (unsigned int *)DevAlloc(g, off.size() * 4);
+   int *dent = (int *)DevAlloc(g, ent.size() * 4);
+   if(!doff || !dent)
+   {
+      DevFree(g, degdev); DevFree(g, doff); DevFree(g, dent);
+      return 0;
+   }
+   if(!DevCopy(g, degdev, deg.data(), (size_t)nsrc * 4, 1)
+   || !DevCopy(g, doff, off.data(), off.size() * 4, 1)
+   || !DevCopy(g, dent, ent.data(), ent.size() * 4, 1))
+      return 0;
+   return RegisterUplink(g, Src, Dst, degdev, doff, dent, nslab, maxdeg, first);
+}
+
+// Downlink Src -> Dst (Dst of lower dimension, not the vertices): for each
+// local item of each Src element, the index of the Dst entity with the same
+// nodes (the highest such index, as the reference's chain head), 0 if none.
+int BuildDownlink(GmlCtx *g, int Src, int Dst)
+{
+   if(g->LnkMat[Src][Dst]) return 1;
+   std::vector<int> stab, dtab;
+   int sst = 0, dst = 0;
+   if(!HostTable(g, Src, &stab, &sst) || !HostTable(g, Dst, &dtab, &dst))
+   {
+      fprintf(stderr, "gml: downlink %s -> %s needs both mesh tables\n", BalTypStr[Src], BalTypStr[Dst]);
+      return 0;
+   }
+   long long ns = g->NmbEle[Src], nd = g->NmbEle[Dst];
+   int nn = Dst == GmlEdges ? 2 : FaceNodes(Dst);
+   std::vector<std::pair<Key, int>> keys;
+   keys.reserve((size_t)nd);
+   for(long long i = 0; i < nd; i++)
+   {
+      Key k = {{-1, -1, -1, -1}};
+      for(int j = 0; j < nn; j++) k[j] = dtab[(size_t)i * dst + j];
+      std::sort(k.begin(), k.begin() + nn);
+      keys.push_back({k, (int)i});
+   }
+   std::sort(keys.begin(), keys.end(), [](const std::pair<Key, int> &a, const std::pair<Key, int> &b) {
+      return a.first != b.first ? a.first < b.first : a.second > b.second; });
+
+   int w = LenMatBas[Src][Dst], cnt, vec, typ;
+   GetCntVec(w, &cnt, &vec, &typ);
+   int rw = cnt * vec, ni = NmbItems(Src, Dst);
+   std::string nam = std::string(BalTypStr[Src]) + BalTypStr[Dst] + "Lnk";
+   int idx = NewDatIdx(g);
+   if(!idx) return 0;
+   Data *d = &g->dat[idx];
+   d->kind = DkLnk; d->MshTyp = Src; d->LnkTyp = Dst;
+   d->ItmTyp = typ; d->NmbItm = cnt; d->ItmSiz = OclTypSiz[typ]; d->ItmLen = vec;
+   d->NmbLin = ns; d->LinSiz = (size_t)rw * 4; d->MemSiz = (size_t)ns * d->LinSiz;
+   d->nam = nam;
+   if(!AllocData(g, d) || !EnsureHost(d)) return 0;
+   int *t = (int *)d->host;
+   for(long long i = 0; i < ns; i++)
+      for(int j = 0; j < ni && j < rw; j++)
+      {
+         Key k;
+         int n = ItemKey(Src, &stab[(size_t)i * sst], Dst, j, &k);
+         if(Dst >= GmlTriangles && n != nn) continue;
+         auto lo = std::lower_bound(keys.begin(), keys.end(), k,
+                                    [](const std::pair<Key, int> &a, const Key &b) { return a.first < b; });
+         if(lo != keys.end() && lo->first == k)
+            t[(size_t)i * rw + j] = lo->second;
+      }
+   Upload(g, idx);
+   g->LnkMat[Src][Dst] = idx;
+   return 1;
+}
+
+} // namespace gml
+
+using namespace gml;
+
+extern "C" {
+
+int GmlxBuildLink(size_t GmlIdx, int Src, int Dst)
+{
+   GML_CTX(g, GmlIdx);
+   if(Src < 0 || Src >= GmlMaxEleTyp || Dst < 0 || Dst >= GmlMaxEleTyp) return 0;
+   if(MshTypDim[Src] < MshTypDim[Dst]) return BuildUplink(g, Src, Dst);
+   if(MshTypDim[Src] > MshTypDim[Dst]) return Dst == GmlVertices ? (g->TypIdx[Src] ? 1 : 0) : BuildDownlink(g, Src, Dst);
+   return 0;
+}
+
+int GmlxGetUplinkTable(size_t GmlIdx, int Src, int Dst, int *deg, int *ball, int width)
+{
+   GML_CTX(g, GmlIdx);
+   if(Src < 0 || Src >= GmlMaxEleTyp || Dst < 0 || Dst >= GmlMaxEleTyp) return 0;
+   Uplink *u = &g->up[Src][Dst];
+   if(!u->BalIdx || width < u->MaxDeg) return 0;
+   long long n = g->NmbEle[Src], nchunk = (n + 63) / 64;
+   std::vector<unsigned int> off((size_t)nchunk + 1);
+   std::vector<int> ent((size_t)u->nslab * 64 + 64);
+   if(!DevCopy(g, deg, g->dat[u->DegIdx].dev, (size_t)n * 4, 2)
+   || !DevCopy(g, off.data(), g->dat[u->OffIdx].dev, off.size() * 4, 2)
+   || !DevCopy(g, ent.data(), g->dat[u->BalIdx].dev, ent.size() * 4, 2))
+      return 0;
+   memset(ball, 0, (size_t)n * width * sizeof(int));
+   for(long long v = 0; v < n; v++)
+      for(int k = 0; k < deg[v]; k++)
+         ball[(size_t)v * width + k] = ent[((size_t)off[v >> 6] + k) * 64 + (v & 63)];
+   return 1;
+}
+
+int GmlSetNeighbours(size_t GmlIdx, int typ)
+{
+   (void)GmlIdx; (void)typ;
+   fprintf(stderr, "gml: GmlSetNeighbours is not provided by this engine (neighbour links are outside its scope)\n");
+   return 0;
+}
+
+} // extern "C"

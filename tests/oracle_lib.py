@@ -1,0 +1,140 @@
+"""ctypes view of oracle/liboracle.so, the CPU restatement used as the checker.
+
+Test infrastructure only: tests, __graft_entry__.smoke() and bench.py's CPU
+baseline load it; the product never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_F = C.c_float
+
+
+class F4(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        L = C.CDLL(LIB)
+        sigs = {
+            "orc_vertex_ball": (_I, [_I, _I, _I, _I, _P, _I, _P, _P]),
+            "orc_get_cnt_vec": (None, [_I, _P, _P]),
+            "orc_evaluate_numbering": (_F, [_I, _I, _P]),
+            "orc_meanratio_f64": (_D, [_P, _P, _P, _P]),
+            "orc_meanratio_f32": (_F, [F4, F4, F4, F4]),
+            "orc_ref_compute_quality_f64": (_D, [_P, _P, _P, _P]),
+            "orc_caltetqal_f64": (_D, [_P, _P, _P, _P]),
+            "orc_caltriqal_f64": (_D, [_P, _P, _P]),
+            "orc_tet_meanratio_f64_all": (None, [_I, _P, _P, _P]),
+            "orc_tet_meanratio_f32_all": (None, [_I, _P, _P, _P]),
+            "orc_ball_min": (None, [_I, _P, _P, _I, _P, _P]),
+            "orc_reduce": (_D, [_I, C.c_longlong, _P]),
+            "orc_smooth_sweep_f32": (None, [_I, _I, _P, _P, _P, _P, _I, _P, _P, _F]),
+            "orc_smooth_sweep_f64": (None, [_I, _I, _P, _P, _P, _P, _I, _P, _P, _D]),
+            "orc_ref_nodesmooth_sweep_f64": (None, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
+            "orc_bsr_spmv_f64": (None, [_I, _I, _P, _P, _P, _P, _P]),
+            "orc_bsr_spmv_f32": (None, [_I, _I, _P, _P, _P, _P, _P]),
+            "orc_num_threads": (_I, []),
+        }
+        for n, (r, a) in sigs.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def p(a):
+    assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def d3(x):
+    return np.ascontiguousarray(x, np.float64)
+
+
+def meanratio64(a, b, c, d):
+    a, b, c, d = (d3(x) for x in (a, b, c, d))
+    return lib().orc_meanratio_f64(p(a), p(b), p(c), p(d))
+
+
+def vertex_ball(tet, nver, width):
+    tet = np.ascontiguousarray(tet, np.int32)
+    deg = np.zeros(nver, np.int32)
+    ball = np.zeros((nver, width), np.int32)
+    md = lib().orc_vertex_ball(nver, len(tet), tet.shape[1], tet.shape[1], p(tet), width, p(deg), p(ball))
+    return deg, ball, md
+
+
+def tet_quality64(tet, crd):
+    q = np.empty(len(tet), np.float64)
+    lib().orc_tet_meanratio_f64_all(len(tet), p(np.ascontiguousarray(tet, np.int32)),
+                                    p(np.ascontiguousarray(crd, np.float32)), p(q))
+    return q
+
+
+def tet_quality32(tet, crd):
+    q = np.empty(len(tet), np.float32)
+    lib().orc_tet_meanratio_f32_all(len(tet), p(np.ascontiguousarray(tet, np.int32)),
+                                    p(np.ascontiguousarray(crd, np.float32)), p(q))
+    return q
+
+
+def ball_min(deg, ball, val):
+    out = np.empty(len(deg), np.float32)
+    lib().orc_ball_min(len(deg), p(deg), p(ball), ball.shape[1], p(np.ascontiguousarray(val, np.float32)), p(out))
+    return out
+
+
+def reduce(op, x):
+    x = np.ascontiguousarray(x, np.float32)
+    return lib().orc_reduce(op, len(x), p(x))
+
+
+def smooth32(tet, ref, deg, ball, crd, nsweep, omega):
+    crd = np.array(crd, np.float32, copy=True)
+    bary = np.empty((len(tet), 4), np.float32)
+    for _ in range(nsweep):
+        lib().orc_smooth_sweep_f32(len(crd), len(tet), p(tet), p(ref), p(deg), p(ball), ball.shape[1],
+                                   p(crd), p(bary), omega)
+    return crd
+
+
+def smooth64(tet, ref, deg, ball, crd, nsweep, omega):
+    c = np.ascontiguousarray(np.asarray(crd, np.float64)[:, :3])
+    bary = np.empty((len(tet), 3), np.float64)
+    for _ in range(nsweep):
+        lib().orc_smooth_sweep_f64(len(c), len(tet), p(tet), p(ref), p(deg), p(ball), ball.shape[1],
+                                   p(c), p(bary), omega)
+    return c
+
+
+def ref_nodesmooth64(tet, deg, ball, crd, nsweep):
+    c = np.ascontiguousarray(np.asarray(crd, np.float64)[:, :3])
+    opt = np.empty((len(tet), 4, 3), np.float64)
+    res = np.empty(len(c), np.float64)
+    for _ in range(nsweep):
+        lib().orc_ref_nodesmooth_sweep_f64(len(c), len(tet), p(tet), p(deg), p(ball), ball.shape[1],
+                                           p(c), p(opt), p(res))
+    return c, opt, res
+
+
+def bsr_spmv64(rowptr, col, val, x, b):
+    y = np.empty(len(rowptr) - 1, dtype=np.float64).repeat(b)
+    lib().orc_bsr_spmv_f64(len(rowptr) - 1, b, p(rowptr), p(col), p(val), p(np.ascontiguousarray(x, np.float64)), p(y))
+    return y

@@ -1,0 +1,224 @@
+"""GmlCompileKernel on host-only handles: argument classification and the
+generated-kernel contract (names, locals, counters, reads, writes of
+sources/gmlib3.c:2502-2938), hipRTC compilation for gfx950, host-built links
+against the oracle, the high-degree twin kernel, and the reference's own
+example bodies compiled unmodified (when the reference tree is present)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import gmlib_amd as G
+from tests import oracle_lib as O
+
+KDIR = os.path.join(os.path.dirname(G.__file__), "kernels")
+REF = "/root/reference/examples"
+
+
+def body(name):
+    return open(os.path.join(KDIR, name + ".cl")).read()
+
+
+def src_of(g, k):
+    return g.L.GmlxGetKernelSource(g.h, k).decode()
+
+
+@pytest.fixture
+def mesh():
+    return G.cube_mesh(4, 0.2, 3, True)
+
+
+def test_downlink_kernel_contract(mesh):
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        k = g.compile(body("tet_meanratio"), "tet_meanratio", G.GmlTetrahedra,
+                      [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        s = src_of(g, k)
+        hdr = s[s.index("// KERNEL HEADER"):]
+        # link argument first, then data in user order
+        assert hdr.index("int4 *TetVerTab") < hdr.index("float4 *VerCrdTab") < hdr.index("float *qalTab")
+        assert "int       TetIdx = cnt + count.s1;" in hdr
+        assert "TetVer = TetVerTab[ cnt + count.s1 ];" in hdr
+        for i in range(4):
+            assert "VerCrd[%d] = VerCrdTab[ TetVer.s%d ];" % (i, i) in hdr
+        assert "qalTab[ cnt + count.s1 ] = qal;" in hdr
+        assert "float4 VerCrd[4];" in hdr
+
+
+def test_uplink_kernel_contract_with_ref_and_params(mesh):
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        g.new_parameters(16, body("smooth_params"))
+        b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+        k = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
+                      [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+        s = src_of(g, k)
+        assert "#define   BaryDegMax 32" in s
+        assert "int16 VerTetBal[2];" in s and "int VerTetDeg;" in s and "int BaryDeg;" in s
+        assert "Bary[23] = (BaryDeg >= 24) ? BaryTab[ gml_VerTetBal[23] ] : BaryNul;" in s
+        assert "VerRef = VerRefTab[ cnt + count.s1 ];" in s
+        assert "VerCrdTab[ cnt + count.s1 ] = VerCrd;" in s
+        assert "VerRefTab[ cnt + count.s1 ] =" not in s
+        assert "float omega;" in s
+        assert g.link_info(G.GmlVertices, G.GmlTetrahedra) == (len(crd), 32, 0, 0)
+
+
+def test_host_uplink_matches_oracle(mesh):
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        g.set_mesh(crd, ref, tet)
+        assert g.L.GmlxBuildLink(g.h, G.GmlVertices, G.GmlTetrahedra)
+        deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, len(crd), 32)
+    d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
+    assert np.array_equal(deg, d0) and np.array_equal(ball, b0)
+
+
+def test_high_degree_twin_kernel():
+    # a fan of 40 tets around vertex 0 exceeds the 32-wide base rows
+    n = 40
+    crd = np.zeros((n + 3, 4), np.float32)
+    ang = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    crd[1:n + 1, 0], crd[1:n + 1, 1] = np.cos(ang), np.sin(ang)
+    crd[n + 1, 2], crd[n + 2, 2] = 1.0, -1.0
+    tet = []
+    for i in range(n):
+        a, b = 1 + i, 1 + (i + 1) % n
+        tet.append([0, a, b, n + 1] if i % 2 == 0 else [0, b, a, n + 2])
+    tet = np.array(tet, np.int32)
+    ref = np.zeros(len(crd), np.int32)
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+        k = g.compile(body("ball_min"), "ball_min", G.GmlVertices, [(q, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+        # vertex 0 is the first with degree > 32: every row from it on is high
+        assert g.link_info(G.GmlVertices, G.GmlTetrahedra) == (0, 32, len(crd), 64)
+        assert "#define   qalDegMax 32" in src_of(g, k)
+        assert "#define   qalDegMax 64" in src_of(g, k + 1)
+        deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, len(crd), 64)
+    d0, b0, md = O.vertex_ball(tet, len(crd), 64)
+    assert md == n and np.array_equal(deg, d0) and np.array_equal(ball, b0)
+
+
+def test_single_precision_literals(mesh):
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        bod = "// 1.5 in a comment\nqal = 0.5 * VerCrd[0].x + 1e-3 + 2. + .25 + 0x1p3 + 3.0f + (float)(1./3.); /* 7.5 */"
+        s = src_of(g, g.compile(bod, "lit", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)]))
+        user = s[s.index("// USER'S KERNEL CODE"):]
+        assert "0.5f * VerCrd[0].x + 1e-3f + 2.f + .25f + 0x1p3 + 3.0f + (float)(1.f/3.f)" in user
+        assert "// 1.5 in a comment" in user and "/* 7.5 */" in user
+
+
+def test_compile_error_is_reported(mesh, capfd):
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        tri = (C.c_int * 6)(vi, G.GmlReadMode, 0, q, G.GmlWriteMode, 0)
+        k = g.L.GmlxCompileKernelv(g.h, b"qal = undefined_thing;", b"bad", G.GmlTetrahedra, 2, tri)
+        assert k == 0
+    assert "undefined_thing" in capfd.readouterr().out
+
+
+def test_lazy_uplink_accessor_and_voyeurs(mesh):
+    # the NodeSmoothing pattern: 4 x float4 per tet read through the ball
+    # with voyeurs; only the selected item is fetched
+    crd, ref, tet = mesh
+    bod = """{
+   float4 s = (float4)(0.f);
+   for(int i = 0; i < OptCrdDegMax; i++)
+   {
+      s += (VerTetVoy[i] == 0) ? OptCrd[i][0] : (float4)(0.f);
+      s += (VerTetVoy[i] == 1) ? OptCrd[i][1] : (float4)(0.f);
+      s += (VerTetVoy[i] == 2) ? OptCrd[i][2] : (float4)(0.f);
+      s += (VerTetVoy[i] == 3) ? OptCrd[i][3] : (float4)(0.f);
+   }
+   res = dot(s, s);
+}"""
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        o = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "OptCrd")
+        r = g.solution(G.GmlVertices, 1, G.GmlFlt, "res")
+        k = g.compile(bod, "gat", G.GmlVertices, [(r, G.GmlWriteMode, 0), (o, G.GmlReadMode | G.GmlVoyeurs, 0)])
+        s = src_of(g, k)
+        assert "char      VerTetVoy[32];" in s
+        assert "VerTetVoy[k] = (char)(v & 7);" in s
+        assert "gml_upn<float4, 4, 32> OptCrd{OptCrdTab, gml_VerTetBal, OptCrdDeg, OptCrdNul};" in s
+
+
+# --- the reference's example bodies, unmodified ------------------------------
+def ref_body(path):
+    return open(os.path.join(REF, path)).read()
+
+
+needs_ref = pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+
+
+@needs_ref
+def test_reference_example_bodies_compile(mesh):
+    crd, ref, tet = mesh
+    tri = np.array([[0, 1, 2, 0], [1, 2, 3, 0]], np.int32)
+    with G.Gml(host_only=True) as g:
+        L, h = g.L, g.h
+        vi, ti = g.set_mesh(crd, ref, tet)
+        g.new_parameters(8, ref_body("MeshQuality/parameters.cl"))
+        qal = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        opt = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "OptCrd")
+        res = g.solution(G.GmlVertices, 1, G.GmlFlt, "res")
+        vdat = g.solution(G.GmlVertices, 1, G.GmlFlt4, "VerDat")
+        tdat = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "TetDat")
+        R, W, RW = G.GmlReadMode, G.GmlWriteMode, G.GmlReadMode | G.GmlWriteMode
+        T, V = G.GmlTetrahedra, G.GmlVertices
+        cases = [
+            ("MeshQuality/compute_quality.cl", "compute_quality", T, [(vi, R, 0), (qal, W, 0)]),
+            ("NodeSmoothing/quality.cl", "quality", T, [(vi, R, 0), (qal, W, 0)]),
+            ("NodeSmoothing/scatter.cl", "scatter", T, [(vi, R, 0), (opt, W, 0)]),
+            ("NodeSmoothing/gather.cl", "gather", V, [(res, W, 0), (opt, R | G.GmlVoyeurs, 0), (vi, RW, 0)]),
+            ("GpuBandwidth/direct_vertices.cl", "direct_vertices", V, [(vi, R, 0), (vdat, W, 0)]),
+            ("GpuBandwidth/direct_tets.cl", "direct_tets", T, [(ti, R, 0), (tdat, W, 0)]),
+            ("GpuBandwidth/indirect_tets.cl", "indirect_tets", T, [(vi, R, 0), (tdat, W, 0)]),
+            ("GpuBandwidth/vertices_ball.cl", "vertices_ball", V, [(vdat, W, 0), (tdat, R, 0)]),
+        ]
+        for path, name, typ, args in cases:
+            k = g.compile(ref_body(path), name, typ, args)
+            assert k > 0, path
+        # TestFeatures: downlink with refs, uplink with voyeurs, FP64 data
+        sol = g.solution(V, 2, G.GmlFlt4, "SolAtVer")
+        mid = g.solution(T, 1, G.GmlFlt4, "TetMid")
+        rv = g.solution(T, 1, G.GmlFlt, "ResVec")
+        f64 = g.solution(T, 1, G.GmlDbl, "F64Dat")
+        assert g.compile(ref_body("TestFeatures/downlink.cl"), "downlink", T,
+                         [(vi, R | G.GmlRefFlag, 0), (sol, R, 0), (rv, W, 0), (mid, W, 0)])
+        assert g.compile(ref_body("TestFeatures/uplink.cl"), "uplink", V, [(sol, W, 0), (mid, R | G.GmlVoyeurs, 0)])
+        assert g.compile(ref_body("TestFeatures/double_precision.cl"), "double_precision", T, [(f64, R, 0)])
+
+
+@needs_ref
+def test_reference_ball_quality_bodies_compile():
+    # BallQuality: triangles, a vertex->triangle uplink gathering the
+    # triangle nodes and their quality
+    crd = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0], [1, 1, 0, 0]], np.float32)
+    tri = np.array([[0, 1, 2, 0], [1, 3, 2, 0]], np.int32)
+    with G.Gml(host_only=True) as g:
+        L, h = g.L, g.h
+        vi = L.GmlNewMeshData(h, G.GmlVertices, 4)
+        ti = L.GmlNewMeshData(h, G.GmlTriangles, 2)
+        r = np.zeros(4, np.int32)
+        assert L.GmlSetDataBlock(h, G.GmlVertices, 0, 3, G.ptr(crd), C.c_void_p(crd.ctypes.data + 48), G.ptr(r), G.ptr(r))
+        assert L.GmlSetDataBlock(h, G.GmlTriangles, 0, 1, G.ptr(tri), C.c_void_p(tri.ctypes.data + 16), G.ptr(r), G.ptr(r))
+        g.new_parameters(8, ref_body("BallQuality/parameters.cl"))
+        tq = g.solution(G.GmlTriangles, 1, G.GmlFlt, "tq")
+        vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+        assert g.compile(ref_body("BallQuality/triqal.cl"), "triqal", G.GmlTriangles, [(vi, G.GmlReadMode, 0), (tq, G.GmlWriteMode, 0)])
+        assert g.compile(ref_body("BallQuality/verqal.cl"), "verqal", G.GmlVertices,
+                         [(ti, G.GmlReadMode, 0), (tq, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+        deg, ball = g.uplink_table(G.GmlVertices, G.GmlTriangles, 4, 8)
+        assert list(deg) == [1, 2, 2, 1]
+        assert list(ball[1, :2]) == [(1 << 4) | 0, (0 << 4) | 1] and list(ball[2, :2]) == [(1 << 4) | 2, (0 << 4) | 2]

@@ -1,0 +1,231 @@
+"""GPU parity of the generated-kernel path on an MI355X, against the oracle.
+
+Tolerances (from the workload definition): topology, ball lists and min/max
+bit-exact; FP32 quality values and sums within 1e-5 relative of the FP64
+oracle; smoothed coordinates within 1e-5 absolute (unit cube) of the FP64
+oracle.  Each test prints its measured error against the FP64 oracle.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import gmlib_amd as G
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+KDIR = os.path.join(os.path.dirname(G.__file__), "kernels")
+
+
+def body(name):
+    return open(os.path.join(KDIR, name + ".cl")).read()
+
+
+def quality_setup(g, crd, ref, tet):
+    vi, ti = g.set_mesh(crd, ref, tet)
+    q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+    k = g.compile(body("tet_meanratio"), "tet_meanratio", G.GmlTetrahedra,
+                  [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+    return vi, ti, q, k
+
+
+@pytest.mark.parametrize("n,jit,seed", [(20, 0.2, 1), (7, 0.3, 2), (1, 0.0, 3)])
+def test_tet_quality_and_reductions(n, jit, seed):
+    crd, ref, tet = G.cube_mesh(n, jit, seed)
+    print("case: quality n=%d jitter=%g seed=%d" % (n, jit, seed), flush=True)
+    with G.Gml() as g:
+        vi, ti, q, k = quality_setup(g, crd, ref, tet)
+        for _ in range(9):          # past the launch-shape tuner
+            g.launch(k)
+        qg = g.download(q, np.float32, (len(tet),))
+        q64 = O.tet_quality64(tet, crd)
+        err = np.abs(qg - q64) / np.maximum(np.abs(q64), 1e-3)
+        print("quality max rel err vs FP64 oracle: %.3e" % err.max())
+        assert err.max() <= 1e-5
+        # reductions: min/max bit-exact vs the same FP32 array, sums vs FP64
+        assert g.reduce(q, G.GmlMin) == float(qg.min())
+        assert g.reduce(q, G.GmlMax) == float(qg.max())
+        assert g.reduce(q, G.GmlLinf) == float(np.abs(qg).max())
+        assert g.reduce(q, G.GmlL0) == float(np.count_nonzero(qg))
+        s = g.reduce(q, G.GmlSum)
+        exact = math.fsum(qg.astype(np.float64))
+        assert abs(s - exact) <= 1e-12 * len(qg)
+        print("sum: gpu %.17g fsum(gpu array) %.17g fp64-oracle %.17g" % (s, exact, math.fsum(q64)))
+        assert abs(s - math.fsum(q64)) <= 1e-5 * abs(math.fsum(q64))
+        assert g.reduce(q, G.GmlL1) == pytest.approx(math.fsum(np.abs(qg.astype(np.float64))), rel=1e-13)
+        assert g.reduce(q, G.GmlL2) == pytest.approx(math.fsum(qg.astype(np.float64) ** 2), rel=1e-13)
+        assert g.kernel_time(k) > 0
+
+
+def test_gpu_vertex_ball_bit_exact():
+    crd, ref, tet = G.cube_mesh(20, 0.2, 1)
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        assert g.L.GmlxBuildLink(g.h, G.GmlVertices, G.GmlTetrahedra)
+        deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, len(crd), 32)
+        assert g.link_info(G.GmlVertices, G.GmlTetrahedra) == (len(crd), 32, 0, 0)
+    d0, b0, md = O.vertex_ball(tet, len(crd), 32)
+    assert md == 24
+    assert np.array_equal(deg, d0)
+    assert np.array_equal(ball, b0)
+
+
+def test_gpu_ball_random_soup_and_high_degree():
+    # random tets over few vertices: degrees above 32 give high rows
+    rng = np.random.default_rng(5)
+    nv = 64
+    tet = np.array([rng.choice(nv, 4, replace=False) for _ in range(700)], np.int32)
+    crd = np.zeros((nv, 4), np.float32)
+    ref = np.zeros(nv, np.int32)
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        assert g.L.GmlxBuildLink(g.h, G.GmlVertices, G.GmlTetrahedra)
+        d0, b0, md = O.vertex_ball(tet, nv, 128)
+        deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, nv, 128)
+        info = g.link_info(G.GmlVertices, G.GmlTetrahedra)
+    assert md > 32 and info[1] == 32 and info[3] >= md
+    first = int(np.flatnonzero(d0 > 32)[0])
+    assert info[0] == first and info[2] == nv - first
+    assert np.array_equal(deg, d0) and np.array_equal(ball, b0)
+
+
+def test_ball_min_generated():
+    crd, ref, tet = G.cube_mesh(20, 0.2, 1)
+    with G.Gml() as g:
+        vi, ti, q, kq = quality_setup(g, crd, ref, tet)
+        vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+        kb = g.compile(body("ball_min"), "ball_min", G.GmlVertices, [(q, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+        g.launch(kq)
+        for _ in range(8):
+            g.launch(kb)
+        qg = g.download(q, np.float32, (len(tet),))
+        vg = g.download(vq, np.float32, (len(crd),))
+        d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
+        assert np.array_equal(vg, O.ball_min(d0, b0, qg))
+        assert g.reduce(vq, G.GmlMin) == float(qg.min())
+
+
+def smoothing_setup(g, crd, ref, tet, omega):
+    vi, ti = g.set_mesh(crd, ref, tet)
+    par = g.new_parameters(16, body("smooth_params"))
+    C.cast(par, C.POINTER(C.c_float))[0] = omega
+    assert g.L.GmlUploadParameters(g.h) == 1
+    b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+    ks = g.compile(body("smooth_scatter"), "smooth_scatter", G.GmlTetrahedra,
+                   [(vi, G.GmlReadMode, 0), (b, G.GmlWriteMode, 0)])
+    kg = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
+                   [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+    return vi, b, ks, kg
+
+
+@pytest.mark.parametrize("n,sweeps", [(20, 10), (9, 3)])
+def test_smoothing_sweeps_generated(n, sweeps):
+    crd, ref, tet = G.cube_mesh(n, 0.2, 7)
+    omega = 0.2
+    with G.Gml() as g:
+        vi, b, ks, kg = smoothing_setup(g, crd, ref, tet, omega)
+        for _ in range(sweeps):
+            g.launch(ks)
+            g.launch(kg)
+        cg, rg = g.vertices(len(crd))
+    d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
+    c32 = O.smooth32(tet, ref, d0, b0, crd, sweeps, omega)
+    c64 = O.smooth64(tet, ref, d0, b0, crd, sweeps, omega)
+    e32 = np.abs(cg[:, :3] - c32[:, :3]).max()
+    e64 = np.abs(cg[:, :3] - c64).max()
+    print("smoothing n=%d sweeps=%d: max |gpu - oracle f32| %.3e, |gpu - oracle f64| %.3e" % (n, sweeps, e32, e64))
+    assert e64 <= 1e-5 and e32 <= 1e-5
+    assert np.array_equal(cg[ref != 0], crd[ref != 0])
+    assert np.array_equal(rg, ref)
+
+
+def test_reference_nodesmooth_pattern_generated():
+    # the NodeSmoothing pattern of the reference (ideal apex per face,
+    # 4 x float4 per tet gathered through the ball with voyeurs, mix 0.2)
+    scatter = """{
+   float4 e[6], c, nrm; float l[6], m;
+   e[0] = VerCrd[0] - VerCrd[1]; e[1] = VerCrd[0] - VerCrd[2]; e[2] = VerCrd[0] - VerCrd[3];
+   e[3] = VerCrd[1] - VerCrd[2]; e[4] = VerCrd[1] - VerCrd[3]; e[5] = VerCrd[2] - VerCrd[3];
+   for(int i = 0; i < 6; i++) l[i] = dot(e[i], e[i]);
+   m = 0.816496 * sqrt(max(max(l[3], l[4]), l[5])); c = (VerCrd[1] + VerCrd[2] + VerCrd[3]) * (float4)(1./3.);
+   OptCrd[0] = c + fast_normalize(cross(e[4], e[5])) * (float4)(m);
+   m = 0.816496 * sqrt(max(max(l[1], l[2]), l[5])); c = (VerCrd[0] + VerCrd[2] + VerCrd[3]) * (float4)(1./3.);
+   OptCrd[1] = c + fast_normalize(cross(e[5], e[2])) * (float4)(m);
+   m = 0.816496 * sqrt(max(max(l[0], l[2]), l[4])); c = (VerCrd[1] + VerCrd[0] + VerCrd[3]) * (float4)(1./3.);
+   OptCrd[2] = c + fast_normalize(cross(e[2], e[4])) * (float4)(m);
+   m = 0.816496 * sqrt(max(max(l[0], l[1]), l[3])); c = (VerCrd[1] + VerCrd[2] + VerCrd[0]) * (float4)(1./3.);
+   OptCrd[3] = c + fast_normalize(cross(e[3], e[1])) * (float4)(m);
+}"""
+    gather = """{
+   float4 c = (float4)(0.);
+   for(int i = 0; i < OptCrdDegMax; i++)
+   {
+      c += (VerTetVoy[i] == 0) ? OptCrd[i][0] : (float4)(0);
+      c += (VerTetVoy[i] == 1) ? OptCrd[i][1] : (float4)(0);
+      c += (VerTetVoy[i] == 2) ? OptCrd[i][2] : (float4)(0);
+      c += (VerTetVoy[i] == 3) ? OptCrd[i][3] : (float4)(0);
+   }
+   c *= (float4)(1./(OptCrdDeg));
+   c = mix(VerCrd, c, (float)0.2);
+   res = dot(VerCrd - c, VerCrd - c);
+   VerCrd = c;
+}"""
+    crd, ref, tet = G.cube_mesh(12, 0.2, 4)
+    sweeps = 3
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        opt = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "OptCrd")
+        res = g.solution(G.GmlVertices, 1, G.GmlFlt, "res")
+        ks = g.compile(scatter, "scatter", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (opt, G.GmlWriteMode, 0)])
+        kg = g.compile(gather, "gather", G.GmlVertices,
+                       [(res, G.GmlWriteMode, 0), (opt, G.GmlReadMode | G.GmlVoyeurs, 0), (vi, G.GmlReadMode | G.GmlWriteMode, 0)])
+        assert "gml_upn<float4, 4, 32>" in g.L.GmlxGetKernelSource(g.h, kg).decode()
+        for _ in range(sweeps):
+            g.launch(ks)
+            g.launch(kg)
+        cg, _ = g.vertices(len(crd))
+        rsum = g.reduce(res, G.GmlSum)
+    d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
+    c64, _, r64 = O.ref_nodesmooth64(tet, d0, b0, crd, sweeps)
+    e = np.abs(cg[:, :3] - c64).max()
+    print("reference NodeSmoothing pattern: max |gpu - fp64 oracle| = %.3e, residual gpu %.6e fp64 %.6e"
+          % (e, rsum, r64.sum()))
+    assert e <= 1e-5
+    assert rsum == pytest.approx(r64.sum(), rel=1e-4)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 1025, 1 << 20, (1 << 22) + 3])
+def test_reductions_sizes(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n).astype(np.float32)
+    x[::7] = 0.
+    with G.Gml() as g:
+        g.L.GmlNewMeshData(g.h, G.GmlVertices, n)
+        d = g.solution(G.GmlVertices, 1, G.GmlFlt, "x")
+        nb = C.c_size_t(0)
+        dp = g.L.GmlxGetDevicePointer(g.h, d, C.byref(nb))
+        g.L.GmlxSync(g.h)          # the zero fill runs on the library's stream
+        G._hip_h2d(dp, x)
+        assert g.reduce(d, G.GmlMin) == float(x.min())
+        assert g.reduce(d, G.GmlMax) == float(x.max())
+        assert g.reduce(d, G.GmlLinf) == float(np.abs(x).max())
+        assert g.reduce(d, G.GmlL0) == float(np.count_nonzero(x))
+        x64 = x.astype(np.float64)
+        for op, ref in ((G.GmlSum, math.fsum(x64)), (G.GmlL1, math.fsum(np.abs(x64))), (G.GmlL2, math.fsum(x64 * x64))):
+            got = g.reduce(d, op)
+            assert abs(got - ref) <= 1e-13 * max(1.0, math.fsum(np.abs(x64)) if op != G.GmlL2 else ref), op
+        # the same vector twice in a row, then another vector (reference Q3)
+        assert g.reduce(d, G.GmlSum) == g.reduce(d, G.GmlSum)
+
+
+def test_set_get_lines_on_device():
+    with G.Gml() as g:
+        vi = g.L.GmlNewMeshData(g.h, G.GmlVertices, 5)
+        for i in range(5):
+            assert G.set_line(g.h, vi, i, C.c_double(i), C.c_double(i + .25), C.c_double(-i), C.c_int(i % 2))
+        crd, ref = g.vertices(5)
+        assert np.array_equal(crd[:, 1], np.arange(5) + .25) and np.array_equal(ref, [0, 1, 0, 1, 0])
+        x = C.c_double()
+        assert G.get_line(g.h, vi, 3, C.byref(x), C.byref(C.c_double()), C.byref(C.c_double())) and x.value == 3.0
