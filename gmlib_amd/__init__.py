@@ -25,6 +25,7 @@ GmlRefFlag, GmlReadMode, GmlWriteMode, GmlVoyeurs, GmlManual = 1, 2, 4, 8, 16
  GmlDbl, GmlDbl2, GmlDbl4, GmlDbl8, GmlDbl16,
  GmlByt, GmlByt2, GmlByt4, GmlByt8, GmlByt16, GmlMaxOclTyp) = range(21)
 GmlMin, GmlMax, GmlSum, GmlL0, GmlL1, GmlL2, GmlLinf, GmlMaxRed = range(8)
+GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather, GmlxBuiltinSweep = range(5)
 
 _S = C.c_size_t
 _I = C.c_int
@@ -88,6 +89,19 @@ _SIGS = {
     "GmlxBuildLink": (_I, [_S, _I, _I]),
     "GmlxGetUplinkTable": (_I, [_S, _I, _I, _P, _P, _I]),
     "GmlxCubeMesh": (_I, [_I, _D, C.c_ulonglong, _I, _P, _P, _P]),
+    "GmlxSetVector": (_I, [_S, _I, _P]),
+    "GmlxGetVector": (_I, [_S, _I, _P]),
+    "GmlxFreeVector": (_I, [_S, _I]),
+    "GmlxFreeMatrix": (_I, [_S, _I]),
+    "GmlxDotVec": (_I, [_S, _I, _I, _DP]),
+    "GmlxSolveCG": (_I, [_S, _I, _I, _I, _D, _I, _I, _DP, _IP]),
+    "GmlxTetQuality": (_I, [_S, _I]),
+    "GmlxBallMin": (_I, [_S, _I, _I]),
+    "GmlxSmoothSweeps": (_I, [_S, _I, _I, _D]),
+    "GmlxSetBuiltinShape": (_I, [_I, _I, _I]),
+    "GmlxGetBuiltinRunTime": (_D, [_S, _I, C.POINTER(C.c_longlong)]),
+    "GmlxSetSmoothMode": (_I, [_I]),
+    "GmlxGetSweepPlanInfo": (_I, [_S, C.POINTER(C.c_longlong)]),
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)

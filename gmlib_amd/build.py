@@ -24,6 +24,8 @@ SOURCES = [
     "gml_topology.hip",
     "gml_reduce.hip",
     "gml_algebra.hip",
+    "gml_mesh_kernels.hip",
+    "gml_plan.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

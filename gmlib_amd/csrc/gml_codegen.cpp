@@ -252,7 +252,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
 
    // --- header
    s += "// KERNEL HEADER\n";
-   s += Fmt("extern \"C\" __global__ void __launch_bounds__(1024) %s(", PrcNam);
+   s += Fmt("extern \"C\" __global__ void __launch_bounds__(256) %s(", PrcNam);
    for(auto &a : args)
       s += "\n   " + ArgDecl(a) + ",";
    s += "\n   GmlParSct *GmlPar,\n   const int2 count,\n   const int gml_remap)\n{\n";
@@ -454,7 +454,7 @@ static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std:
 // ---------------------------------------------------------------------------
 // Launch
 // ---------------------------------------------------------------------------
-static const int kShapes[][2] = {{256, 1}, {128, 1}, {512, 1}, {64, 1}, {1024, 1}, {256, 0}, {256, 1}};
+static const int kShapes[][2] = {{256, 1}, {128, 1}, {64, 1}, {256, 0}, {128, 0}, {64, 0}, {256, 1}};
 static const int kNmbShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
@@ -896,7 +896,7 @@ int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap)
 {
    GML_CTX(g, GmlIdx);
    if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
-   if(GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
+   if(GrpSiz < 64 || GrpSiz > 256 || (GrpSiz & 63)) return 0;
    for(int i : {idx, g->krn[idx].HghIdx})
    {
       if(!i) continue;

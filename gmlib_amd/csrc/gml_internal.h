@@ -87,6 +87,20 @@ struct Kernel {
 struct Matrix;
 struct Vector;
 
+// Tile plan of the fused smoothing sweep (gml_plan.cpp)
+struct SweepPlan {
+   int TV = 0;
+   long long ntile = 0;
+   int max_nv = 0, max_nt = 0;              // largest per-tile counts (LDS sizing)
+   long long sum_nv = 0, sum_nt = 0;        // totals over tiles (halo redundancy)
+   unsigned int *vofs = nullptr;            // [ntile+1] offsets into vlist
+   int *vlist = nullptr;                    // distinct vertices per tile, ascending
+   unsigned int *tofs = nullptr;            // [ntile+1] offsets into tloc
+   unsigned short *tloc = nullptr;          // 4 tile-local vertex indices per distinct tet
+   unsigned short *lidx = nullptr;          // ball slot -> tile-local tet index (SELL-64)
+   size_t bytes = 0;
+};
+
 struct GmlCtx {
    int          dev = 0;
    hipStream_t  stream = nullptr;
@@ -113,6 +127,9 @@ struct GmlCtx {
    void        *dummy_par = nullptr; // parameter buffer when none was declared
    // device budget (0 = unlimited): used by the serving admission path
    size_t       budget = 0;
+   // fused smoothing: tile plan and the second coordinate buffer
+   SweepPlan   *plan = nullptr;
+   void        *xalt = nullptr;
    // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
    // generated and compiled but never launched
    bool         host_only = false;
@@ -148,6 +165,11 @@ size_t ReduceScratchBytes();
 
 // block algebra (gml_algebra.hip)
 void FreeAlgebra(GmlCtx *g);
+
+// built-in mesh kernels (gml_mesh_kernels.hip, gml_plan.cpp)
+void FreeBuiltinStats(GmlCtx *g);
+SweepPlan *NewSweepPlan(GmlCtx *g, int TV);
+void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 
 // launcher (gml_codegen.cpp)
 int  RunKernel(GmlCtx *g, Kernel *k);

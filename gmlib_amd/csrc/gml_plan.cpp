@@ -1,0 +1,136 @@
+// Tile plan of the fused smoothing sweep (host-side construction).
+//
+// Vertices are cut into tiles of TV consecutive (Hilbert-ordered) vertices,
+// one work-group each.  For a tile the plan lists
+//   - its distinct ball tets, ascending (the union of the tiles' ball rows),
+//     each as the 4 tile-local indices of its vertices in node order,
+//   - the distinct vertices of those tets, ascending (global indices),
+//   - for every ball slot of every vertex (SELL-64 layout of the ball), the
+//     tile-local index of the slot's tet (uint16).
+// The kernel stages the tile's vertex coordinates in LDS with coalesced
+// loads of the ascending vertex list, forms every distinct barycentre once in
+// LDS, and sums each vertex's ball from LDS in the ball's own order -- the
+// same additions, in the same order, as the scatter + gather kernels.
+
+#include "gml_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
+namespace gml {
+
+SweepPlan *NewSweepPlan(GmlCtx *g, int TV)
+{
+   Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
+   const long long nv = g->NmbEle[GmlVertices];
+   const int ti = g->TypIdx[GmlTetrahedra];
+   if(!u->BalIdx || !ti || TV < 64 || (TV & 63) || TV > 1024) return nullptr;
+   Data *td = &g->dat[ti];
+   if(!td->host_valid && !Download(g, ti)) return nullptr;
+   const int *tet = (const int *)td->host;
+   const long long nchunk = (nv + 63) / 64;
+   std::vector<int> deg((size_t)nv);
+   std::vector<unsigned int> off((size_t)nchunk + 1);
+   std::vector<int> ent((size_t)u->nslab * 64 + 64);
+   if(!DevCopy(g, deg.data(), g->dat[u->DegIdx].dev, (size_t)nv * 4, 2)
+   || !DevCopy(g, off.data(), g->dat[u->OffIdx].dev, off.size() * 4, 2)
+   || !DevCopy(g, ent.data(), g->dat[u->BalIdx].dev, ent.size() * 4, 2))
+      return nullptr;
+
+   const long long ntile = (nv + TV - 1) / TV;
+   std::vector<std::vector<int>> tv((size_t)ntile);
+   std::vector<std::vector<unsigned short>> tl((size_t)ntile);
+   std::vector<unsigned short> lidx((size_t)u->nslab * 64 + 64, 0);
+   std::atomic<bool> fits(true);
+
+   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   std::vector<std::thread> th;
+   for(unsigned w = 0; w < nth; w++)
+      th.emplace_back([&, w]() {
+         std::vector<int> ts, vs;
+         for(long long t = w; t < ntile; t += nth)
+         {
+            const long long v0 = t * TV, v1 = std::min(nv, v0 + TV);
+            ts.clear();
+            for(long long v = v0; v < v1; v++)
+               for(int k = 0; k < deg[v]; k++)
+                  ts.push_back(ent[((size_t)off[v >> 6] + k) * 64 + (v & 63)] >> 4);
+            std::sort(ts.begin(), ts.end());
+            ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+            vs.clear();
+            for(int e : ts)
+               for(int j = 0; j < 4; j++) vs.push_back(tet[(size_t)e * 4 + j]);
+            std::sort(vs.begin(), vs.end());
+            vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+            if(ts.size() > 65535 || vs.size() > 65535) { fits = false; continue; }
+            auto &L = tl[t];
+            L.resize(ts.size() * 4);
+            for(size_t i = 0; i < ts.size(); i++)
+               for(int j = 0; j < 4; j++)
+                  L[i * 4 + j] = (unsigned short)(std::lower_bound(vs.begin(), vs.end(), tet[(size_t)ts[i] * 4 + j]) - vs.begin());
+            for(long long v = v0; v < v1; v++)
+               for(int k = 0; k < deg[v]; k++)
+               {
+                  size_t s = ((size_t)off[v >> 6] + k) * 64 + (v & 63);
+                  lidx[s] = (unsigned short)(std::lower_bound(ts.begin(), ts.end(), ent[s] >> 4) - ts.begin());
+               }
+            tv[t] = vs;
+         }
+      });
+   for(auto &x : th) x.join();
+   if(!fits) return nullptr;
+
+   SweepPlan *p = new SweepPlan();
+   p->TV = TV;
+   p->ntile = ntile;
+   std::vector<unsigned int> vofs((size_t)ntile + 1, 0), tofs((size_t)ntile + 1, 0);
+   for(long long t = 0; t < ntile; t++)
+   {
+      vofs[t + 1] = vofs[t] + (unsigned int)tv[t].size();
+      tofs[t + 1] = tofs[t] + (unsigned int)(tl[t].size() / 4);
+      p->max_nv = std::max(p->max_nv, (int)tv[t].size());
+      p->max_nt = std::max(p->max_nt, (int)(tl[t].size() / 4));
+   }
+   std::vector<int> vlist(vofs[ntile] + 1);
+   std::vector<unsigned short> tloc((size_t)tofs[ntile] * 4 + 4);
+   for(long long t = 0; t < ntile; t++)
+   {
+      std::copy(tv[t].begin(), tv[t].end(), vlist.begin() + vofs[t]);
+      std::copy(tl[t].begin(), tl[t].end(), tloc.begin() + (size_t)tofs[t] * 4);
+   }
+   p->vofs = (unsigned int *)DevAlloc(g, vofs.size() * 4);
+   p->tofs = (unsigned int *)DevAlloc(g, tofs.size() * 4);
+   p->vlist = (int *)DevAlloc(g, vlist.size() * 4);
+   p->tloc = (unsigned short *)DevAlloc(g, tloc.size() * 2);
+   p->lidx = (unsigned short *)DevAlloc(g, lidx.size() * 2);
+   if(!p->vofs || !p->tofs || !p->vlist || !p->tloc || !p->lidx
+   || !DevCopy(g, p->vofs, vofs.data(), vofs.size() * 4, 1)
+   || !DevCopy(g, p->tofs, tofs.data(), tofs.size() * 4, 1)
+   || !DevCopy(g, p->vlist, vlist.data(), vlist.size() * 4, 1)
+   || !DevCopy(g, p->tloc, tloc.data(), tloc.size() * 2, 1)
+   || !DevCopy(g, p->lidx, lidx.data(), lidx.size() * 2, 1))
+   {
+      FreeSweepPlan(g, p);
+      return nullptr;
+   }
+   p->bytes = vofs.size() * 4 + tofs.size() * 4 + vlist.size() * 4 + tloc.size() * 2 + lidx.size() * 2;
+   p->sum_nv = vofs[ntile];
+   p->sum_nt = tofs[ntile];
+   g->MemSiz += p->bytes;
+   DbgPrint(g, "Sweep plan: %lld tiles of %d vertices, %u distinct tets (%.2f per tet), %u tile vertices, "
+            "max %d tets / %d vertices per tile, %zu MB\n", ntile, TV, tofs[ntile],
+            (double)tofs[ntile] / (double)g->NmbEle[GmlTetrahedra], vofs[ntile], p->max_nt, p->max_nv, p->bytes >> 20);
+   return p;
+}
+
+void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
+{
+   if(!p) return;
+   DevFree(g, p->vofs); DevFree(g, p->tofs); DevFree(g, p->vlist);
+   DevFree(g, p->tloc); DevFree(g, p->lidx);
+   g->MemSiz -= p->bytes;
+   delete p;
+}
+
+} // namespace gml

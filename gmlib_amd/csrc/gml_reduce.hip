@@ -80,7 +80,22 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ 
    if(OP == GmlMin || OP == GmlMax || OP == GmlLinf)
    {
       float a = (float)red_init<OP>();
-      for(long long i = tid; i < n4; i += nth)
+      long long i = tid;
+      // four independent float4 loads in flight per thread
+      for(; i + 3 * nth < n4; i += 4 * nth)
+      {
+         float4 v[4];
+#pragma unroll
+         for(int u = 0; u < 4; u++) v[u] = x4[i + u * nth];
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+         {
+            if(OP == GmlMin)      a = fminf(a, fminf(fminf(v[u].x, v[u].y), fminf(v[u].z, v[u].w)));
+            else if(OP == GmlMax) a = fmaxf(a, fmaxf(fmaxf(v[u].x, v[u].y), fmaxf(v[u].z, v[u].w)));
+            else                  a = fmaxf(a, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
+         }
+      }
+      for(; i < n4; i += nth)
       {
          float4 v = x4[i];
          if(OP == GmlMin)      a = fminf(a, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
@@ -99,7 +114,20 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ 
    else
    {
       double a0 = 0., a1 = 0.;
-      for(long long i = tid; i < n4; i += nth)
+      long long i = tid;
+      for(; i + 3 * nth < n4; i += 4 * nth)
+      {
+         float4 v[4];
+#pragma unroll
+         for(int u = 0; u < 4; u++) v[u] = x4[i + u * nth];
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+         {
+            a0 += red_elem<OP>(v[u].x) + red_elem<OP>(v[u].y);
+            a1 += red_elem<OP>(v[u].z) + red_elem<OP>(v[u].w);
+         }
+      }
+      for(; i < n4; i += nth)
       {
          float4 v = x4[i];
          a0 += red_elem<OP>(v.x) + red_elem<OP>(v.y);

@@ -306,6 +306,10 @@ void GmlStop(size_t GmlIdx)
    }
    for(int r = 0; r < GmlMaxRed; r++)
       for(auto &e : g->red_evts[r]) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+   FreeBuiltinStats(g);
+   FreeSweepPlan(g, g->plan);
+   g->plan = nullptr;
+   if(g->xalt) { DevFree(g, g->xalt); g->xalt = nullptr; }
    FreeAlgebra(g);
    for(int i = 1; i <= GmlMaxDat; i++)
       FreeDataSlot(g, i);

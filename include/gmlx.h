@@ -37,7 +37,8 @@ const char *GmlxGetKernelSource(size_t GmlIdx, int KrnIdx);
    kernel (0 in both while it is still tuning). */
 int    GmlxGetKernelShape(size_t GmlIdx, int KrnIdx, int *GrpSiz, int *Remap);
 
-/* Pin the launch shape of a kernel (disables its tuner). */
+/* Pin the launch shape of a kernel (disables its tuner); GrpSiz is 64, 128
+   or 256 (generated kernels are compiled for at most 256 work-items). */
 int    GmlxSetKernelShape(size_t GmlIdx, int KrnIdx, int GrpSiz, int Remap);
 
 /* --- device residency ---------------------------------------------------- */
@@ -67,6 +68,61 @@ int    GmlxBuildLink(size_t GmlIdx, int SrcTyp, int DstTyp);
    ball[NmbSrc * width] with entries (element << 4) | local index in
    descending element order, 0-padded.  width must be >= the max degree. */
 int    GmlxGetUplinkTable(size_t GmlIdx, int SrcTyp, int DstTyp, int *deg, int *ball, int width);
+
+/* --- block vectors and solver ------------------------------------------- */
+
+/* Copy a vector made by GmlNewVector in or out (NmbLin * BlkSiz packed
+   values of its float type).  No counterpart: the reference offers no way to
+   read a vector back. */
+int    GmlxSetVector(size_t GmlIdx, int VecIdx, const void *val);
+int    GmlxGetVector(size_t GmlIdx, int VecIdx, void *val);
+int    GmlxFreeVector(size_t GmlIdx, int VecIdx);
+int    GmlxFreeMatrix(size_t GmlIdx, int MatIdx);
+
+/* Dot product of two FP32 vectors with FP64 accumulation. */
+int    GmlxDotVec(size_t GmlIdx, int VecA, int VecB, double *res);
+
+/* Block-Jacobi preconditioned conjugate gradient on A x = rhs for an SPD
+   FP32 block matrix (block size 1, 3 or 4).  refine = 0: one FP32 solve to
+   the relative tolerance tol; refine = 1: iterative refinement, x held in
+   FP64, residual rhs - A x accumulated in FP64, FP32 inner solves, until the
+   FP64 relative residual is <= tol.  SolIdx may be an FP32 or FP64 vector.
+   *relres = |rhs - A x| / |rhs| evaluated in FP64, *iters = inner iterations.
+   Returns 1 or a negative error. */
+int    GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, int maxit,
+                   int refine, double *relres, int *iters);
+
+/* --- hand-written mesh kernels --------------------------------------------- */
+
+/* HIP kernels written for gfx950 that compute exactly what the generated
+   kernels of the bodies in gmlib_amd/kernels compute (tet_meanratio.cl,
+   ball_min.cl, smooth_scatter.cl + smooth_gather.cl), on the handle's vertex
+   and tet tables and vertex->tet ball (built on first use).  For callers that
+   run those workloads at scale; launches are asynchronous on the handle's
+   stream. */
+enum gmlx_builtin { GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather,
+                    GmlxBuiltinSweep };
+
+/* mean-ratio quality of every tet into QalIdx (a GmlFlt tet solution) */
+int    GmlxTetQuality(size_t GmlIdx, int QalIdx);
+/* per-vertex minimum of a GmlFlt tet field over the vertex ball */
+int    GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx);
+/* nsweep Jacobi sweeps of barycentre smoothing of the vertex coordinates
+   (vertices with ref != 0 stay fixed).  Fused mode (default): one kernel per
+   sweep over vertex tiles (LDS-staged coordinates and barycentres, tile plan
+   built on first call, coordinates double-buffered: after a sweep the vertex
+   table's device address is the other buffer).  Two-kernel mode: barycentre
+   scatter into BaryIdx (a GmlFlt4 tet solution), then ball gather. */
+int    GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega);
+/* 1: fused sweep (default), 0: scatter + gather kernels */
+int    GmlxSetSmoothMode(int fused);
+/* tile plan of the fused sweep: {tile size, tiles, sum of tile vertices,
+   sum of tile tets, max vertices per tile, max tets per tile, bytes} */
+int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
+/* launch shape of a built-in kernel for every handle of the process */
+int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);
+/* summed device time (s) and launch count of a built-in kernel on a handle */
+double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches);
 
 /* --- synthetic meshes ------------------------------------------------------ */
 

@@ -1,0 +1,74 @@
+"""The hand-written mesh kernels against the generated drop-in kernels of the
+same bodies (bit-exact) and against the oracle."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import gmlib_amd as G
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+KDIR = os.path.join(os.path.dirname(G.__file__), "kernels")
+
+
+def body(name):
+    return open(os.path.join(KDIR, name + ".cl")).read()
+
+
+@pytest.mark.parametrize("n,grp,remap,fused", [(20, 256, 1, 1), (13, 64, 0, 1), (11, 128, 1, 1),
+                                                (20, 256, 1, 0), (5, 1024, 1, 0)])
+def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused):
+    for w in range(4):
+        assert G.lib().GmlxSetBuiltinShape(w, grp, remap)
+    assert G.lib().GmlxSetBuiltinShape(G.GmlxBuiltinSweep, min(grp, 256), remap)
+    assert G.lib().GmlxSetSmoothMode(fused)
+    crd, ref, tet = G.cube_mesh(n, 0.2, 3)
+    omega = 0.2
+    res = {}
+    for path in ("generated", "builtin"):
+        with G.Gml() as g:
+            vi, ti = g.set_mesh(crd, ref, tet)
+            par = g.new_parameters(16, body("smooth_params"))
+            C.cast(par, C.POINTER(C.c_float))[0] = omega
+            g.L.GmlUploadParameters(g.h)
+            q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+            vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+            b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+            if path == "generated":
+                kq = g.compile(body("tet_meanratio"), "q", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+                kb = g.compile(body("ball_min"), "bm", G.GmlVertices, [(q, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+                ks = g.compile(body("smooth_scatter"), "s", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (b, G.GmlWriteMode, 0)])
+                kg = g.compile(body("smooth_gather"), "gt", G.GmlVertices,
+                               [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+                g.launch(kq); g.launch(kb)
+                q0 = g.download(q, np.float32, (len(tet),)); v0 = g.download(vq, np.float32, (len(crd),))
+                for _ in range(5):
+                    g.launch(ks); g.launch(kg)
+            else:
+                assert g.L.GmlxTetQuality(g.h, q) == 1
+                assert g.L.GmlxBallMin(g.h, q, vq) == 1
+                q0 = g.download(q, np.float32, (len(tet),)); v0 = g.download(vq, np.float32, (len(crd),))
+                assert g.L.GmlxSmoothSweeps(g.h, b, 5, omega) == 1
+                nl = C.c_longlong()
+                which = G.GmlxBuiltinSweep if fused else G.GmlxBuiltinGather
+                assert g.L.GmlxGetBuiltinRunTime(g.h, which, C.byref(nl)) > 0 and nl.value == 5
+                if fused:
+                    info = (C.c_longlong * 7)()
+                    assert g.L.GmlxGetSweepPlanInfo(g.h, info) and info[0] == min(grp, 256)
+                    print("plan: tiles %d, tets per tet %.3f, max tets/vertices per tile %d/%d"
+                          % (info[1], info[3] / len(tet), info[5], info[4]))
+            c, r = g.vertices(len(crd))
+            res[path] = (q0, v0, c)
+    for a, b_ in zip(res["generated"], res["builtin"]):
+        assert np.array_equal(a, b_)
+    q64 = O.tet_quality64(tet, crd)
+    assert np.max(np.abs(res["builtin"][0] - q64) / np.maximum(np.abs(q64), 1e-3)) <= 1e-5
+    d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
+    assert np.array_equal(res["builtin"][1], O.ball_min(d0, b0, res["builtin"][0]))
+    c64 = O.smooth64(tet, ref, d0, b0, crd, 5, omega)
+    assert np.abs(res["builtin"][2][:, :3] - c64).max() <= 1e-5
+    for w in range(5):
+        G.lib().GmlxSetBuiltinShape(w, 256, 1)
+    G.lib().GmlxSetSmoothMode(1)
