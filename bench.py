@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--omega", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: run oracle sweeps for about this long (0: skip)")
+    ap.add_argument("--sweep-tile", type=int, default=256, help="vertices per tile of the fused sweep")
+    ap.add_argument("--paths", default="generated,builtin")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="launch a plain float4 copy kernel (16V read + 16V write) this many times first, "
                          "to calibrate HBM byte counters")
@@ -123,8 +125,11 @@ def main():
         return [L.GmlxGetBuiltinRunTime(h, w, C.byref(cnt))
                 for w in (G.GmlxBuiltinSweep, G.GmlxBuiltinQuality)]
 
+    L.GmlxSetBuiltinShape(G.GmlxBuiltinSweep, args.sweep_tile, 1)
     paths = {}
     for path, step in (("generated", step_generated), ("builtin", step_builtin)):
+        if path not in args.paths.split(","):
+            continue
         for _ in range(args.warmup):
             qmin = step()
         sg = ktimes(path)
@@ -140,12 +145,16 @@ def main():
             per = {"scatter": (scat_bytes, ek[0] / nl), "gather": (gath_bytes, ek[1] / nl),
                    "quality": (qual_bytes, ek[2] / args.steps)}
         else:
-            per = {"sweep": (fused_bytes, ek[0] / nl), "quality": (qual_bytes, ek[1] / args.steps)}
-            info = (C.c_longlong * 7)()
-            if L.GmlxGetSweepPlanInfo(h, info):
-                log("sweep plan: tile %d, %d tiles, tile tets / T = %.3f, tile vertices / V = %.3f, "
-                    "max %d tets %d vertices per tile, %.2f GB"
-                    % (info[0], info[1], info[3] / T, info[2] / V, info[5], info[4], info[6] / 1e9))
+            info = (C.c_longlong * 9)()
+            if not L.GmlxGetSweepPlanInfo(h, info):
+                raise RuntimeError("no sweep plan")
+            log("sweep plan: mode %d, tile %d, %d tiles, tile tets / T = %.3f, tile vertices / V = %.3f, "
+                "1-ring slots %d, %.2f GB" % (info[8], info[0], info[1], info[3] / T, info[2] / V, info[7], info[6] / 1e9))
+            # operator form: 1-ring slots (4 B) + deg, ring count, ref (4 B each) + old and new
+            # coordinates (16 B each), every datum once; exact fused form: tet->vertex rows as the
+            # plan's 8 B + ball slots 2 B x 4T + 40 V
+            sweep_alg = 4 * info[7] + 44 * V if info[8] == 2 else fused_bytes
+            per = {"sweep": (sweep_alg, ek[0] / nl), "quality": (qual_bytes, ek[1] / args.steps)}
         kern = {k: {"avg_us": round(v[1] * 1e6, 2), "alg_GBs": round(v[0] / v[1] / 1e9, 1),
                     "frac": round(v[0] / v[1] / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": v[0]}
                 for k, v in per.items()}
@@ -163,6 +172,13 @@ def main():
     P = paths[best]
     dom = max(P["kernels"], key=lambda k: P["kernels"][k]["avg_us"] * (args.sweeps if k != "quality" else 1))
     D = P["kernels"][dom]
+    # HBM traffic per launch of the dominant kernel, as measured by the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs on this workload (null elsewhere)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_256cube.json")
+    if n == 256 and os.path.exists(tf):
+        key = dom if dom != "quality" else ("quality_builtin" if best == "builtin" else "quality_generated")
+        traffic = json.load(open(tf)).get(key)
     ms, gbs = P["ms_per_step"], P["GBs"]
 
     # CPU baseline: the oracle's FP32 sweep on the host cores (bounded sample)
@@ -207,7 +223,7 @@ def main():
                    "vertices": V, "tets": T, "sweeps": args.sweeps, "path": best,
                    "bytes_per_step": step_bytes},
         "roofline": {"bound": "hbm", "achieved": D["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": D["frac"], "traffic": None, "kernel": "%s (%s path)" % (dom, best),
+                     "frac": D["frac"], "traffic": traffic, "kernel": "%s (%s path)" % (dom, best),
                      "bytes_per_launch": D["bytes_per_launch"], "avg_us": D["avg_us"]},
         "paths": paths,
         "cpu_baseline": cpu,

@@ -93,11 +93,17 @@ struct SweepPlan {
    long long ntile = 0;
    int max_nv = 0, max_nt = 0;              // largest per-tile counts (LDS sizing)
    long long sum_nv = 0, sum_nt = 0;        // totals over tiles (halo redundancy)
+   long long sum_ring = 0;                  // operator form: total 1-ring slots (V + 2E)
    unsigned int *vofs = nullptr;            // [ntile+1] offsets into vlist
    int *vlist = nullptr;                    // distinct vertices per tile, ascending
    unsigned int *tofs = nullptr;            // [ntile+1] offsets into tloc
    unsigned short *tloc = nullptr;          // 4 tile-local vertex indices per distinct tet
    unsigned short *lidx = nullptr;          // ball slot -> tile-local tet index (SELL-64)
+   // operator form: per vertex, its 1-ring as (tile-local index << 8 | count
+   // of ball tets containing the neighbour), SELL-64 over the vertices
+   unsigned int *nbr = nullptr, *noff = nullptr;
+   int *ncnt = nullptr;
+   long long nslab = 0;
    size_t bytes = 0;
 };
 
@@ -128,7 +134,8 @@ struct GmlCtx {
    // device budget (0 = unlimited): used by the serving admission path
    size_t       budget = 0;
    // fused smoothing: tile plan and the second coordinate buffer
-   SweepPlan   *plan = nullptr;
+   SweepPlan   *plan = nullptr;             // exact fused form
+   SweepPlan   *oplan = nullptr;            // operator form
    void        *xalt = nullptr;
    // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
    // generated and compiled but never launched
@@ -168,7 +175,7 @@ void FreeAlgebra(GmlCtx *g);
 
 // built-in mesh kernels (gml_mesh_kernels.hip, gml_plan.cpp)
 void FreeBuiltinStats(GmlCtx *g);
-SweepPlan *NewSweepPlan(GmlCtx *g, int TV);
+SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 
 // launcher (gml_codegen.cpp)

@@ -51,6 +51,12 @@ void FreeBuiltinStats(GmlCtx *g)
       }
 }
 
+// clang ext-vector types: arrays of these stay in registers (HIP's float4 is
+// a struct wrapper whose arrays the compiler may place in scratch)
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ unsigned int xcd_block(int remap)
 {
    unsigned int b = blockIdx.x;
@@ -62,78 +68,67 @@ __device__ __forceinline__ unsigned int xcd_block(int remap)
    return b;
 }
 
-__device__ __forceinline__ int4 ld_nt(const int4 *p)
-{
-   int4 v;
-   v.x = __builtin_nontemporal_load(&p->x);
-   v.y = __builtin_nontemporal_load(&p->y);
-   v.z = __builtin_nontemporal_load(&p->z);
-   v.w = __builtin_nontemporal_load(&p->w);
-   return v;
-}
+__device__ __forceinline__ v4i ld_nt(const v4i *p) { return __builtin_nontemporal_load(p); }
 
-__device__ __forceinline__ float dot4(float4 a, float4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+__device__ __forceinline__ float dot4(v4f a, v4f b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 
 // mean ratio: the expression sequence of tet_meanratio.cl
-__device__ __forceinline__ float meanratio(float4 p0, float4 p1, float4 p2, float4 p3)
+__device__ __forceinline__ float meanratio(v4f p0, v4f p1, v4f p2, v4f p3)
 {
-   float4 e0 = p1 - p0, e1 = p2 - p0, e2 = p3 - p0, e3 = p2 - p1, e4 = p3 - p1, e5 = p3 - p2;
+   v4f e0 = p1 - p0, e1 = p2 - p0, e2 = p3 - p0, e3 = p2 - p1, e4 = p3 - p1, e5 = p3 - p2;
    float s = dot4(e0, e0) + dot4(e1, e1) + dot4(e2, e2) + dot4(e3, e3) + dot4(e4, e4) + dot4(e5, e5);
-   float4 c = make_float4(e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x, 0.f);
+   v4f c = {e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x, 0.f};
    float v = dot4(c, e2) * (1.f / 6.f);
    float q = 12.f * cbrtf(9.f * v * v) / s;
    if(v < 0.f) q = -q;
    return q;
 }
 
-__global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const int4 *__restrict__ tet,
-                                                       const float4 *__restrict__ crd, float *__restrict__ q, int remap)
+__global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *__restrict__ tet,
+                                                       const v4f *__restrict__ crd, float *__restrict__ q, int remap)
 {
    long long t = (long long)xcd_block(remap) * blockDim.x + threadIdx.x;
    if(t >= nt) return;
-   int4 v = ld_nt(&tet[t]);
+   v4i v = ld_nt(&tet[t]);
    q[t] = meanratio(crd[v.x], crd[v.y], crd[v.z], crd[v.w]);
 }
 
-__global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const int4 *__restrict__ tet,
-                                                          const float4 *__restrict__ crd, float4 *__restrict__ bary, int remap)
+__global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
+                                                          const v4f *__restrict__ crd, v4f *__restrict__ bary, int remap)
 {
    long long t = (long long)xcd_block(remap) * blockDim.x + threadIdx.x;
    if(t >= nt) return;
-   int4 v = ld_nt(&tet[t]);
-   float4 a = crd[v.x], b = crd[v.y], c = crd[v.z], d = crd[v.w];
+   v4i v = ld_nt(&tet[t]);
+   v4f a = crd[v.x], b = crd[v.y], c = crd[v.z], d = crd[v.w];
    bary[t] = (((a + b) + c) + d) * 0.25f;
 }
 
 // Ball gather: slot k of vertex v is ent[(off[v >> 6] + k) * 64 + (v & 63)].
 __global__ void __launch_bounds__(1024) k_smooth_gather(long long nv, const int *__restrict__ deg,
                                                          const unsigned int *__restrict__ off, const int *__restrict__ ent,
-                                                         const float4 *__restrict__ bary, const int *__restrict__ ref,
-                                                         float4 *__restrict__ crd, float omega, int remap)
+                                                         const v4f *__restrict__ bary, const int *__restrict__ ref,
+                                                         v4f *__restrict__ crd, float omega, int remap)
 {
    long long v = (long long)xcd_block(remap) * blockDim.x + threadIdx.x;
    if(v >= nv) return;
    const int d = deg[v];
    const int *row = ent + (size_t)off[v >> 6] * 64 + (v & 63);
-   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-   int k = 0;
-   for(; k + 8 <= d; k += 8)
+   v4f s = {0.f, 0.f, 0.f, 0.f};
+   for(int k = 0; k < d; k += 8)
    {
       int e[8];
-      float4 b[8];
 #pragma unroll
-      for(int i = 0; i < 8; i++) e[i] = __builtin_nontemporal_load(&row[(k + i) * 64]) >> 4;
+      for(int i = 0; i < 8; i++) e[i] = k + i < d ? __builtin_nontemporal_load(&row[(k + i) * 64]) >> 4 : -1;
+      v4f b[8];
 #pragma unroll
-      for(int i = 0; i < 8; i++) b[i] = bary[e[i]];
+      for(int i = 0; i < 8; i++) b[i] = e[i] >= 0 ? bary[e[i]] : (v4f){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for(int i = 0; i < 8; i++) s += b[i];
+      for(int i = 0; i < 8; i++) if(e[i] >= 0) s += b[i];
    }
-   for(; k < d; k++)
-      s += bary[__builtin_nontemporal_load(&row[k * 64]) >> 4];
    const int r = ref[v];
    if(!r && d > 0)
    {
-      float4 p = crd[v];
+      v4f p = crd[v];
       s *= 1.f / (float)d;
       crd[v] = p + (s - p) * omega;
    }
@@ -159,34 +154,63 @@ __global__ void __launch_bounds__(1024) k_ball_min(long long nv, const int *__re
 // coordinates to the other buffer.
 __global__ void __launch_bounds__(256) k_sweep_fused(long long nv, const unsigned int *__restrict__ vofs,
                                                       const int *__restrict__ vlist, const unsigned int *__restrict__ tofs,
-                                                      const ushort4 *__restrict__ tloc, const int *__restrict__ deg,
+                                                      const v4u16 *__restrict__ tloc, const int *__restrict__ deg,
                                                       const unsigned int *__restrict__ off, const unsigned short *__restrict__ lidx,
-                                                      const int *__restrict__ ref, const float4 *__restrict__ xo,
-                                                      float4 *__restrict__ xn, float omega, int remap)
+                                                      const int *__restrict__ ref, const v4f *__restrict__ xo,
+                                                      v4f *__restrict__ xn, float omega, int remap)
 {
-   extern __shared__ float4 lds[];
+   extern __shared__ v4f lds[];
    const unsigned int tile = xcd_block(remap);
    const int tid = threadIdx.x, TV = blockDim.x;
    const unsigned int v0 = vofs[tile], nvt = vofs[tile + 1] - v0;
    const unsigned int t0 = tofs[tile], ntt = tofs[tile + 1] - t0;
-   float4 *X = lds, *Bc = lds + nvt;
-   for(unsigned int i = tid; i < nvt; i += TV)
-      X[i] = xo[__builtin_nontemporal_load(&vlist[v0 + i])];
-   __syncthreads();
-   for(unsigned int j = tid; j < ntt; j += TV)
+   v4f *X = lds, *Bc = lds + nvt;
+   // 1. tile coordinates: index loads first, then the gathers, 4 in flight
+   for(unsigned int i = tid; i < nvt; i += 4 * TV)
    {
-      ushort4 l = tloc[t0 + j];
-      Bc[j] = (((X[l.x] + X[l.y]) + X[l.z]) + X[l.w]) * 0.25f;
+      int vi[4];
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         vi[u] = i + u * TV < nvt ? __builtin_nontemporal_load(&vlist[v0 + i + u * TV]) : -1;
+      v4f x[4];
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         x[u] = vi[u] >= 0 ? xo[vi[u]] : (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         if(vi[u] >= 0) X[i + u * TV] = x[u];
    }
    __syncthreads();
+   // 2. distinct barycentres, in the scatter kernel's operation order
+   for(unsigned int j = tid; j < ntt; j += 4 * TV)
+   {
+      v4u16 l[4];
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         l[u] = j + u * TV < ntt ? tloc[t0 + j + u * TV] : (v4u16){0, 0, 0, 0};
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         if(j + u * TV < ntt)
+            Bc[j + u * TV] = (((X[l[u].x] + X[l[u].y]) + X[l[u].z]) + X[l[u].w]) * 0.25f;
+   }
+   __syncthreads();
+   // 3. ball sums from LDS in ball order, slot indices loaded 8 at a time
    const long long v = (long long)tile * TV + tid;
    if(v >= nv) return;
    const int d = deg[v];
    const unsigned short *row = lidx + (size_t)off[v >> 6] * 64 + (v & 63);
-   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-   for(int k = 0; k < d; k++)
-      s += Bc[__builtin_nontemporal_load(&row[k * 64])];
-   float4 p = xo[v];
+   v4f s = {0.f, 0.f, 0.f, 0.f};
+   for(int k = 0; k < d; k += 8)
+   {
+      int li[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+         li[u] = k + u < d ? (int)__builtin_nontemporal_load(&row[(k + u) * 64]) : -1;
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+         if(li[u] >= 0) s += Bc[li[u]];
+   }
+   v4f p = xo[v];
    if(!ref[v] && d > 0)
    {
       s *= 1.f / (float)d;
@@ -195,9 +219,62 @@ __global__ void __launch_bounds__(256) k_sweep_fused(long long nv, const unsigne
    xn[v] = p;
 }
 
+// Operator-form Jacobi sweep over one tile: stage the tile's 1-ring
+// coordinates in LDS, then x_v += omega (sum_u c_vu x_u / (4 deg v) - x_v)
+// with the 1-ring read as (tile-local index << 8 | c_vu) slots.
+__global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned int *__restrict__ vofs,
+                                                   const int *__restrict__ vlist, const int *__restrict__ deg,
+                                                   const int *__restrict__ ncnt, const unsigned int *__restrict__ noff,
+                                                   const unsigned int *__restrict__ nbr, const int *__restrict__ ref,
+                                                   const v4f *__restrict__ xo, v4f *__restrict__ xn, float omega, int remap)
+{
+   extern __shared__ v4f lds[];
+   const unsigned int tile = xcd_block(remap);
+   const int tid = threadIdx.x, TV = blockDim.x;
+   const unsigned int v0 = vofs[tile], nvt = vofs[tile + 1] - v0;
+   v4f *X = lds;
+   for(unsigned int i = tid; i < nvt; i += 4 * TV)
+   {
+      int vi[4];
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         vi[u] = i + u * TV < nvt ? __builtin_nontemporal_load(&vlist[v0 + i + u * TV]) : -1;
+      v4f x[4];
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         x[u] = vi[u] >= 0 ? xo[vi[u]] : (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for(int u = 0; u < 4; u++)
+         if(vi[u] >= 0) X[i + u * TV] = x[u];
+   }
+   __syncthreads();
+   const long long v = (long long)tile * TV + tid;
+   if(v >= nv) return;
+   const int d = deg[v], m = ncnt[v];
+   const unsigned int *row = nbr + (size_t)noff[v >> 6] * 64 + (v & 63);
+   v4f s = {0.f, 0.f, 0.f, 0.f};
+   for(int k = 0; k < m; k += 8)
+   {
+      unsigned int e[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+         e[u] = k + u < m ? __builtin_nontemporal_load(&row[(k + u) * 64]) : 0u;
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+         if(k + u < m) s += X[e[u] >> 8] * (float)(e[u] & 255u);
+   }
+   v4f p = xo[v];
+   if(!ref[v] && d > 0)
+   {
+      s *= 0.25f / (float)d;
+      p = p + (s - p) * omega;
+   }
+   xn[v] = p;
+}
+
 struct Shape { int grp = 256, remap = 1; };
 static Shape g_shape[8];
-static int g_smooth_fused = 1;
+static int g_smooth_mode = 2;
 
 template<class F>
 static int TimedLaunch(GmlCtx *g, int which, F launch)
@@ -238,8 +315,8 @@ int GmlxTetQuality(size_t GmlIdx, int QalIdx)
       return -2;
    long long nt = g->NmbEle[GmlTetrahedra];
    const Shape sh = g_shape[GmlxBuiltinQuality];
-   const int4 *tet = (const int4 *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
-   const float4 *crd = (const float4 *)g->dat[g->TypIdx[GmlVertices]].dev;
+   const v4i *tet = (const v4i *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
+   const v4f *crd = (const v4f *)g->dat[g->TypIdx[GmlVertices]].dev;
    float *q = (float *)g->dat[QalIdx].dev;
    g->dat[QalIdx].host_valid = false;
    if(!nt) return 1;
@@ -282,16 +359,42 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
    long long nv = g->NmbEle[GmlVertices], nt = g->NmbEle[GmlTetrahedra];
    Data *vd = &g->dat[g->TypIdx[GmlVertices]];
-   const int4 *tet = (const int4 *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
+   const v4i *tet = (const v4i *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
    const int *ref = (const int *)g->dat[g->RefIdx[GmlVertices]].dev;
    vd->host_valid = false;
    if(!nv || !nsweep) return 1;
 
-   if(g_smooth_fused)
+   if(g_smooth_mode == 2)
+   {
+      const Shape sf = g_shape[GmlxBuiltinSweep];
+      if(g->oplan && g->oplan->TV != sf.grp) { FreeSweepPlan(g, g->oplan); g->oplan = nullptr; }
+      if(!g->oplan && !(g->oplan = NewSweepPlan(g, sf.grp, 1))) return -4;
+      if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
+      SweepPlan *p = g->oplan;
+      size_t lds = (size_t)p->max_nv * 16;
+      if(lds > 160 * 1024) return -4;
+      if(lds > 64 * 1024)
+         HIPCHK(hipFuncSetAttribute((const void *)k_sweep_op, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      for(int i = 0; i < nsweep; i++)
+      {
+         const v4f *xo = (const v4f *)vd->dev;
+         v4f *xn = (v4f *)g->xalt;
+         int rc = TimedLaunch(g, GmlxBuiltinSweep, [&]() {
+            hipLaunchKernelGGL(k_sweep_op, dim3((unsigned int)p->ntile), dim3(p->TV), lds, g->stream, nv,
+                               p->vofs, p->vlist, (const int *)g->dat[u->DegIdx].dev, p->ncnt, p->noff, p->nbr,
+                               ref, xo, xn, (float)omega, sf.remap);
+         });
+         if(rc != 1) return rc;
+         void *t = vd->dev; vd->dev = g->xalt; g->xalt = t;
+      }
+      return 1;
+   }
+
+   if(g_smooth_mode == 1)
    {
       const Shape sf = g_shape[GmlxBuiltinSweep];
       if(g->plan && g->plan->TV != sf.grp) { FreeSweepPlan(g, g->plan); g->plan = nullptr; }
-      if(!g->plan && !(g->plan = NewSweepPlan(g, sf.grp))) return -4;
+      if(!g->plan && !(g->plan = NewSweepPlan(g, sf.grp, 0))) return -4;
       if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
       SweepPlan *p = g->plan;
       size_t lds = (size_t)(p->max_nv + p->max_nt) * 16;
@@ -300,11 +403,11 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
          HIPCHK(hipFuncSetAttribute((const void *)k_sweep_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       for(int i = 0; i < nsweep; i++)
       {
-         const float4 *xo = (const float4 *)vd->dev;
-         float4 *xn = (float4 *)g->xalt;
+         const v4f *xo = (const v4f *)vd->dev;
+         v4f *xn = (v4f *)g->xalt;
          int rc = TimedLaunch(g, GmlxBuiltinSweep, [&]() {
             hipLaunchKernelGGL(k_sweep_fused, dim3((unsigned int)p->ntile), dim3(p->TV), lds, g->stream, nv,
-                               p->vofs, p->vlist, p->tofs, (const ushort4 *)p->tloc,
+                               p->vofs, p->vlist, p->tofs, (const v4u16 *)p->tloc,
                                (const int *)g->dat[u->DegIdx].dev, (const unsigned int *)g->dat[u->OffIdx].dev,
                                p->lidx, ref, xo, xn, (float)omega, sf.remap);
          });
@@ -315,39 +418,41 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
       return 1;
    }
 
-   float4 *crd = (float4 *)vd->dev;
+   v4f *crd = (v4f *)vd->dev;
    const Shape ss = g_shape[GmlxBuiltinScatter], sg = g_shape[GmlxBuiltinGather];
    b->host_valid = false;
    for(int i = 0; i < nsweep; i++)
    {
       int rc = TimedLaunch(g, GmlxBuiltinScatter, [&]() {
          hipLaunchKernelGGL(k_smooth_scatter, dim3((unsigned int)((nt + ss.grp - 1) / ss.grp)), dim3(ss.grp), 0,
-                            g->stream, nt, tet, (const float4 *)crd, (float4 *)b->dev, ss.remap);
+                            g->stream, nt, tet, (const v4f *)crd, (v4f *)b->dev, ss.remap);
       });
       if(rc != 1) return rc;
       rc = TimedLaunch(g, GmlxBuiltinGather, [&]() {
          hipLaunchKernelGGL(k_smooth_gather, dim3((unsigned int)((nv + sg.grp - 1) / sg.grp)), dim3(sg.grp), 0,
                             g->stream, nv, (const int *)g->dat[u->DegIdx].dev, (const unsigned int *)g->dat[u->OffIdx].dev,
-                            (const int *)g->dat[u->BalIdx].dev, (const float4 *)b->dev, ref, crd, (float)omega, sg.remap);
+                            (const int *)g->dat[u->BalIdx].dev, (const v4f *)b->dev, ref, crd, (float)omega, sg.remap);
       });
       if(rc != 1) return rc;
    }
    return 1;
 }
 
-int GmlxSetSmoothMode(int fused)
+int GmlxSetSmoothMode(int mode)
 {
-   g_smooth_fused = fused ? 1 : 0;
+   if(mode < 0 || mode > 2) return 0;
+   g_smooth_mode = mode;
    return 1;
 }
 
 int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
 {
    GML_CTX(g, GmlIdx);
-   SweepPlan *p = g->plan;
+   SweepPlan *p = g_smooth_mode == 2 ? g->oplan : g->plan;
    if(!p || !info) return 0;
    info[0] = p->TV; info[1] = p->ntile; info[2] = p->sum_nv; info[3] = p->sum_nt;
    info[4] = p->max_nv; info[5] = p->max_nt; info[6] = (long long)p->bytes;
+   info[7] = p->sum_ring; info[8] = g_smooth_mode;
    return 1;
 }
 

@@ -11,6 +11,12 @@
 // loads of the ascending vertex list, forms every distinct barycentre once in
 // LDS, and sums each vertex's ball from LDS in the ball's own order -- the
 // same additions, in the same order, as the scatter + gather kernels.
+//
+// The operator form (op = 1) keeps the tile vertex lists and replaces the
+// tet lists by each vertex's 1-ring: the mean ball barycentre of v is
+// sum_u c_vu x_u / (4 deg v), c_vu = number of ball tets of v that contain
+// u (c_vv = deg v), so a sweep applies a fixed sparse operator; the 1-ring
+// is stored as (tile-local index << 8 | c_vu), ascending local index.
 
 #include "gml_internal.h"
 
@@ -20,7 +26,7 @@
 
 namespace gml {
 
-SweepPlan *NewSweepPlan(GmlCtx *g, int TV)
+SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
 {
    Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
    const long long nv = g->NmbEle[GmlVertices];
@@ -41,7 +47,8 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV)
    const long long ntile = (nv + TV - 1) / TV;
    std::vector<std::vector<int>> tv((size_t)ntile);
    std::vector<std::vector<unsigned short>> tl((size_t)ntile);
-   std::vector<unsigned short> lidx((size_t)u->nslab * 64 + 64, 0);
+   std::vector<unsigned short> lidx(op ? 1 : (size_t)u->nslab * 64 + 64, 0);
+   std::vector<std::vector<unsigned int>> ring(op ? (size_t)nv : 0);
    std::atomic<bool> fits(true);
 
    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -64,6 +71,34 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV)
             std::sort(vs.begin(), vs.end());
             vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
             if(ts.size() > 65535 || vs.size() > 65535) { fits = false; continue; }
+            if(op)
+            {
+               // 1-ring counts of every tile vertex, ascending tile-local index
+               std::vector<unsigned int> cnt;
+               for(long long v = v0; v < v1; v++)
+               {
+                  cnt.clear();
+                  for(int k = 0; k < deg[v]; k++)
+                  {
+                     int e = ent[((size_t)off[v >> 6] + k) * 64 + (v & 63)] >> 4;
+                     for(int j = 0; j < 4; j++)
+                        cnt.push_back((unsigned int)(std::lower_bound(vs.begin(), vs.end(), tet[(size_t)e * 4 + j]) - vs.begin()));
+                  }
+                  std::sort(cnt.begin(), cnt.end());
+                  auto &R = ring[v];
+                  R.clear();
+                  for(size_t i = 0; i < cnt.size();)
+                  {
+                     size_t j = i;
+                     while(j < cnt.size() && cnt[j] == cnt[i]) j++;
+                     if(j - i > 255) fits = false;
+                     R.push_back((cnt[i] << 8) | (unsigned int)(j - i));
+                     i = j;
+                  }
+               }
+               tv[t] = vs;
+               continue;
+            }
             auto &L = tl[t];
             L.resize(ts.size() * 4);
             for(size_t i = 0; i < ts.size(); i++)
@@ -93,6 +128,49 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV)
       p->max_nt = std::max(p->max_nt, (int)(tl[t].size() / 4));
    }
    std::vector<int> vlist(vofs[ntile] + 1);
+   if(op)
+   {
+      for(long long t = 0; t < ntile; t++)
+         std::copy(tv[t].begin(), tv[t].end(), vlist.begin() + vofs[t]);
+      std::vector<unsigned int> noff((size_t)nchunk + 1, 0);
+      std::vector<int> ncnt((size_t)nv);
+      for(long long c = 0; c < nchunk; c++)
+      {
+         unsigned int w = 0;
+         for(long long v = c * 64; v < std::min(nv, c * 64 + 64); v++) w = std::max(w, (unsigned int)ring[v].size());
+         noff[c + 1] = noff[c] + w;
+      }
+      p->nslab = noff[nchunk];
+      std::vector<unsigned int> nbr((size_t)p->nslab * 64 + 64, 0);
+      for(long long v = 0; v < nv; v++)
+      {
+         ncnt[v] = (int)ring[v].size();
+         p->sum_ring += ncnt[v];
+         for(size_t k = 0; k < ring[v].size(); k++)
+            nbr[((size_t)noff[v >> 6] + k) * 64 + (v & 63)] = ring[v][k];
+      }
+      p->vofs = (unsigned int *)DevAlloc(g, vofs.size() * 4);
+      p->vlist = (int *)DevAlloc(g, vlist.size() * 4);
+      p->noff = (unsigned int *)DevAlloc(g, noff.size() * 4);
+      p->ncnt = (int *)DevAlloc(g, ncnt.size() * 4);
+      p->nbr = (unsigned int *)DevAlloc(g, nbr.size() * 4);
+      if(!p->vofs || !p->vlist || !p->noff || !p->ncnt || !p->nbr
+      || !DevCopy(g, p->vofs, vofs.data(), vofs.size() * 4, 1)
+      || !DevCopy(g, p->vlist, vlist.data(), vlist.size() * 4, 1)
+      || !DevCopy(g, p->noff, noff.data(), noff.size() * 4, 1)
+      || !DevCopy(g, p->ncnt, ncnt.data(), ncnt.size() * 4, 1)
+      || !DevCopy(g, p->nbr, nbr.data(), nbr.size() * 4, 1))
+      {
+         FreeSweepPlan(g, p);
+         return nullptr;
+      }
+      p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + ncnt.size() * 4 + nbr.size() * 4;
+      p->sum_nv = vofs[ntile];
+      g->MemSiz += p->bytes;
+      DbgPrint(g, "Operator sweep plan: %lld tiles of %d vertices, %u tile vertices, %lld ring slabs, %zu MB\n",
+               ntile, TV, vofs[ntile], p->nslab, p->bytes >> 20);
+      return p;
+   }
    std::vector<unsigned short> tloc((size_t)tofs[ntile] * 4 + 4);
    for(long long t = 0; t < ntile; t++)
    {
@@ -129,6 +207,7 @@ void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
    if(!p) return;
    DevFree(g, p->vofs); DevFree(g, p->tofs); DevFree(g, p->vlist);
    DevFree(g, p->tloc); DevFree(g, p->lidx);
+   DevFree(g, p->nbr); DevFree(g, p->noff); DevFree(g, p->ncnt);
    g->MemSiz -= p->bytes;
    delete p;
 }

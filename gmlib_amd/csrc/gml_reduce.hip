@@ -16,7 +16,7 @@
 namespace gml {
 
 static const int kRedBlock = 256;
-static const int kRedMaxGrid = 2048;
+static const int kRedMaxGrid = 16384;
 
 struct RedScratch {
    double       part[kRedMaxGrid];
@@ -165,6 +165,7 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ 
 
 static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s)
 {
+   // about four float4 per thread, up to 16384 work-groups
    long long want = (n / 4 + kRedBlock * 4 - 1) / (kRedBlock * 4);
    int grid = (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
    switch(op)

@@ -308,7 +308,8 @@ void GmlStop(size_t GmlIdx)
       for(auto &e : g->red_evts[r]) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
    FreeBuiltinStats(g);
    FreeSweepPlan(g, g->plan);
-   g->plan = nullptr;
+   FreeSweepPlan(g, g->oplan);
+   g->plan = g->oplan = nullptr;
    if(g->xalt) { DevFree(g, g->xalt); g->xalt = nullptr; }
    FreeAlgebra(g);
    for(int i = 1; i <= GmlMaxDat; i++)

@@ -108,16 +108,23 @@ int    GmlxTetQuality(size_t GmlIdx, int QalIdx);
 /* per-vertex minimum of a GmlFlt tet field over the vertex ball */
 int    GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx);
 /* nsweep Jacobi sweeps of barycentre smoothing of the vertex coordinates
-   (vertices with ref != 0 stay fixed).  Fused mode (default): one kernel per
-   sweep over vertex tiles (LDS-staged coordinates and barycentres, tile plan
-   built on first call, coordinates double-buffered: after a sweep the vertex
-   table's device address is the other buffer).  Two-kernel mode: barycentre
-   scatter into BaryIdx (a GmlFlt4 tet solution), then ball gather. */
+   (vertices with ref != 0 stay fixed).  Modes (GmlxSetSmoothMode):
+   2 (default) operator form: one kernel per sweep over 256-vertex tiles,
+     1-ring coordinates staged in LDS, x_v += omega (sum_u c_vu x_u /
+     (4 deg v) - x_v) with c_vu the number of ball tets sharing u; equal to
+     the two-kernel result within FP32 round-off;
+   1 fused exact: one kernel per sweep, barycentres formed in LDS, results
+     bit-identical to the scatter + gather kernels;
+   0 two kernels: barycentre scatter into BaryIdx (a GmlFlt4 tet solution),
+     then ball gather.
+   Modes 1 and 2 build a tile plan on first use and double-buffer the
+   coordinates: after a sweep the vertex table's device address is the other
+   buffer (query it again with GmlxGetDevicePointer). */
 int    GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega);
-/* 1: fused sweep (default), 0: scatter + gather kernels */
-int    GmlxSetSmoothMode(int fused);
-/* tile plan of the fused sweep: {tile size, tiles, sum of tile vertices,
-   sum of tile tets, max vertices per tile, max tets per tile, bytes} */
+int    GmlxSetSmoothMode(int mode);
+/* tile plan of the current sweep mode (info: 9 values): {tile size, tiles,
+   sum of tile vertices, sum of tile tets, max vertices per tile, max tets per
+   tile, plan bytes, total 1-ring slots (operator form), mode} */
 int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
 /* launch shape of a built-in kernel for every handle of the process */
 int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);

@@ -142,3 +142,18 @@ def test_extract_edges_counts():
         assert g.L.GmlGetDataBlock(g.h, G.GmlEdges, 0, ne - 1, G.ptr(e), C.c_void_p(e.ctypes.data + (ne - 1) * 8),
                                    G.ptr(r), C.c_void_p(r.ctypes.data + (ne - 1) * 4))
         assert np.all(e[:, 0] < e[:, 1]) and len({tuple(x) for x in e}) == ne
+
+
+def test_c_user_program_compiles_and_links():
+    """A plain C GMlib program builds against include/gmlib3.h and links
+    libgml.so with gcc (running it needs the GPU: tests/test_gpu_parity.py)."""
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "gmlib_amd", "examples", "mesh_quality.c")
+    out = os.path.join(tempfile.mkdtemp(), "mesh_quality")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
+                        "-L", os.path.join(ROOT, "gmlib_amd"), "-lgml",
+                        "-Wl,-rpath," + os.path.join(ROOT, "gmlib_amd"), "-o", out],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    assert os.path.exists(out)

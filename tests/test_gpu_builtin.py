@@ -18,7 +18,8 @@ def body(name):
 
 
 @pytest.mark.parametrize("n,grp,remap,fused", [(20, 256, 1, 1), (13, 64, 0, 1), (11, 128, 1, 1),
-                                                (20, 256, 1, 0), (5, 1024, 1, 0)])
+                                                (20, 256, 1, 0), (5, 1024, 1, 0),
+                                                (20, 256, 1, 2), (13, 64, 0, 2)])
 def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused):
     for w in range(4):
         assert G.lib().GmlxSetBuiltinShape(w, grp, remap)
@@ -55,14 +56,21 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
                 which = G.GmlxBuiltinSweep if fused else G.GmlxBuiltinGather
                 assert g.L.GmlxGetBuiltinRunTime(g.h, which, C.byref(nl)) > 0 and nl.value == 5
                 if fused:
-                    info = (C.c_longlong * 7)()
+                    info = (C.c_longlong * 9)()
                     assert g.L.GmlxGetSweepPlanInfo(g.h, info) and info[0] == min(grp, 256)
-                    print("plan: tiles %d, tets per tet %.3f, max tets/vertices per tile %d/%d"
-                          % (info[1], info[3] / len(tet), info[5], info[4]))
+                    print("plan: tiles %d, tile tets per tet %.3f, tile vertices per vertex %.3f"
+                          % (info[1], info[3] / len(tet), info[2] / len(crd)))
             c, r = g.vertices(len(crd))
             res[path] = (q0, v0, c)
-    for a, b_ in zip(res["generated"], res["builtin"]):
+    for a, b_ in zip(res["generated"][:2], res["builtin"][:2]):
         assert np.array_equal(a, b_)
+    cg, cb = res["generated"][2], res["builtin"][2]
+    if fused == 2:
+        # operator form: same sweep up to FP32 summation order
+        print("operator form vs two kernels: max |dx| %.3e" % np.abs(cg - cb).max())
+        assert np.abs(cg - cb).max() <= 2e-6
+    else:
+        assert np.array_equal(cg, cb)
     q64 = O.tet_quality64(tet, crd)
     assert np.max(np.abs(res["builtin"][0] - q64) / np.maximum(np.abs(q64), 1e-3)) <= 1e-5
     d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
@@ -71,4 +79,4 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
     assert np.abs(res["builtin"][2][:, :3] - c64).max() <= 1e-5
     for w in range(5):
         G.lib().GmlxSetBuiltinShape(w, 256, 1)
-    G.lib().GmlxSetSmoothMode(1)
+    G.lib().GmlxSetSmoothMode(2)

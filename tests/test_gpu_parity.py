@@ -229,3 +229,21 @@ def test_set_get_lines_on_device():
         assert np.array_equal(crd[:, 1], np.arange(5) + .25) and np.array_equal(ref, [0, 1, 0, 1, 0])
         x = C.c_double()
         assert G.get_line(g.h, vi, 3, C.byref(x), C.byref(C.c_double()), C.byref(C.c_double())) and x.value == 3.0
+
+
+def test_c_user_program_runs():
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = os.path.join(tempfile.mkdtemp(), "mesh_quality")
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "gmlib_amd", "examples", "mesh_quality.c"),
+                    "-L", os.path.join(root, "gmlib_amd"), "-lgml", "-Wl,-rpath," + os.path.join(root, "gmlib_amd"),
+                    "-o", out], check=True)
+    r = subprocess.run([out, "6"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    crd, ref, tet = G.cube_mesh(6, 0.2, 1)
+    q64 = O.tet_quality64(tet, crd)
+    got = float(r.stdout.split("min quality ")[1].split(",")[0])
+    assert abs(got - q64.min()) <= 1e-5 * abs(q64.min())
