@@ -13,7 +13,15 @@ kernel's roofline fraction against the 8 TB/s HBM peak, timed with HIP
 events on the library's stream, and the CPU baseline (the oracle's FP32
 restatement of the same sweep, OpenMP on the host cores, bounded sample).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+Second workload (--workload cg): block-sparse SpMV and block-Jacobi CG on the
+3x3-block vector Laplacian of the edge graph of a 128^3-cell Kuhn cube
+(2,146,689 block rows, 31.8M blocks), FP32 values from a seeded SPD edge
+weighting.  A step is one GmlMultMatVec (y = A x), counted as 40 B per block
+(36 B values + 4 B column) + 28 B per row (x, y, row length); then a plain
+FP32 CG solve and an FP32 CG with FP64 residual refinement to 1e-10, each
+checked against the oracle's FP64 product b - A x on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -39,6 +47,7 @@ def body(name):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -50,15 +59,22 @@ def main():
     ap.add_argument("--sweep-tile", type=int, default=256, help="vertices per tile of the fused sweep")
     ap.add_argument("--paths", default="generated,builtin")
     ap.add_argument("--copy-calib", type=int, default=0,
-                    help="launch a plain float4 copy kernel (16V read + 16V write) this many times first, "
-                         "to calibrate HBM byte counters")
+                    help="launch a plain copy-like kernel of known size this many times first (smoothing: "
+                         "float4 vertex copy, 16V read + 16V write; cg: GmlScaleVec, 12 B per row read and "
+                         "written), to calibrate HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is single-device (multi-GPU sharding is a design "
               "note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cg":
+        run_cg(args)
+    else:
+        run_smoothing(args)
 
+
+def run_smoothing(args):
     import numpy as np
     import gmlib_amd as G
 
@@ -226,6 +242,184 @@ def main():
                      "frac": D["frac"], "traffic": traffic, "kernel": "%s (%s path)" % (dom, best),
                      "bytes_per_launch": D["bytes_per_launch"], "avg_us": D["avg_us"]},
         "paths": paths,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+def run_cg(args):
+    import numpy as np
+    import gmlib_amd as G
+    import gmlib_amd.workloads as W
+    from tests import oracle_lib as O
+
+    n = args.n if args.n != 256 else 128
+    t0 = time.time()
+    crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
+    e = W.cube_edges(tet)
+    rp, col, val = W.edge_laplacian3(len(crd), e, 2024)
+    del crd, ref, tet, e
+    nv, nb = len(rp) - 1, len(col)
+    rhs = np.random.default_rng(7).normal(size=nv * 3).astype(np.float32)
+    log("edge Laplacian %d^3: %d block rows, %d 3x3 blocks (%.1f per row), built in %.1fs"
+        % (n, nv, nb, nb / nv, time.time() - t0))
+    spmv_bytes = 40 * nb + 28 * nv
+
+    g = G.Gml(0)
+    L, h = g.L, g.h
+    m = L.GmlNewMatrix(h, nv, nb, 3, G.ptr(val), G.ptr(col), G.ptr(rp), G.GmlFlt)
+    vx = L.GmlNewVector(h, nv, 3, G.ptr(rhs), G.GmlFlt)
+    vy = L.GmlNewVector(h, nv, 3, None, G.GmlFlt)
+    vs = L.GmlNewVector(h, nv, 3, None, G.GmlDbl)
+    if min(m, vx, vy, vs) <= 0:
+        raise RuntimeError("matrix/vector creation failed")
+    if args.copy_calib:
+        one = C.c_double(1.0)
+        for _ in range(args.copy_calib):
+            L.GmlScaleVec(h, vy, C.byref(one))
+        L.GmlxSync(h)
+        log("copy_calib: %d GmlScaleVec launches (k_scale) of %d bytes read + %d bytes written"
+            % (args.copy_calib, 12 * nv, 12 * nv))
+
+    # SpMV: the step.  Launch shapes (work-group size, XCD remap) are swept
+    # first on a few launches each; the best one is timed.
+    cnt = C.c_longlong()
+
+    def time_spmv(k):
+        L.GmlxSync(h)
+        k0 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinSpMV, C.byref(cnt))
+        c0 = cnt.value
+        t0 = time.perf_counter()
+        for _ in range(k):
+            if L.GmlMultMatVec(h, m, vy, vx) != 1:
+                raise RuntimeError("GmlMultMatVec failed")
+        L.GmlxSync(h)
+        dt = (time.perf_counter() - t0) / k
+        k1 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinSpMV, C.byref(cnt))
+        return dt, (k1 - k0) / (cnt.value - c0)
+
+    shapes = {}
+    for grp in (64, 128, 256):
+        for rm in (0, 1):
+            L.GmlxSetBuiltinShape(G.GmlxBuiltinSpMV, grp, rm)
+            time_spmv(2)
+            shapes["%d/%d" % (grp, rm)] = time_spmv(5)[1]
+    best = min(shapes, key=shapes.get)
+    grp, rm = (int(v) for v in best.split("/"))
+    log("SpMV shapes (group/remap: kernel us): %s -> %s"
+        % ({k: round(v * 1e6, 1) for k, v in shapes.items()}, best))
+    for w in (G.GmlxBuiltinSpMV, G.GmlxBuiltinCgSpMV):
+        L.GmlxSetBuiltinShape(w, grp, rm)
+    for _ in range(args.warmup):
+        L.GmlMultMatVec(h, m, vy, vx)
+    dt, kavg = time_spmv(args.steps)
+    y = np.empty(nv * 3, np.float32)
+    L.GmlxGetVector(h, vy, G.ptr(y))
+    y64 = O.bsr_spmv64(rp, col, val, rhs.astype(np.float64), 3)
+    spmv_err = float(np.abs(y - y64).max() / np.abs(y64).max())
+    gbs = spmv_bytes / dt / 1e9
+    kgbs = spmv_bytes / kavg / 1e9
+    log("SpMV: %.1f us/step (kernel %.1f us), %.1f GB/s step, %.1f GB/s kernel (%.1f%% of peak), "
+        "max err / max|y| vs FP64 oracle %.2e" % (dt * 1e6, kavg * 1e6, gbs, kgbs, kgbs / HBM_PEAK_GBS * 100, spmv_err))
+
+    # the two CG forms, each checked with the oracle's FP64 residual
+    # per CG iteration, every datum once: SpMV kernel (q = A p, p.q) 40 B per
+    # block + 28 B per row (row length, p read, q written); update kernel 108 B
+    # per row (x, p, r, q, 3x3 Minv read; x, r, z written) + direction update
+    # 36 B per row (z, p read; p written)
+    cg_spmv_bytes = 40 * nb + 28 * nv
+    cg_upd_bytes = 144 * nv
+    bn = float(np.linalg.norm(rhs.astype(np.float64)))
+    solves = {}
+    for name, tol, refine in (("fp32", 1e-5, 0), ("fp32_fp64_refined", 1e-10, 1)):
+        rr, it = C.c_double(), C.c_int()
+        ca, cb = C.c_longlong(), C.c_longlong()
+        s0 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinCgSpMV, C.byref(ca)); n0 = ca.value
+        u0 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinCgUpdate, C.byref(cb))
+        t0 = time.perf_counter()
+        rc = L.GmlxSolveCG(h, m, vx, vs, tol, 20000, refine, C.byref(rr), C.byref(it))
+        ts = time.perf_counter() - t0
+        if rc != 1:
+            raise RuntimeError("GmlxSolveCG(%s) returned %d" % (name, rc))
+        s1 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinCgSpMV, C.byref(ca))
+        u1 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinCgUpdate, C.byref(cb))
+        nl = max(1, ca.value - n0)
+        x = np.empty(nv * 3, np.float64)
+        L.GmlxGetVector(h, vs, G.ptr(x))
+        res = float(np.linalg.norm(rhs.astype(np.float64) - O.bsr_spmv64(rp, col, val, x, 3)) / bn)
+        solves[name] = {"tol": tol, "iterations": it.value, "seconds": round(ts, 4),
+                        "ms_per_iteration": round(ts / max(1, it.value) * 1e3, 4),
+                        "relres_gpu_fp64": rr.value, "relres_oracle_fp64": res,
+                        "cg_spmv_us": round((s1 - s0) / nl * 1e6, 2),
+                        "cg_spmv_GBs": round(cg_spmv_bytes * nl / (s1 - s0) / 1e9, 1),
+                        "cg_update_us": round((u1 - u0) / nl * 1e6, 2),
+                        "cg_update_GBs": round(cg_upd_bytes * nl / (u1 - u0) / 1e9, 1),
+                        "kernel_share": round((s1 - s0 + u1 - u0) / ts, 3)}
+        log("CG %s: %s" % (name, solves[name]))
+        if res > max(2 * tol, 1e-9) * (10 if not refine else 1):
+            raise RuntimeError("CG %s residual %.3e above its bound" % (name, res))
+
+    # plain streaming rate of this GPU: GmlScaleVec (read + write) over a 2 GiB vector
+    # (skipped in counter-calibration runs, whose GmlScaleVec launches must all be the small one)
+    big = L.GmlNewVector(h, 1 << 27, 4, None, G.GmlFlt) if not args.copy_calib else 0
+    probe = None
+    if big > 0:
+        one = C.c_double(1.0)
+        for _ in range(3):
+            L.GmlScaleVec(h, big, C.byref(one))
+        L.GmlxSync(h)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            L.GmlScaleVec(h, big, C.byref(one))
+        L.GmlxSync(h)
+        probe = round(2 * (1 << 29) * 4 * 10 / (time.perf_counter() - t0) / 1e9, 1)
+        L.GmlxFreeVector(h, big)
+        log("streaming probe (GmlScaleVec, 2 GiB read + 2 GiB written): %.1f GB/s" % probe)
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        OL = O.lib()
+        y32 = np.empty(nv * 3, np.float32)
+        OL.orc_bsr_spmv_f32(nv, 3, O.p(rp), O.p(col), O.p(val), O.p(rhs), O.p(y32))
+        t0 = time.perf_counter()
+        ns = 0
+        while ns < 3 or time.perf_counter() - t0 < args.cpu_seconds:
+            OL.orc_bsr_spmv_f32(nv, 3, O.p(rp), O.p(col), O.p(val), O.p(rhs), O.p(y32))
+            ns += 1
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(ns * spmv_bytes / ct / 1e9, 2), "unit": "GB/s",
+               "cores": OL.orc_num_threads(), "kind": "port",
+               "sample": "%d block-CSR SpMVs of the same matrix, oracle FP32 C + OpenMP, %.2f s" % (ns, ct)}
+        log("cpu baseline: %s" % cpu)
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_cg128.json")
+    if n == 128 and os.path.exists(tf):
+        traffic = json.load(open(tf)).get("spmv")
+    out = {
+        "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SPD edge Laplacian of a jittered Kuhn cube mesh, N(0,1) right-hand side)",
+        "config": {"workload": "3x3 block SpMV + block-Jacobi CG, edge Laplacian of a %d^3-cell cube "
+                               "(%d block rows, %d blocks)" % (n, nv, nb),
+                   "block_rows": nv, "blocks": nb, "bytes_per_step": spmv_bytes, "spmv_max_rel_err": spmv_err,
+                   "spmv_shape": best, "spmv_shapes_us": {k: round(v * 1e6, 2) for k, v in shapes.items()}},
+        "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_spmv<3,float,float> (GmlMultMatVec)",
+                     "bytes_per_launch": spmv_bytes, "avg_us": round(kavg * 1e6, 2)},
+        "cg": solves,
+        "stream_probe_GBs": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -25,7 +25,8 @@ GmlRefFlag, GmlReadMode, GmlWriteMode, GmlVoyeurs, GmlManual = 1, 2, 4, 8, 16
  GmlDbl, GmlDbl2, GmlDbl4, GmlDbl8, GmlDbl16,
  GmlByt, GmlByt2, GmlByt4, GmlByt8, GmlByt16, GmlMaxOclTyp) = range(21)
 GmlMin, GmlMax, GmlSum, GmlL0, GmlL1, GmlL2, GmlLinf, GmlMaxRed = range(8)
-GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather, GmlxBuiltinSweep = range(5)
+(GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather, GmlxBuiltinSweep,
+ GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate) = range(8)
 
 _S = C.c_size_t
 _I = C.c_int

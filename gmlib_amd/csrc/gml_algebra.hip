@@ -63,51 +63,120 @@ void FreeAlgebra(GmlCtx *g)
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// One block row per lane.  Slots are taken two at a time, both slots' loads
+// issued before either product, so a wave keeps ~2(B*B+B+1) loads in flight;
+// the products are added in slot order, as a one-at-a-time loop would.
 template<int B, class T, class X>
-__global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ deg, const unsigned int *__restrict__ off,
-                                              const int *__restrict__ col, const T *__restrict__ val,
-                                              const X *__restrict__ x, X *__restrict__ y)
+__device__ __forceinline__ void row_product(int r, int l, int d, size_t base, const int *__restrict__ col,
+                                            const T *__restrict__ val, const X *__restrict__ x, X *acc)
 {
-   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-   if(r >= n) return;
-   const int l = r & 63, d = deg[r];
-   const size_t base = (size_t)off[r >> 6];
-   X acc[B];
-#pragma unroll
-   for(int i = 0; i < B; i++) acc[i] = 0;
-   for(int k = 0; k < d; k++)
+   int k = 0;
+   for(; k + 1 < d; k += 2)
    {
-      const size_t s = base + k;
-      const int j = col[s * 64 + l];
-      X xj[B];
+      const size_t s0 = base + k, s1 = s0 + 1;
+      const int j0 = col[s0 * 64 + l], j1 = col[s1 * 64 + l];
+      T v0[B * B], v1[B * B];
 #pragma unroll
-      for(int i = 0; i < B; i++) xj[i] = x[(size_t)j * B + i];
-      const T *v = val + s * (B * B) * 64 + l;
+      for(int e = 0; e < B * B; e++)
+      {
+         v0[e] = val[(s0 * (B * B) + e) * 64 + l];
+         v1[e] = val[(s1 * (B * B) + e) * 64 + l];
+      }
+      X x0[B], x1[B];
+#pragma unroll
+      for(int m = 0; m < B; m++) { x0[m] = x[(size_t)j0 * B + m]; x1[m] = x[(size_t)j1 * B + m]; }
 #pragma unroll
       for(int i = 0; i < B; i++)
       {
          X t = 0;
 #pragma unroll
-         for(int m = 0; m < B; m++) t += (X)v[(i * B + m) * 64] * xj[m];
+         for(int m = 0; m < B; m++) t += (X)v0[i * B + m] * x0[m];
+         acc[i] += t;
+      }
+#pragma unroll
+      for(int i = 0; i < B; i++)
+      {
+         X t = 0;
+#pragma unroll
+         for(int m = 0; m < B; m++) t += (X)v1[i * B + m] * x1[m];
          acc[i] += t;
       }
    }
+   if(k < d)
+   {
+      const size_t s0 = base + k;
+      const int j0 = col[s0 * 64 + l];
+#pragma unroll
+      for(int i = 0; i < B; i++)
+      {
+         X t = 0;
+#pragma unroll
+         for(int m = 0; m < B; m++) t += (X)val[(s0 * (B * B) + i * B + m) * 64 + l] * x[(size_t)j0 * B + m];
+         acc[i] += t;
+      }
+   }
+}
+
+template<int B, class T, class X>
+__global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ deg, const unsigned int *__restrict__ off,
+                                              const int *__restrict__ col, const T *__restrict__ val,
+                                              const X *__restrict__ x, X *__restrict__ y, int remap)
+{
+   const int r = (int)(xcd_block(remap) * blockDim.x + threadIdx.x);
+   if(r >= n) return;
+   X acc[B];
+#pragma unroll
+   for(int i = 0; i < B; i++) acc[i] = 0;
+   row_product<B, T, X>(r, r & 63, deg[r], (size_t)off[r >> 6], col, val, x, acc);
 #pragma unroll
    for(int i = 0; i < B; i++) y[(size_t)r * B + i] = acc[i];
 }
 
+// Elementwise vector kernels: 16 bytes per load and store (float4 /
+// double2), the tail of n done by the last thread.
+template<class T> struct Vec16;
+template<> struct Vec16<float>  { typedef float4  t; static const int n = 4; };
+template<> struct Vec16<double> { typedef double2 t; static const int n = 2; };
+
 template<class T>
-__global__ void k_add3(long long n, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w, T *__restrict__ y)
+__global__ void __launch_bounds__(256) k_add3(long long n, const T *__restrict__ u, const T *__restrict__ v,
+                                              const T *__restrict__ w, T *__restrict__ y)
 {
-   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   if(i < n) y[i] = u[i] + v[i] + w[i];
+   typedef typename Vec16<T>::t V;
+   const int W = Vec16<T>::n;
+   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * W;
+   if(i + W <= n)
+   {
+      V a = *(const V *)&u[i], b = *(const V *)&v[i], c = *(const V *)&w[i];
+      a += b;
+      a += c;
+      *(V *)&y[i] = a;
+   }
+   else
+      for(long long k = i; k < n; k++) y[k] = u[k] + v[k] + w[k];
 }
 
 template<class T>
-__global__ void k_scale(long long n, T *__restrict__ u, T s)
+__global__ void __launch_bounds__(256) k_scale(long long n, T *__restrict__ u, T s)
 {
-   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   if(i < n) u[i] *= s;
+   typedef typename Vec16<T>::t V;
+   const int W = Vec16<T>::n;
+   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * W;
+   if(i + W <= n)
+   {
+      V a = *(const V *)&u[i];
+      a *= s;
+      *(V *)&u[i] = a;
+   }
+   else
+      for(long long k = i; k < n; k++) u[k] *= s;
+}
+
+// grid of k_add3 / k_scale over n entries
+template<class T> static unsigned int VecBlocks(long long n)
+{
+   const long long w = Vec16<T>::n * 256;
+   return (unsigned int)((n + w - 1) / w);
 }
 
 template<class T>
@@ -141,21 +210,22 @@ static unsigned int Blocks(long long n, int b = 256) { return (unsigned int)((n 
 template<class T, class X>
 static int SpMV(hipStream_t s, const Matrix *m, const X *x, X *y)
 {
-   unsigned int nb = Blocks(m->NmbLin);
+   const Shape sh = BuiltinShape(GmlxBuiltinSpMV);
+   const dim3 nb(Blocks(m->NmbLin, sh.grp)), gs(sh.grp);
    const int *deg = m->deg, *col = m->col;
    const unsigned int *off = m->off;
    const T *val = (const T *)m->val;
-   int n = m->NmbLin;
+   int n = m->NmbLin, rm = sh.remap;
    switch(m->b)
    {
-      case 1: hipLaunchKernelGGL((k_spmv<1, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 2: hipLaunchKernelGGL((k_spmv<2, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 3: hipLaunchKernelGGL((k_spmv<3, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 4: hipLaunchKernelGGL((k_spmv<4, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 5: hipLaunchKernelGGL((k_spmv<5, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 6: hipLaunchKernelGGL((k_spmv<6, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 7: hipLaunchKernelGGL((k_spmv<7, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
-      case 8: hipLaunchKernelGGL((k_spmv<8, T, X>), dim3(nb), dim3(256), 0, s, n, deg, off, col, val, x, y); break;
+      case 1: hipLaunchKernelGGL((k_spmv<1, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 2: hipLaunchKernelGGL((k_spmv<2, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 3: hipLaunchKernelGGL((k_spmv<3, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 4: hipLaunchKernelGGL((k_spmv<4, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 5: hipLaunchKernelGGL((k_spmv<5, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 6: hipLaunchKernelGGL((k_spmv<6, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 7: hipLaunchKernelGGL((k_spmv<7, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 8: hipLaunchKernelGGL((k_spmv<8, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
       default: return -1;
    }
    return hipGetLastError() == hipSuccess ? 1 : -6;
@@ -167,17 +237,31 @@ static Matrix *Mat(GmlCtx *g, int i) { return (i >= 1 && i <= GmlMaxMat) ? g->ma
 // ===========================================================================
 // Block-Jacobi preconditioned CG (FP32 vectors, FP64 dot accumulation), and
 // iterative refinement with the residual b - A x evaluated in FP64.
-// All scalars stay on the device; the host polls the residual every few
-// iterations without stalling the stream.
+// One iteration is three passes:
+//   k_cg_spmv:    q = A p, p.q -> alpha = rz / pq
+//   k_cg_update:  x += alpha p, r -= alpha q, z = Minv r, r.z, r.r -> beta
+//   k_cg_pupdate: p = z + beta p
+// (Forming p = z + beta p inside the SpMV, where it is gathered, measured
+// slower: it doubles the gathered loads of the product.)
+// Grid sums are deterministic and fence-free: each work-group stores its
+// partial with a plain store, and a one-work-group kernel (k_cg_finish) adds
+// the partials in block order right after; the kernel boundary orders the
+// two.  (A last-block ticket would need a device-scope fence per work-group,
+// which on CDNA writes back and invalidates the XCD's L2 every time.)
+// All scalars stay on the device; the host polls r.r every few iterations
+// without stalling the stream.
 // ===========================================================================
 static const int kCgBlock = 256;
-static const int kCgGrid = 1024;
+static const int kCgGrid = 4096;
 
 struct CgScal {
-   double part[3][kCgGrid];
    double rz, pq, rr, alpha, beta, rz0, rr0;
-   unsigned int ticket[3];
-   int done;
+};
+
+// per-block partial sums: 2 slots of np doubles each
+struct CgPart {
+   double *p;
+   unsigned int np;
 };
 
 __device__ __forceinline__ double wsum(double v)
@@ -186,59 +270,115 @@ __device__ __forceinline__ double wsum(double v)
    return v;
 }
 
-// sum v over the grid into part[slot]; the last block returns true with the
-// total in *tot (all its threads), and re-arms the ticket
-__device__ bool grid_sum(double v, CgScal *S, int slot, double *tot)
+// this work-group's sums of a (into slot 0) and b (into slot 1 if two)
+__device__ __forceinline__ void block_sums(double a, double b, CgPart P, bool two)
 {
-   __shared__ double lds[kCgBlock / 64];
-   __shared__ bool last;
-   v = wsum(v);
-   if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+   __shared__ double lds[2][kCgBlock / 64];
+   a = wsum(a);
+   b = wsum(b);
+   if((threadIdx.x & 63) == 0) { lds[0][threadIdx.x >> 6] = a; lds[1][threadIdx.x >> 6] = b; }
    __syncthreads();
    if(threadIdx.x == 0)
    {
-      double s = 0;
-      for(int i = 0; i < (int)(blockDim.x >> 6); i++) s += lds[i];
-      S->part[slot][blockIdx.x] = s;
-      __threadfence();
-      last = atomicAdd(&S->ticket[slot], 1u) == gridDim.x - 1;
+      double s = 0, t = 0;
+      for(int i = 0; i < (int)(blockDim.x >> 6); i++) { s += lds[0][i]; t += lds[1][i]; }
+      P.p[blockIdx.x] = s;
+      if(two) P.p[P.np + blockIdx.x] = t;
    }
-   __syncthreads();
-   if(!last) return false;
-   __threadfence();
+}
+
+// sum of n partials in a fixed order, by one work-group of kFinBlock threads,
+// eight independent loads in flight per thread
+static const int kFinBlock = 1024;
+
+__device__ double fin_sum(const double *part, unsigned int n)
+{
+   __shared__ double lds[kFinBlock / 64];
    double s = 0;
-   for(unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-      s += ((volatile double *)S->part[slot])[i];
+   for(unsigned int base = 0; base < n; base += 8 * blockDim.x)
+   {
+      double v[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+      {
+         const unsigned int i = base + u * blockDim.x + threadIdx.x;
+         v[u] = i < n ? part[i] : 0.;
+      }
+#pragma unroll
+      for(int u = 0; u < 8; u++) s += v[u];
+   }
    s = wsum(s);
    __syncthreads();
    if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
    __syncthreads();
    s = 0;
    for(int i = 0; i < (int)(blockDim.x >> 6); i++) s += lds[i];
-   *tot = s;
-   if(threadIdx.x == 0) atomicExch(&S->ticket[slot], 0u);
-   return true;
+   return s;
 }
 
-// p . q where q = A p has just been computed: alpha = rz / pq
-__global__ void __launch_bounds__(kCgBlock) k_cg_alpha(long long n, const float *__restrict__ p, const float *__restrict__ q, CgScal *S)
+enum { CgFinAlpha, CgFinBeta, CgFinInit, CgFinResid, CgFinDot };
+
+// one work-group: finish the grid sums of the kernel just run (n partials)
+template<int W>
+__global__ void __launch_bounds__(kFinBlock) k_cg_finish(CgScal *S, CgPart P, unsigned int n)
 {
-   double acc = 0;
-   for(long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-      acc += (double)p[i] * (double)q[i];
-   double tot;
-   if(grid_sum(acc, S, 0, &tot) && threadIdx.x == 0)
-   {
-      S->pq = tot;
-      S->alpha = (S->done || tot == 0.) ? 0. : S->rz / tot;
-   }
+   const double s0 = fin_sum(P.p, n);
+   const double s1 = (W == CgFinBeta || W == CgFinInit) ? fin_sum(P.p + P.np, n) : 0.;
+   if(threadIdx.x) return;
+   if(W == CgFinAlpha) { S->pq = s0; S->alpha = s0 == 0. ? 0. : S->rz / s0; }
+   if(W == CgFinBeta)  { S->beta = S->rz == 0. ? 0. : s0 / S->rz; S->rz = s0; S->rr = s1; }
+   if(W == CgFinInit)  { S->rz = s0; S->rz0 = s0; S->rr = s1; S->rr0 = s1; S->beta = 0.; }
+   if(W == CgFinResid) S->rr0 = s0;
+   if(W == CgFinDot)   S->pq = s0;
 }
 
-// x += alpha p; r -= alpha q; z = Minv r (block b x b per row); r.z and r.r
+// q = A p and this work-group's part of p.q
+template<int B>
+__global__ void __launch_bounds__(256) k_cg_spmv(int n, const int *__restrict__ deg, const unsigned int *__restrict__ off,
+                                                 const int *__restrict__ col, const float *__restrict__ val,
+                                                 const float *__restrict__ p, float *__restrict__ q, CgPart P, int remap)
+{
+   const int r = (int)(xcd_block(remap) * blockDim.x + threadIdx.x);
+   double pq = 0;
+   if(r < n)
+   {
+      float acc[B];
+#pragma unroll
+      for(int i = 0; i < B; i++) acc[i] = 0.f;
+      row_product<B, float, float>(r, r & 63, deg[r], (size_t)off[r >> 6], col, val, p, acc);
+#pragma unroll
+      for(int i = 0; i < B; i++)
+      {
+         const size_t e = (size_t)r * B + i;
+         q[e] = acc[i];
+         pq += (double)p[e] * (double)acc[i];
+      }
+   }
+   block_sums(pq, 0., P, false);
+}
+
+// p = z + beta p, four entries per thread
+__global__ void __launch_bounds__(256) k_cg_pupdate(long long n, float *__restrict__ p, const float *__restrict__ z,
+                                                    const CgScal *S)
+{
+   const float b = (float)S->beta;
+   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+   if(i + 3 < n)
+   {
+      float4 pv = *(const float4 *)&p[i], zv = *(const float4 *)&z[i];
+      pv.x = zv.x + b * pv.x; pv.y = zv.y + b * pv.y; pv.z = zv.z + b * pv.z; pv.w = zv.w + b * pv.w;
+      *(float4 *)&p[i] = pv;
+   }
+   else
+      for(long long k = i; k < n; k++) p[k] = z[k] + b * p[k];
+}
+
+// x += alpha p; r -= alpha q; z = Minv r (Minv entry e of row i at e*nrow+i); r.z, r.r -> beta
 template<int B>
 __global__ void __launch_bounds__(kCgBlock) k_cg_update(int nrow, float *__restrict__ x, float *__restrict__ r,
                                                         const float *__restrict__ p, const float *__restrict__ q,
-                                                        const float *__restrict__ Minv, float *__restrict__ z, CgScal *S)
+                                                        const float *__restrict__ Minv, float *__restrict__ z,
+                                                        const CgScal *S, CgPart P)
 {
    const float a = (float)S->alpha;
    double rz = 0, rr = 0;
@@ -253,40 +393,25 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_update(int nrow, float *__restr
          rv[i] = r[k] - a * q[k];
          r[k] = rv[i];
       }
-      const float *M = Minv + (size_t)row * B * B;
 #pragma unroll
       for(int i = 0; i < B; i++)
       {
          float t = 0.f;
 #pragma unroll
-         for(int m = 0; m < B; m++) t += M[i * B + m] * rv[m];
+         for(int m = 0; m < B; m++) t += Minv[(size_t)(i * B + m) * nrow + row] * rv[m];
          z[(size_t)row * B + i] = t;
          rz += (double)rv[i] * (double)t;
          rr += (double)rv[i] * (double)rv[i];
       }
    }
-   double tot;
-   if(grid_sum(rz, S, 1, &tot) && threadIdx.x == 0)
-   {
-      S->beta = (S->done || S->rz == 0.) ? 0. : tot / S->rz;
-      S->rz = tot;
-   }
-   if(grid_sum(rr, S, 2, &tot) && threadIdx.x == 0)
-      S->rr = tot;
+   block_sums(rz, rr, P, true);
 }
 
-__global__ void k_cg_pupdate(long long n, float *__restrict__ p, const float *__restrict__ z, const CgScal *S)
-{
-   const float b = (float)S->beta;
-   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   if(i < n) p[i] = z[i] + b * p[i];
-}
-
-// init: r = rhs (x = 0), z = Minv r, p = z, rz = r.z, rr = r.r
+// init: r = rhs (x = 0), z = Minv r, p = z, beta = 0, rz = r.z, rr = r.r
 template<int B>
 __global__ void __launch_bounds__(kCgBlock) k_cg_init(int nrow, const float *__restrict__ rhs, float *__restrict__ x, float *__restrict__ r,
                                                       float *__restrict__ p, float *__restrict__ z,
-                                                      const float *__restrict__ Minv, CgScal *S)
+                                                      const float *__restrict__ Minv, CgPart P)
 {
    double rz = 0, rr = 0;
    for(int row = blockIdx.x * blockDim.x + threadIdx.x; row < nrow; row += gridDim.x * blockDim.x)
@@ -300,65 +425,84 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_init(int nrow, const float *__r
          r[k] = rv[i];
          x[k] = 0.f;
       }
-      const float *M = Minv + (size_t)row * B * B;
 #pragma unroll
       for(int i = 0; i < B; i++)
       {
          float t = 0.f;
 #pragma unroll
-         for(int m = 0; m < B; m++) t += M[i * B + m] * rv[m];
+         for(int m = 0; m < B; m++) t += Minv[(size_t)(i * B + m) * nrow + row] * rv[m];
          z[(size_t)row * B + i] = t;
          p[(size_t)row * B + i] = t;
          rz += (double)rv[i] * (double)t;
          rr += (double)rv[i] * (double)rv[i];
       }
    }
-   double tot;
-   if(grid_sum(rz, S, 1, &tot) && threadIdx.x == 0)
-   {
-      S->rz = tot;
-      S->rz0 = tot;
-      S->done = 0;
-   }
-   if(grid_sum(rr, S, 2, &tot) && threadIdx.x == 0)
-   {
-      S->rr = tot;
-      S->rr0 = tot;
-   }
+   block_sums(rz, rr, P, true);
 }
 
-// block-Jacobi: invert the diagonal block of every row (FP64 Gauss-Jordan)
+// block-Jacobi: invert the diagonal block of every row (FP64 Gauss-Jordan
+// with partial pivoting; every loop unrolled and the row swap done by
+// selects, so the B x 2B tableau stays in registers); entry e of row r's
+// inverse lands at Minv[e * nrow + r]
 template<int B>
-__global__ void k_block_jacobi(int nrow, const int *__restrict__ deg, const unsigned int *__restrict__ off,
-                               const int *__restrict__ col, const float *__restrict__ val, float *__restrict__ Minv)
+__global__ void __launch_bounds__(256) k_block_jacobi(int nrow, const int *__restrict__ deg, const unsigned int *__restrict__ off,
+                                                      const int *__restrict__ col, const float *__restrict__ val,
+                                                      float *__restrict__ Minv)
 {
    int r = blockIdx.x * blockDim.x + threadIdx.x;
    if(r >= nrow) return;
    const int l = r & 63, d = deg[r];
    const size_t base = off[r >> 6];
    double a[B][2 * B];
+#pragma unroll
    for(int i = 0; i < B; i++)
+#pragma unroll
       for(int j = 0; j < 2 * B; j++) a[i][j] = (j - B == i) ? 1. : 0.;
    for(int k = 0; k < d; k++)
       if(col[(base + k) * 64 + l] == r)
+      {
+#pragma unroll
          for(int i = 0; i < B; i++)
+#pragma unroll
             for(int j = 0; j < B; j++) a[i][j] += val[((base + k) * B * B + i * B + j) * 64 + l];
+      }
+#pragma unroll
    for(int c = 0; c < B; c++)
    {
+      // pivot: the largest |a[i][c]|, i >= c, first one on ties
+      double best = fabs(a[c][c]);
       int piv = c;
-      for(int i = c + 1; i < B; i++) if(fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
-      if(piv != c) for(int j = 0; j < 2 * B; j++) { double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
-      double inv = a[c][c] != 0. ? 1. / a[c][c] : 0.;
+#pragma unroll
+      for(int i = c + 1; i < B; i++)
+         if(fabs(a[i][c]) > best) { best = fabs(a[i][c]); piv = i; }
+#pragma unroll
+      for(int i = c + 1; i < B; i++)
+      {
+         const bool sw = (i == piv);
+#pragma unroll
+         for(int j = 0; j < 2 * B; j++)
+         {
+            const double t = a[c][j];
+            a[c][j] = sw ? a[i][j] : t;
+            a[i][j] = sw ? t : a[i][j];
+         }
+      }
+      const double inv = a[c][c] != 0. ? 1. / a[c][c] : 0.;
+#pragma unroll
       for(int j = 0; j < 2 * B; j++) a[c][j] *= inv;
+#pragma unroll
       for(int i = 0; i < B; i++)
          if(i != c)
          {
-            double f = a[i][c];
+            const double f = a[i][c];
+#pragma unroll
             for(int j = 0; j < 2 * B; j++) a[i][j] -= f * a[c][j];
          }
    }
+#pragma unroll
    for(int i = 0; i < B; i++)
-      for(int j = 0; j < B; j++) Minv[(size_t)r * B * B + i * B + j] = (float)a[i][B + j];
+#pragma unroll
+      for(int j = 0; j < B; j++) Minv[(size_t)(i * B + j) * nrow + r] = (float)a[i][B + j];
 }
 
 // FP64 residual r64 = b - A x64 (grid-stride over rows) and its squared norm
@@ -366,7 +510,7 @@ template<int B>
 __global__ void __launch_bounds__(kCgBlock) k_resid64(int nrow, const int *__restrict__ deg, const unsigned int *__restrict__ off,
                                                       const int *__restrict__ col, const float *__restrict__ val,
                                                       const double *__restrict__ x, const float *__restrict__ rhs,
-                                                      double *__restrict__ r64, CgScal *S)
+                                                      double *__restrict__ r64, CgPart P)
 {
    double nrm = 0;
    for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrow; r += gridDim.x * blockDim.x)
@@ -389,9 +533,7 @@ __global__ void __launch_bounds__(kCgBlock) k_resid64(int nrow, const int *__res
          nrm += acc[i] * acc[i];
       }
    }
-   double tot;
-   if(grid_sum(nrm, S, 0, &tot) && threadIdx.x == 0)
-      S->rr0 = tot;
+   block_sums(nrm, 0., P, false);
 }
 
 __global__ void k_axpy64(long long n, double *__restrict__ x, const float *__restrict__ d, double a)
@@ -406,20 +548,20 @@ __global__ void k_f64_to_f32(long long n, const double *__restrict__ a, float *_
    if(i < n) b[i] = (float)a[i];
 }
 
-__global__ void __launch_bounds__(kCgBlock) k_dot(long long n, const float *__restrict__ a, const float *__restrict__ b, CgScal *S)
+__global__ void __launch_bounds__(kCgBlock) k_dot(long long n, const float *__restrict__ a, const float *__restrict__ b,
+                                                  CgPart P)
 {
    double acc = 0;
    for(long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
       acc += (double)a[i] * (double)b[i];
-   double tot;
-   if(grid_sum(acc, S, 0, &tot) && threadIdx.x == 0)
-      S->pq = tot;
+   block_sums(acc, 0., P, false);
 }
 
 struct CgWork {
    float *x, *r, *p, *q, *z, *Minv, *d32;
    double *x64, *r64;
    CgScal *S;
+   CgPart P;
    double *host;   // pinned: rr readback
 };
 
@@ -537,11 +679,14 @@ int GmlMultMatVec(size_t GmlIdx, int MatIdx, int VecIdx1, int VecIdx2)
       return -4;
    }
    if(g->host_only) return -6;
-   int rc;
-   if(m->flt == GmlFlt && x->flt == GmlFlt) rc = SpMV<float, float>(g->stream, m, (const float *)x->dev, (float *)y->dev);
-   else if(m->flt == GmlFlt)                rc = SpMV<float, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
-   else if(x->flt == GmlDbl)                rc = SpMV<double, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
-   else return -4;
+   if(m->flt == GmlDbl && x->flt != GmlDbl) return -4;
+   int rc = 1;
+   int tl = TimedLaunch(g, GmlxBuiltinSpMV, [&]() {
+      if(m->flt == GmlFlt && x->flt == GmlFlt) rc = SpMV<float, float>(g->stream, m, (const float *)x->dev, (float *)y->dev);
+      else if(m->flt == GmlFlt)                rc = SpMV<float, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
+      else                                     rc = SpMV<double, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
+   });
+   if(tl != 1) return tl;
    if(rc != 1) return rc;
    g->MemAcc += m->MemAcc;
    g->FltOpp += m->FltOpp;
@@ -592,10 +737,10 @@ int GmlAddVec3(size_t GmlIdx, int VecIdx1, int VecIdx2, int VecIdx3, int VecIdx4
    if(g->host_only) return -6;
    long long n = (long long)a->NmbLin * a->b;
    if(a->flt == GmlFlt)
-      hipLaunchKernelGGL(k_add3<float>, dim3(Blocks(n)), dim3(256), 0, g->stream, n, (const float *)a->dev,
+      hipLaunchKernelGGL(k_add3<float>, dim3(VecBlocks<float>(n)), dim3(256), 0, g->stream, n, (const float *)a->dev,
                          (const float *)b->dev, (const float *)c->dev, (float *)y->dev);
    else
-      hipLaunchKernelGGL(k_add3<double>, dim3(Blocks(n)), dim3(256), 0, g->stream, n, (const double *)a->dev,
+      hipLaunchKernelGGL(k_add3<double>, dim3(VecBlocks<double>(n)), dim3(256), 0, g->stream, n, (const double *)a->dev,
                          (const double *)b->dev, (const double *)c->dev, (double *)y->dev);
    if(hipGetLastError() != hipSuccess) return -6;
    g->MemAcc += (double)a->NmbLin * a->b * (a->flt == GmlFlt ? 4 : 8) * 3;
@@ -619,9 +764,9 @@ int GmlScaleVec(size_t GmlIdx, int VecIdx, double *mul)
    if(g->host_only) return -6;
    long long n = (long long)v->NmbLin * v->b;
    if(v->flt == GmlFlt)
-      hipLaunchKernelGGL(k_scale<float>, dim3(Blocks(n)), dim3(256), 0, g->stream, n, (float *)v->dev, (float)s);
+      hipLaunchKernelGGL(k_scale<float>, dim3(VecBlocks<float>(n)), dim3(256), 0, g->stream, n, (float *)v->dev, (float)s);
    else
-      hipLaunchKernelGGL(k_scale<double>, dim3(Blocks(n)), dim3(256), 0, g->stream, n, (double *)v->dev, s);
+      hipLaunchKernelGGL(k_scale<double>, dim3(VecBlocks<double>(n)), dim3(256), 0, g->stream, n, (double *)v->dev, s);
    if(hipGetLastError() != hipSuccess) return -6;
    g->MemAcc += (double)v->NmbLin * v->b * (v->flt == GmlFlt ? 4 : 8) * 2;
    g->FltOpp += (double)v->NmbLin * v->b;
@@ -719,7 +864,10 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    const int nrow = A->NmbLin, B = A->b;
    const long long n = (long long)nrow * B;
    hipStream_t s = g->stream;
+   const Shape sh = BuiltinShape(GmlxBuiltinCgSpMV), su = BuiltinShape(GmlxBuiltinCgUpdate);
+   const unsigned int nbs = Blocks(nrow, sh.grp);
    CgWork w = {};
+   w.P.np = std::max<unsigned int>(kCgGrid, nbs);
    bool ok = true;
    auto al = [&](size_t by) { void *p = DevAlloc(g, by); if(!p) ok = false; return p; };
    w.x = (float *)al(n * 4); w.r = (float *)al(n * 4); w.p = (float *)al(n * 4);
@@ -727,10 +875,12 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    w.Minv = (float *)al((size_t)nrow * B * B * 4);
    w.x64 = (double *)al(n * 8); w.r64 = (double *)al(n * 8);
    w.S = (CgScal *)al(sizeof(CgScal));
+   w.P.p = (double *)al((size_t)2 * w.P.np * 8);
    if(ok && hipHostMalloc((void **)&w.host, 64, hipHostMallocDefault) != hipSuccess) ok = false;
    auto cleanup = [&]() {
       DevFree(g, w.x); DevFree(g, w.r); DevFree(g, w.p); DevFree(g, w.q); DevFree(g, w.z);
       DevFree(g, w.d32); DevFree(g, w.Minv); DevFree(g, w.x64); DevFree(g, w.r64); DevFree(g, w.S);
+      DevFree(g, w.P.p);
       if(w.host) (void)hipHostFree(w.host);
    };
    if(!ok) { cleanup(); return -5; }
@@ -738,19 +888,21 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    HIPCHK(hipMemsetAsync(w.x64, 0, n * 8, s));
    unsigned int nbr = Blocks(nrow), nbn = Blocks(n);
    unsigned int gr = std::min<unsigned int>(kCgGrid, Blocks(nrow, kCgBlock));
+   unsigned int gu = std::min<unsigned int>(kCgGrid, Blocks(nrow, su.grp));
    unsigned int gn = std::min<unsigned int>(kCgGrid, Blocks(n, kCgBlock));
 #define GML_B(K, ...) do { switch(B) { case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break; \
    case 3: hipLaunchKernelGGL(K<3>, __VA_ARGS__); break; default: hipLaunchKernelGGL(K<4>, __VA_ARGS__); } } while(0)
    GML_B(k_block_jacobi, dim3(nbr), dim3(256), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val, w.Minv);
 
    // |rhs|^2
-   hipLaunchKernelGGL(k_dot, dim3(gn), dim3(kCgBlock), 0, s, n, (const float *)b->dev, (const float *)b->dev, w.S);
+   hipLaunchKernelGGL(k_dot, dim3(gn), dim3(kCgBlock), 0, s, n, (const float *)b->dev, (const float *)b->dev, w.P);
+   hipLaunchKernelGGL(k_cg_finish<CgFinDot>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, gn);
    HIPCHK(hipMemcpyAsync(w.host, &w.S->pq, 8, hipMemcpyDeviceToHost, s));
    HIPCHK(hipStreamSynchronize(s));
    const double bnorm = sqrt(w.host[0]);
    if(bnorm == 0.)
    {
-      HIPCHK(hipMemsetAsync(xo->dev, 0, n * 4, s));
+      HIPCHK(hipMemsetAsync(xo->dev, 0, n * (xo->flt == GmlDbl ? 8 : 4), s));
       HIPCHK(hipStreamSynchronize(s));
       *relres = 0.; *iters = 0;
       cleanup();
@@ -760,7 +912,8 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    // one inner PCG solve of A d = rhs32 to relative tol itol, d in w.x
    int total = 0;
    auto pcg = [&](const float *rhs32, double itol, int cap) -> int {
-      GML_B(k_cg_init, dim3(gr), dim3(kCgBlock), 0, s, nrow, rhs32, w.x, w.r, w.p, w.z, w.Minv, w.S);
+      GML_B(k_cg_init, dim3(gr), dim3(kCgBlock), 0, s, nrow, rhs32, w.x, w.r, w.p, w.z, w.Minv, w.P);
+      hipLaunchKernelGGL(k_cg_finish<CgFinInit>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, gr);
       HIPCHK(hipMemcpyAsync(w.host, &w.S->rr0, 8, hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       const double r0 = w.host[0];
@@ -774,10 +927,17 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
          const int chunk = 8;
          for(int c = 0; c < chunk && it < cap; c++, it++)
          {
-            if(SpMV<float, float>(s, A, w.p, w.q) != 1) { (void)hipEventDestroy(ev); return -6; }
-            hipLaunchKernelGGL(k_cg_alpha, dim3(gn), dim3(kCgBlock), 0, s, n, w.p, w.q, w.S);
-            GML_B(k_cg_update, dim3(gr), dim3(kCgBlock), 0, s, nrow, w.x, w.r, w.p, w.q, w.Minv, w.z, w.S);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(nbn), dim3(256), 0, s, n, w.p, w.z, w.S);
+            int t1 = TimedLaunch(g, GmlxBuiltinCgSpMV, [&]() {
+               GML_B(k_cg_spmv, dim3(nbs), dim3(sh.grp), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val,
+                     w.p, w.q, w.P, sh.remap);
+               hipLaunchKernelGGL(k_cg_finish<CgFinAlpha>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, nbs);
+            });
+            int t2 = TimedLaunch(g, GmlxBuiltinCgUpdate, [&]() {
+               GML_B(k_cg_update, dim3(gu), dim3(su.grp), 0, s, nrow, w.x, w.r, w.p, w.q, w.Minv, w.z, w.S, w.P);
+               hipLaunchKernelGGL(k_cg_finish<CgFinBeta>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, gu);
+               hipLaunchKernelGGL(k_cg_pupdate, dim3(Blocks((n + 3) / 4)), dim3(256), 0, s, n, w.p, w.z, w.S);
+            });
+            if(t1 != 1 || t2 != 1) { (void)hipStreamSynchronize(s); (void)hipEventDestroy(ev); return -6; }
          }
          // read back the residual of the previous chunk while this one runs
          if(pending >= 0)
@@ -810,7 +970,8 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
       for(int outer = 0; outer < 40 && total < maxit; outer++)
       {
          GML_B(k_resid64, dim3(gr), dim3(kCgBlock), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val,
-               w.x64, (const float *)b->dev, w.r64, w.S);
+               w.x64, (const float *)b->dev, w.r64, w.P);
+         hipLaunchKernelGGL(k_cg_finish<CgFinResid>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, gr);
          HIPCHK(hipMemcpyAsync(w.host, &w.S->rr0, 8, hipMemcpyDeviceToHost, s));
          HIPCHK(hipStreamSynchronize(s));
          double rn = sqrt(w.host[0]);
@@ -818,7 +979,7 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
          if(rel <= tol) break;
          // correction: A d = r / |r| in FP32
          hipLaunchKernelGGL(k_f64_to_f32, dim3(nbn), dim3(256), 0, s, n, w.r64, w.d32);
-         hipLaunchKernelGGL(k_scale<float>, dim3(nbn), dim3(256), 0, s, n, w.d32, (float)(1.0 / rn));
+         hipLaunchKernelGGL(k_scale<float>, dim3(VecBlocks<float>(n)), dim3(256), 0, s, n, w.d32, (float)(1.0 / rn));
          int it = pcg(w.d32, 1e-4, maxit - total);
          if(it < 0) { cleanup(); return it; }
          total += it;
@@ -831,7 +992,8 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    else
       hipLaunchKernelGGL(k_f64_to_f32, dim3(nbn), dim3(256), 0, s, n, w.x64, (float *)xo->dev);
    GML_B(k_resid64, dim3(gr), dim3(kCgBlock), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val,
-         w.x64, (const float *)b->dev, w.r64, w.S);
+         w.x64, (const float *)b->dev, w.r64, w.P);
+   hipLaunchKernelGGL(k_cg_finish<CgFinResid>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, gr);
    HIPCHK(hipMemcpyAsync(w.host, &w.S->rr0, 8, hipMemcpyDeviceToHost, s));
    if(hipStreamSynchronize(s) != hipSuccess) rc = -6;
    rel = sqrt(w.host[0]) / bnorm;
@@ -849,13 +1011,15 @@ int GmlxDotVec(size_t GmlIdx, int VecA, int VecB, double *res)
    if(!a || !b || a->flt != GmlFlt || b->flt != GmlFlt || a->NmbLin != b->NmbLin || a->b != b->b) return -1;
    if(g->host_only) return -6;
    CgScal *S = (CgScal *)DevAlloc(g, sizeof(CgScal));
-   if(!S) return -5;
-   HIPCHK(hipMemsetAsync(S, 0, sizeof(CgScal), g->stream));
+   CgPart P = {(double *)DevAlloc(g, (size_t)2 * kCgGrid * 8), (unsigned int)kCgGrid};
+   if(!S || !P.p) { DevFree(g, S); DevFree(g, P.p); return -5; }
    long long n = (long long)a->NmbLin * a->b;
    unsigned int gn = std::min<unsigned int>(kCgGrid, Blocks(n, kCgBlock));
-   hipLaunchKernelGGL(k_dot, dim3(gn), dim3(kCgBlock), 0, g->stream, n, (const float *)a->dev, (const float *)b->dev, S);
+   hipLaunchKernelGGL(k_dot, dim3(gn), dim3(kCgBlock), 0, g->stream, n, (const float *)a->dev, (const float *)b->dev, P);
+   hipLaunchKernelGGL(k_cg_finish<CgFinDot>, dim3(1), dim3(kFinBlock), 0, g->stream, S, P, gn);
    bool ok = DevCopy(g, res, &S->pq, 8, 2);
    DevFree(g, S);
+   DevFree(g, P.p);
    return ok ? 1 : -6;
 }
 

@@ -173,6 +173,61 @@ size_t ReduceScratchBytes();
 // block algebra (gml_algebra.hip)
 void FreeAlgebra(GmlCtx *g);
 
+void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
+
+// launch shape of a built-in kernel (work-group size, XCD block remap),
+// process-wide, set by GmlxSetBuiltinShape
+const int kNmbBuiltin = 16;
+struct Shape { int grp = 256, remap = 1; };
+Shape BuiltinShape(int which);
+
+// Block index of this work-group in launch order, or, with remap, remapped
+// so that each of the 8 XCDs (which take dispatched blocks round-robin) runs
+// one contiguous range of blocks and its L2 sees neighbouring data.
+__device__ __forceinline__ unsigned int xcd_block(int remap)
+{
+   unsigned int b = blockIdx.x;
+   if(remap)
+   {
+      unsigned int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+      b = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+   }
+   return b;
+}
+
+// built-in kernel timing: HIP event pairs per launch, per handle and kernel
+struct KernStat {
+   std::vector<LaunchRec> ev;
+   double time = 0.;
+   long long launches = 0;
+};
+KernStat *Stat(GmlCtx *g, int which);
+
+// Record an event pair around launch() on the handle's stream.
+template<class F>
+int TimedLaunch(GmlCtx *g, int which, F launch)
+{
+   KernStat *st = Stat(g, which);
+   LaunchRec r;
+   bool ev = hipEventCreate(&r.beg) == hipSuccess;
+   if(ev && hipEventCreate(&r.end) != hipSuccess) { (void)hipEventDestroy(r.beg); ev = false; }
+   if(ev) (void)hipEventRecord(r.beg, g->stream);
+   launch();
+   if(hipGetLastError() != hipSuccess)
+   {
+      if(ev) { (void)hipEventDestroy(r.beg); (void)hipEventDestroy(r.end); }
+      return -6;
+   }
+   if(ev)
+   {
+      (void)hipEventRecord(r.end, g->stream);
+      st->ev.push_back(r);
+      if(st->ev.size() > 256) HarvestEvents(st->ev, &st->time);
+   }
+   st->launches++;
+   return 1;
+}
+
 // built-in mesh kernels (gml_mesh_kernels.hip, gml_plan.cpp)
 void FreeBuiltinStats(GmlCtx *g);
 SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
@@ -180,6 +235,5 @@ void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 
 // launcher (gml_codegen.cpp)
 int  RunKernel(GmlCtx *g, Kernel *k);
-void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
 
 } // namespace gml

@@ -17,22 +17,16 @@
 
 namespace gml {
 
-struct KernStat {
-   std::vector<LaunchRec> ev;
-   double time = 0.;
-   long long launches = 0;
-};
-
 // per-handle timing records of the built-in kernels (index = GmlxBuiltin*)
 static std::mutex g_stat_mtx;
 static std::vector<std::pair<GmlCtx *, KernStat *>> g_stats;
 
-static KernStat *Stat(GmlCtx *g, int which)
+KernStat *Stat(GmlCtx *g, int which)
 {
    std::lock_guard<std::mutex> lk(g_stat_mtx);
    for(auto &p : g_stats)
       if(p.first == g) return &p.second[which];
-   KernStat *s = new KernStat[8];
+   KernStat *s = new KernStat[kNmbBuiltin];
    g_stats.push_back({g, s});
    return &s[which];
 }
@@ -43,7 +37,7 @@ void FreeBuiltinStats(GmlCtx *g)
    for(size_t i = 0; i < g_stats.size(); i++)
       if(g_stats[i].first == g)
       {
-         for(int w = 0; w < 8; w++)
+         for(int w = 0; w < kNmbBuiltin; w++)
             for(auto &e : g_stats[i].second[w].ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
          delete[] g_stats[i].second;
          g_stats.erase(g_stats.begin() + i);
@@ -56,17 +50,6 @@ void FreeBuiltinStats(GmlCtx *g)
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ unsigned int xcd_block(int remap)
-{
-   unsigned int b = blockIdx.x;
-   if(remap)
-   {
-      unsigned int nb = gridDim.x, q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-      b = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-   }
-   return b;
-}
 
 __device__ __forceinline__ v4i ld_nt(const v4i *p) { return __builtin_nontemporal_load(p); }
 
@@ -272,28 +255,10 @@ __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned i
    xn[v] = p;
 }
 
-struct Shape { int grp = 256, remap = 1; };
-static Shape g_shape[8];
-static int g_smooth_mode = 2;
+static Shape g_shape[kNmbBuiltin];
 
-template<class F>
-static int TimedLaunch(GmlCtx *g, int which, F launch)
-{
-   KernStat *st = Stat(g, which);
-   LaunchRec r;
-   bool ev = hipEventCreate(&r.beg) == hipSuccess && hipEventCreate(&r.end) == hipSuccess;
-   if(ev) (void)hipEventRecord(r.beg, g->stream);
-   launch();
-   if(hipGetLastError() != hipSuccess) return -6;
-   if(ev)
-   {
-      (void)hipEventRecord(r.end, g->stream);
-      st->ev.push_back(r);
-      if(st->ev.size() > 256) HarvestEvents(st->ev, &st->time);
-   }
-   st->launches++;
-   return 1;
-}
+Shape BuiltinShape(int which) { return g_shape[which]; }
+static int g_smooth_mode = 2;
 
 static bool MeshReady(GmlCtx *g)
 {
@@ -458,8 +423,8 @@ int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
 
 int GmlxSetBuiltinShape(int which, int GrpSiz, int Remap)
 {
-   if(which < 0 || which >= 8 || GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
-   if(which == GmlxBuiltinSweep && GrpSiz > 256) return 0;
+   if(which < 0 || which > GmlxBuiltinCgUpdate || GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
+   if(which >= GmlxBuiltinSweep && GrpSiz > 256) return 0;
    g_shape[which].grp = GrpSiz;
    g_shape[which].remap = Remap ? 1 : 0;
    return 1;
@@ -468,7 +433,7 @@ int GmlxSetBuiltinShape(int which, int GrpSiz, int Remap)
 double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches)
 {
    GML_CTX(g, GmlIdx);
-   if(which < 0 || which >= 8 || g->host_only) return -1.;
+   if(which < 0 || which >= kNmbBuiltin || g->host_only) return -1.;
    HIPCHK(hipStreamSynchronize(g->stream));
    KernStat *st = Stat(g, which);
    HarvestEvents(st->ev, &st->time);

@@ -7,9 +7,11 @@
 // host and drops the last partial group, the whole vector is reduced on the
 // GPU: each thread streams float4 loads, min/max stay exact in FP32 and the
 // sums accumulate in FP64; waves combine with cross-lane shuffles, work-groups
-// through LDS, and the last work-group to finish (atomic ticket) combines the
-// per-group partials and re-arms the ticket for the next call.  Only the
-// final double crosses PCIe.
+// through LDS and store one partial each, and a one-work-group kernel
+// combines the partials in block order (deterministic, and free of the
+// per-work-group device-scope fence a last-block ticket would need, which on
+// CDNA writes back and invalidates the XCD's L2).  Only the final double
+// crosses PCIe.
 
 #include "gml_internal.h"
 
@@ -19,9 +21,8 @@ static const int kRedBlock = 256;
 static const int kRedMaxGrid = 16384;
 
 struct RedScratch {
-   double       part[kRedMaxGrid];
-   double       result;
-   unsigned int ticket;
+   double part[kRedMaxGrid];
+   double result;
 };
 
 size_t ReduceScratchBytes() { return sizeof(RedScratch) + 256; }
@@ -70,7 +71,6 @@ template<int OP>
 __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ x, long long n, RedScratch *__restrict__ s)
 {
    __shared__ double lds[kRedBlock / 64];
-   __shared__ bool last;
    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    const long long nth = (long long)gridDim.x * blockDim.x;
    double acc = red_init<OP>();
@@ -139,28 +139,32 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ 
    }
 
    acc = block_reduce<OP>(acc, lds);
-   if(threadIdx.x == 0)
-   {
-      s->part[blockIdx.x] = acc;
-      __threadfence();
-      unsigned int t = atomicAdd(&s->ticket, 1u);
-      last = (t == gridDim.x - 1);
-   }
-   __syncthreads();
-   if(!last) return;
+   if(threadIdx.x == 0) s->part[blockIdx.x] = acc;
+}
 
-   // last work-group: combine the partials, publish, re-arm the ticket
-   __threadfence();
-   double v = red_init<OP>();
-   for(unsigned int i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-      v = red_comb<OP>(v, ((volatile double *)s->part)[i]);
-   __syncthreads();
-   v = block_reduce<OP>(v, lds);
-   if(threadIdx.x == 0)
+// one work-group of kFinBlock threads: combine the np partials in a fixed
+// order, eight independent loads in flight per thread
+static const int kFinBlock = 1024;
+
+template<int OP>
+__global__ void __launch_bounds__(kFinBlock) k_reduce_finish(RedScratch *__restrict__ s, int np)
+{
+   __shared__ double lds[kFinBlock / 64];
+   double a = red_init<OP>();
+   for(int base = 0; base < np; base += 8 * kFinBlock)
    {
-      s->result = v;
-      atomicExch(&s->ticket, 0u);
+      double v[8];
+#pragma unroll
+      for(int u = 0; u < 8; u++)
+      {
+         const int i = base + u * kFinBlock + (int)threadIdx.x;
+         v[u] = i < np ? s->part[i] : red_init<OP>();
+      }
+#pragma unroll
+      for(int u = 0; u < 8; u++) a = red_comb<OP>(a, v[u]);
    }
+   a = block_reduce<OP>(a, lds);
+   if(threadIdx.x == 0) s->result = a;
 }
 
 static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s)
@@ -168,17 +172,20 @@ static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, Red
    // about four float4 per thread, up to 16384 work-groups
    long long want = (n / 4 + kRedBlock * 4 - 1) / (kRedBlock * 4);
    int grid = (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
+#define GML_RED(O) do { hipLaunchKernelGGL(k_reduce<O>, dim3(grid), dim3(kRedBlock), 0, st, x, n, s); \
+                         hipLaunchKernelGGL(k_reduce_finish<O>, dim3(1), dim3(kFinBlock), 0, st, s, grid); } while(0)
    switch(op)
    {
-      case GmlMin:  hipLaunchKernelGGL(k_reduce<GmlMin>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlMax:  hipLaunchKernelGGL(k_reduce<GmlMax>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlSum:  hipLaunchKernelGGL(k_reduce<GmlSum>,  dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlL0:   hipLaunchKernelGGL(k_reduce<GmlL0>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlL1:   hipLaunchKernelGGL(k_reduce<GmlL1>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlL2:   hipLaunchKernelGGL(k_reduce<GmlL2>,   dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
-      case GmlLinf: hipLaunchKernelGGL(k_reduce<GmlLinf>, dim3(grid), dim3(kRedBlock), 0, st, x, n, s); break;
+      case GmlMin:  GML_RED(GmlMin);  break;
+      case GmlMax:  GML_RED(GmlMax);  break;
+      case GmlSum:  GML_RED(GmlSum);  break;
+      case GmlL0:   GML_RED(GmlL0);   break;
+      case GmlL1:   GML_RED(GmlL1);   break;
+      case GmlL2:   GML_RED(GmlL2);   break;
+      case GmlLinf: GML_RED(GmlLinf); break;
       default: return -3;
    }
+#undef GML_RED
    return hipGetLastError() == hipSuccess ? 1 : -6;
 }
 

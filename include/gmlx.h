@@ -101,7 +101,7 @@ int    GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol
    run those workloads at scale; launches are asynchronous on the handle's
    stream. */
 enum gmlx_builtin { GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather,
-                    GmlxBuiltinSweep };
+                    GmlxBuiltinSweep, GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate };
 
 /* mean-ratio quality of every tet into QalIdx (a GmlFlt tet solution) */
 int    GmlxTetQuality(size_t GmlIdx, int QalIdx);
@@ -126,9 +126,13 @@ int    GmlxSetSmoothMode(int mode);
    sum of tile vertices, sum of tile tets, max vertices per tile, max tets per
    tile, plan bytes, total 1-ring slots (operator form), mode} */
 int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
-/* launch shape of a built-in kernel for every handle of the process */
+/* launch shape of a built-in kernel for every handle of the process: GrpSiz a
+   multiple of 64, up to 1024 (quality, ball-min, scatter, gather) or 256
+   (sweep, SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
 int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);
-/* summed device time (s) and launch count of a built-in kernel on a handle */
+/* summed device time (s) and launch count of a built-in kernel on a handle
+   (GmlxBuiltinSpMV: the block SpMV of GmlMultMatVec; GmlxBuiltinCgSpMV and
+   GmlxBuiltinCgUpdate: the two kernels of a GmlxSolveCG iteration) */
 double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches);
 
 /* --- synthetic meshes ------------------------------------------------------ */

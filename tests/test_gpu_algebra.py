@@ -77,8 +77,9 @@ def test_spmv_fp64_values_and_errors():
     assert np.allclose(y, y64, rtol=1e-12, atol=1e-12)
 
 
-def test_vector_ops():
-    n, b = 1000, 3
+@pytest.mark.parametrize("n", [1000, 1001])
+def test_vector_ops(n):
+    b = 3
     rng = np.random.default_rng(1)
     u, v, w = (rng.normal(size=n * b).astype(np.float32) for _ in range(3))
     D = rng.normal(size=(n, b, b)).astype(np.float32)
@@ -103,6 +104,30 @@ def test_vector_ops():
         d = C.c_double()
         assert L.GmlxDotVec(h, iv, iw, C.byref(d)) == 1
         assert d.value == pytest.approx(float(np.dot(v.astype(np.float64), w.astype(np.float64))), rel=1e-12)
+
+
+@pytest.mark.parametrize("n,b", [(1, 1), (7, 3), (513, 3), (1000, 2)])
+def test_vector_ops_fp64_tails(n, b):
+    """16-byte vector kernels: sizes whose entry count is odd or not a
+    multiple of the vector width take the scalar tail."""
+    rng = np.random.default_rng(n)
+    u, v, w = (rng.normal(size=n * b) for _ in range(3))
+    with G.Gml() as g:
+        L, h = g.L, g.h
+        iu, iv, iw = (new_vec(g, n, b, a, G.GmlDbl) for a in (u, v, w))
+        iy = new_vec(g, n, b, None, G.GmlDbl)
+        assert L.GmlAddVec3(h, iu, iv, iw, iy) == 1
+        assert np.array_equal(get_vec(g, iy, n, b, np.float64), (u + v) + w)
+        f = C.c_double(0.75)
+        assert L.GmlScaleVec(h, iu, C.byref(f)) == 1
+        assert np.array_equal(get_vec(g, iu, n, b, np.float64), u * 0.75)
+        fu, fv, fw = (a.astype(np.float32) for a in (u, v, w))
+        ju, jv, jw = (new_vec(g, n, b, a) for a in (fu, fv, fw))
+        jy = new_vec(g, n, b)
+        assert L.GmlAddVec3(h, ju, jv, jw, jy) == 1
+        assert np.array_equal(get_vec(g, jy, n, b), (fu + fv) + fw)
+        assert L.GmlScaleVec(h, ju, C.byref(f)) == 1
+        assert np.array_equal(get_vec(g, ju, n, b), fu * np.float32(0.75))
 
 
 @pytest.mark.parametrize("refine,tol", [(0, 1e-6), (1, 1e-10)])
