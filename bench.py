@@ -21,7 +21,18 @@ weighting.  A step is one GmlMultMatVec (y = A x), counted as 40 B per block
 FP32 CG solve and an FP32 CG with FP64 residual refinement to 1e-10, each
 checked against the oracle's FP64 product b - A x on the host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg]
+Third workload (--workload batch): 512 Kuhn cube meshes with cells per side
+drawn log-uniform in [8, 96] (seeded; jitter +-0.3h), kept on the device as
+one batch.  A step is one GmlxBatchQuality: per-tet mean-ratio quality,
+per-vertex ball minimum and per-mesh min / mean of every mesh, one
+persistent kernel replayed from a HIP graph, launch shape picked by the
+autotuner (GmlxBatchTune).  Counted as 36 B per tet (connectivity, 4 ball
+slots, quality out) + 20 B per vertex (coordinates, ball minimum out).  The
+plain per-mesh path (one handle per mesh; GmlxTetQuality, GmlxBallMin and two
+GmlReduceVector per mesh) is timed beside it on the same meshes, and the
+per-mesh results of both are compared.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -47,7 +58,8 @@ def body(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg"])
+    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch"])
+    ap.add_argument("--nmesh", type=int, default=512, help="batch workload: number of meshes")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -70,6 +82,8 @@ def main():
         sys.exit(2)
     if args.workload == "cg":
         run_cg(args)
+    elif args.workload == "batch":
+        run_batch(args)
     else:
         run_smoothing(args)
 
@@ -422,6 +436,163 @@ def run_cg(args):
         "stream_probe_GBs": probe,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+def batch_sizes(nmesh, seed=2024):
+    """Cells per side of the batch's cube meshes: log-uniform in [8, 96]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(8), np.log(96), nmesh))).astype(int)
+
+
+def run_batch(args):
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    import gmlib_amd as G
+
+    sizes = batch_sizes(args.nmesh)
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        ms = list(ex.map(lambda i: G.cube_mesh(int(sizes[i]), 0.3, 7000 + i), range(len(sizes))))
+    nver, ntet, crd, tet = G.concat_meshes(ms)
+    V, T, M = int(nver.sum()), int(ntet.sum()), len(ms)
+    log("batch: %d meshes, n in [%d, %d], V=%d T=%d, generated in %.1fs" % (M, sizes.min(), sizes.max(), V, T, time.time() - t0))
+    alg = 36 * T + 20 * V
+
+    g = G.Gml(0)
+    L, h = g.L, g.h
+    t0 = time.time()
+    b = L.GmlxNewBatch(h, M, G.ptr(nver), G.ptr(ntet), G.ptr(crd), G.ptr(tet))
+    if b <= 0:
+        raise RuntimeError("GmlxNewBatch failed")
+    t_new = time.time() - t0
+    t0 = time.time()
+    tv, pad = C.c_int(), C.c_int()
+    if L.GmlxBatchTune(h, b, C.byref(tv), C.byref(pad)) != 1:
+        raise RuntimeError("GmlxBatchTune failed")
+    t_tune = time.time() - t0
+    tab = (C.c_double * 27)()
+    nt = L.GmlxGetBatchTuning(h, b, tab)
+    tuning = {"%d/%d" % (tab[3 * i], tab[3 * i + 1]): round(tab[3 * i + 2], 1) for i in range(nt)}
+    info = (C.c_longlong * 12)()
+    L.GmlxGetBatchInfo(h, b, info)
+    log("batch upload %.1fs, tune + plan + graph %.1fs: TV %d, Pad %d; tuner (TV/Pad: us on the sample) %s; "
+        "%d tiles, tile tets / T %.3f, tile vertices / V %.3f, plan %.2f GB, %d work-groups"
+        % (t_new, t_tune, tv.value, pad.value, tuning, info[3], info[7] / T, info[8] / V, info[6] / 1e9, info[9]))
+
+    mn = np.empty(M, np.float32)
+    mean = np.empty(M, np.float64)
+    for _ in range(args.warmup):
+        L.GmlxBatchQuality(h, b, G.ptr(mn), G.ptr(mean))
+    cnt = C.c_longlong()
+    k0 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinBatch, C.byref(cnt))
+    c0 = cnt.value
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if L.GmlxBatchQuality(h, b, G.ptr(mn), G.ptr(mean)) != 1:
+            raise RuntimeError("GmlxBatchQuality failed")
+    dt = (time.perf_counter() - t0) / args.steps
+    kavg = (L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinBatch, C.byref(cnt)) - k0) / (cnt.value - c0)
+    kgbs = alg / kavg / 1e9
+    log("batched: %.3f ms per step (graph %.3f ms), %.0f meshes/s, %.2f Gtets/s, %.1f GB/s algorithmic (%.1f%% of peak)"
+        % (dt * 1e3, kavg * 1e3, M / dt, T / dt / 1e9, kgbs, kgbs / HBM_PEAK_GBS * 100))
+
+    # plain per-mesh path on the same meshes
+    t0 = time.time()
+    hs = []
+    for crd_m, ref_m, tet_m in ms:
+        gm = G.Gml(0)
+        gm.set_mesh(crd_m, ref_m, tet_m)
+        q = gm.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        vq = gm.solution(G.GmlVertices, 1, G.GmlFlt, "vqal")
+        hs.append((gm, q, vq))
+    red = C.c_double()
+
+    def plain_step():
+        out = np.empty((M, 2))
+        for i, (gm, q, vq) in enumerate(hs):
+            if gm.L.GmlxTetQuality(gm.h, q) != 1 or gm.L.GmlxBallMin(gm.h, q, vq) != 1:
+                raise RuntimeError("plain path launch failed")
+            gm.L.GmlReduceVector(gm.h, q, G.GmlMin, C.byref(red))
+            out[i, 0] = red.value
+            gm.L.GmlReduceVector(gm.h, q, G.GmlSum, C.byref(red))
+            out[i, 1] = red.value / len(ms[i][2])
+        return out
+
+    plain_step()
+    log("plain path setup (%d handles, upload, ball build): %.1fs" % (M, time.time() - t0))
+    for _ in range(max(1, args.warmup - 1)):
+        ref_out = plain_step()
+    psteps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(psteps):
+        ref_out = plain_step()
+    pdt = (time.perf_counter() - t0) / psteps
+    min_equal = bool(np.array_equal(ref_out[:, 0].astype(np.float32), mn))
+    mean_err = float(np.max(np.abs(ref_out[:, 1] - mean) / np.abs(mean)))
+    log("plain: %.3f ms per step, %.0f meshes/s; batched / plain = %.2fx; per-mesh min equal %s, mean max rel diff %.2e"
+        % (pdt * 1e3, M / pdt, pdt / dt, min_equal, mean_err))
+    for gm, _, _ in hs:
+        gm.close()
+    if not min_equal or mean_err > 1e-12:
+        raise RuntimeError("batched and plain per-mesh results differ")
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_batch512.json")
+    if args.nmesh == 512 and os.path.exists(tf):
+        traffic = json.load(open(tf)).get("batch_quality")
+    out = {
+        "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
+        "value": round(alg / dt / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered Kuhn cube meshes, cells per side log-uniform in [8, 96])",
+        "config": {"workload": "batched tet quality + ball min + per-mesh min/mean, %d cube meshes (%d vertices, "
+                               "%d tets), one persistent kernel from a HIP graph" % (M, V, T),
+                   "meshes": M, "vertices": V, "tets": T, "bytes_per_step": alg, "tile_vertices": tv.value,
+                   "ball_padding": pad.value, "tuner_us": tuning, "setup_s": {"upload": round(t_new, 2),
+                                                                              "tune_plan_graph": round(t_tune, 2)}},
+        "meshes_per_s": round(M / dt, 1),
+        "plain_per_mesh": {"ms_per_step": round(pdt * 1e3, 3), "meshes_per_s": round(M / pdt, 1),
+                           "speedup_batched": round(pdt / dt, 2), "min_equal": min_equal,
+                           "mean_max_rel_diff": mean_err},
+        "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_batch_quality<%d> (graph: + k_batch_finish)" % pad.value,
+                     "bytes_per_launch": alg, "avg_us": round(kavg * 1e6, 2)},
+        "cpu_baseline": None,
+    }
+    if args.cpu_seconds > 0:
+        from tests import oracle_lib as O
+        OL = O.lib()
+        t0 = time.perf_counter()
+        done_t = done_v = 0
+        i = 0
+        while i < M and (i < 8 or time.perf_counter() - t0 < args.cpu_seconds):
+            crd_m, ref_m, tet_m = ms[i]
+            qm = O.tet_quality32(tet_m, crd_m)
+            deg, ball, _ = O.vertex_ball(tet_m, len(crd_m), 64)
+            O.ball_min(deg, ball, qm)
+            O.reduce(G.GmlMin, qm)
+            O.reduce(G.GmlSum, qm)
+            done_t += len(tet_m)
+            done_v += len(crd_m)
+            i += 1
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round((36 * done_t + 20 * done_v) / ct / 1e9, 2), "unit": "GB/s",
+                               "cores": OL.orc_num_threads(), "kind": "port",
+                               "sample": "the first %d meshes of the batch (%d tets): oracle FP32 quality, ball build "
+                                         "+ ball min, min and sum reductions, C + OpenMP, %.2f s" % (i, done_t, ct)}
+        log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out), flush=True)
     g.close()
 

@@ -26,7 +26,8 @@ GmlRefFlag, GmlReadMode, GmlWriteMode, GmlVoyeurs, GmlManual = 1, 2, 4, 8, 16
  GmlByt, GmlByt2, GmlByt4, GmlByt8, GmlByt16, GmlMaxOclTyp) = range(21)
 GmlMin, GmlMax, GmlSum, GmlL0, GmlL1, GmlL2, GmlLinf, GmlMaxRed = range(8)
 (GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather, GmlxBuiltinSweep,
- GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate) = range(8)
+ GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate, GmlxBuiltinBatch) = range(9)
+GmlxMaxBatch = 16
 
 _S = C.c_size_t
 _I = C.c_int
@@ -103,6 +104,14 @@ _SIGS = {
     "GmlxGetBuiltinRunTime": (_D, [_S, _I, C.POINTER(C.c_longlong)]),
     "GmlxSetSmoothMode": (_I, [_I]),
     "GmlxGetSweepPlanInfo": (_I, [_S, C.POINTER(C.c_longlong)]),
+    "GmlxNewBatch": (_I, [_S, _I, _P, _P, _P, _P]),
+    "GmlxBatchTune": (_I, [_S, _I, _IP, _IP]),
+    "GmlxBatchSetShape": (_I, [_S, _I, _I, _I]),
+    "GmlxBatchQuality": (_I, [_S, _I, _P, _P]),
+    "GmlxGetBatchField": (_I, [_S, _I, _P, _P]),
+    "GmlxGetBatchInfo": (_I, [_S, _I, C.POINTER(C.c_longlong)]),
+    "GmlxGetBatchTuning": (_I, [_S, _I, _DP]),
+    "GmlxFreeBatch": (_I, [_S, _I]),
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)
@@ -170,6 +179,17 @@ def cube_mesh(n, jitter=0.2, seed=1, hilbert=True):
     if not lib().GmlxCubeMesh(n, float(jitter), int(seed), 1 if hilbert else 0, ptr(crd), ptr(ref), ptr(tet)):
         raise RuntimeError("GmlxCubeMesh failed")
     return crd, ref, tet
+
+
+def concat_meshes(meshes):
+    """(nver int32[M], ntet int32[M], crd float32[sum V, 4], tet int32[sum T, 4])
+    of a list of (crd, ref, tet) meshes, tets keeping their local indices:
+    the input layout of GmlxNewBatch."""
+    nver = np.array([len(m[0]) for m in meshes], np.int32)
+    ntet = np.array([len(m[2]) for m in meshes], np.int32)
+    crd = np.ascontiguousarray(np.concatenate([m[0] for m in meshes]), np.float32)
+    tet = np.ascontiguousarray(np.concatenate([m[2] for m in meshes]), np.int32)
+    return nver, ntet, crd, tet
 
 
 class Gml:

@@ -26,6 +26,7 @@ SOURCES = [
     "gml_algebra.hip",
     "gml_mesh_kernels.hip",
     "gml_plan.cpp",
+    "gml_batch.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

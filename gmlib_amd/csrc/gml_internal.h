@@ -85,6 +85,7 @@ struct Kernel {
 };
 
 struct Matrix;
+struct Batch;
 struct Vector;
 
 // Tile plan of the fused smoothing sweep (gml_plan.cpp)
@@ -137,6 +138,8 @@ struct GmlCtx {
    SweepPlan   *plan = nullptr;             // exact fused form
    SweepPlan   *oplan = nullptr;            // operator form
    void        *xalt = nullptr;
+   // batches of meshes processed together (gml_batch.hip)
+   Batch       *bat[GmlxMaxBatch + 1] = {nullptr};
    // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
    // generated and compiled but never launched
    bool         host_only = false;
@@ -228,8 +231,9 @@ int TimedLaunch(GmlCtx *g, int which, F launch)
    return 1;
 }
 
-// built-in mesh kernels (gml_mesh_kernels.hip, gml_plan.cpp)
+// built-in mesh kernels (gml_mesh_kernels.hip, gml_plan.cpp, gml_batch.hip)
 void FreeBuiltinStats(GmlCtx *g);
+void FreeBatches(GmlCtx *g);
 SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 

@@ -311,6 +311,7 @@ void GmlStop(size_t GmlIdx)
    FreeSweepPlan(g, g->oplan);
    g->plan = g->oplan = nullptr;
    if(g->xalt) { DevFree(g, g->xalt); g->xalt = nullptr; }
+   FreeBatches(g);
    FreeAlgebra(g);
    for(int i = 1; i <= GmlMaxDat; i++)
       FreeDataSlot(g, i);

@@ -101,7 +101,8 @@ int    GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol
    run those workloads at scale; launches are asynchronous on the handle's
    stream. */
 enum gmlx_builtin { GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather,
-                    GmlxBuiltinSweep, GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate };
+                    GmlxBuiltinSweep, GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate,
+                    GmlxBuiltinBatch };
 
 /* mean-ratio quality of every tet into QalIdx (a GmlFlt tet solution) */
 int    GmlxTetQuality(size_t GmlIdx, int QalIdx);
@@ -145,6 +146,37 @@ double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches);
    Hilbert curve, 0 keeps lexicographic order.  Returns 1. */
 int    GmlxCubeMesh(int n, double jitter, unsigned long long seed, int order,
                     float *crd, int *ref, int *tet);
+
+/* ---- batches of meshes: many independent tet meshes kept on the device and
+   processed by one persistent kernel replayed from a HIP graph.  A run
+   computes the mean-ratio quality of every tet (as GmlxTetQuality), its
+   minimum over every vertex ball (as GmlxBallMin) and, per mesh, the
+   minimum and mean quality. ---- */
+#define GmlxMaxBatch 16
+/* crd: 4 floats per vertex (x, y, z, unused), meshes concatenated; tet: 4
+   ints per tet, vertex indices local to their mesh, meshes concatenated.
+   Returns a batch index >= 1, or 0. */
+int    GmlxNewBatch(size_t GmlIdx, int NmbMsh, const int *NmbVer, const int *NmbTet,
+                    const float *crd, const int *tet);
+/* autotuner: time every (tile vertices TV in {256, 512, 1024}, ball padding
+   Pad in {1, 2, 4}) candidate on a sample of about 1/16 of the meshes, keep
+   the fastest and build its plan and graph (done by the first
+   GmlxBatchQuality when not called before); *TV, *Pad receive the choice */
+int    GmlxBatchTune(size_t GmlIdx, int BatIdx, int *TV, int *Pad);
+/* use a given shape instead (TV a multiple of 64 in 64..1024, Pad 1, 2 or 4) */
+int    GmlxBatchSetShape(size_t GmlIdx, int BatIdx, int TV, int Pad);
+/* one run; MshMin / MshMean (NmbMsh entries each, either may be NULL)
+   receive each mesh's minimum and mean tet quality */
+int    GmlxBatchQuality(size_t GmlIdx, int BatIdx, float *MshMin, double *MshMean);
+/* per-tet quality and per-vertex ball minimum of the last run, in batch
+   order (either may be NULL) */
+int    GmlxGetBatchField(size_t GmlIdx, int BatIdx, float *TetQal, float *VerMin);
+/* 12 values: meshes, V, T, tiles, TV, Pad, plan bytes, tile tets, tile
+   vertices, work-groups, graph ready, ball slabs */
+int    GmlxGetBatchInfo(size_t GmlIdx, int BatIdx, long long *info);
+/* tuner table: (TV, Pad, microseconds per sample run) per candidate; returns the count */
+int    GmlxGetBatchTuning(size_t GmlIdx, int BatIdx, double *tab);
+int    GmlxFreeBatch(size_t GmlIdx, int BatIdx);
 
 #ifdef __cplusplus
 }
