@@ -28,6 +28,7 @@ GmlMin, GmlMax, GmlSum, GmlL0, GmlL1, GmlL2, GmlLinf, GmlMaxRed = range(8)
 (GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, GmlxBuiltinGather, GmlxBuiltinSweep,
  GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate, GmlxBuiltinBatch) = range(9)
 GmlxMaxBatch = 16
+GmlxJobQuality, GmlxJobSmooth, GmlxJobCG = range(3)
 
 _S = C.c_size_t
 _I = C.c_int
@@ -112,6 +113,12 @@ _SIGS = {
     "GmlxGetBatchInfo": (_I, [_S, _I, C.POINTER(C.c_longlong)]),
     "GmlxGetBatchTuning": (_I, [_S, _I, _DP]),
     "GmlxFreeBatch": (_I, [_S, _I]),
+    "GmlxEdgeLaplacian3": (C.c_longlong, [_I, _I, _P, C.c_ulonglong, _D, _P, _P, _P, C.c_longlong]),
+    "GmlxServeStart": (_S, [_I, _D, _I, _I]),
+    "GmlxServeSubmit": (C.c_longlong, [_S, _I, _I, C.c_ulonglong, _D]),
+    "GmlxServeWait": (_I, [_S, C.c_longlong, _DP, _DP]),
+    "GmlxServeInfo": (_I, [_S, _DP]),
+    "GmlxServeStop": (None, [_S]),
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)
@@ -179,6 +186,20 @@ def cube_mesh(n, jitter=0.2, seed=1, hilbert=True):
     if not lib().GmlxCubeMesh(n, float(jitter), int(seed), 1 if hilbert else 0, ptr(crd), ptr(ref), ptr(tet)):
         raise RuntimeError("GmlxCubeMesh failed")
     return crd, ref, tet
+
+
+def edge_laplacian3(nver, tet, seed, shift=0.01):
+    """(rowptr, col, val[nnzb, 3, 3]) of GmlxEdgeLaplacian3 for a tet mesh."""
+    tet = np.ascontiguousarray(tet, np.int32)
+    rowptr = np.empty(nver + 1, np.int32)
+    nnz = lib().GmlxEdgeLaplacian3(nver, len(tet), ptr(tet), seed, shift, ptr(rowptr), None, None, 0)
+    if nnz < 0:
+        raise ValueError("GmlxEdgeLaplacian3: invalid mesh")
+    col = np.empty(nnz, np.int32)
+    val = np.empty((nnz, 3, 3), np.float32)
+    if lib().GmlxEdgeLaplacian3(nver, len(tet), ptr(tet), seed, shift, ptr(rowptr), ptr(col), ptr(val), nnz) != nnz:
+        raise RuntimeError("GmlxEdgeLaplacian3 failed")
+    return rowptr, col, val
 
 
 def concat_meshes(meshes):

@@ -27,6 +27,8 @@ SOURCES = [
     "gml_mesh_kernels.hip",
     "gml_plan.cpp",
     "gml_batch.hip",
+    "gml_laplacian.cpp",
+    "gml_serve.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

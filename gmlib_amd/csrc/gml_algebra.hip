@@ -26,6 +26,8 @@
 
 namespace gml {
 
+struct CgWork;
+
 struct Matrix {
    int NmbLin = 0, NmbBlk = 0, b = 0, flt = GmlFlt;
    long long nslab = 0;
@@ -33,7 +35,10 @@ struct Matrix {
    unsigned int *off = nullptr;
    void *val = nullptr;
    double MemAcc = 0., FltOpp = 0.;
+   CgWork *cg = nullptr;        // CG workspace, kept from the first solve on
 };
+
+static void FreeCgWork(GmlCtx *g, CgWork *w);
 
 struct Vector {
    int NmbLin = 0, b = 0, flt = GmlFlt;
@@ -46,6 +51,7 @@ void FreeAlgebra(GmlCtx *g)
    for(int i = 0; i <= GmlMaxMat; i++)
       if(g->mat[i])
       {
+         FreeCgWork(g, g->mat[i]->cg);
          DevFree(g, g->mat[i]->deg); DevFree(g, g->mat[i]->col);
          DevFree(g, g->mat[i]->off); DevFree(g, g->mat[i]->val);
          delete g->mat[i];
@@ -557,13 +563,51 @@ __global__ void __launch_bounds__(kCgBlock) k_dot(long long n, const float *__re
    block_sums(acc, 0., P, false);
 }
 
+// CG workspace of a matrix: allocated by its first solve and kept (no
+// allocation or free, hence no device-wide synchronisation, in later
+// solves); the block-Jacobi inverse is computed once
 struct CgWork {
-   float *x, *r, *p, *q, *z, *Minv, *d32;
-   double *x64, *r64;
-   CgScal *S;
-   CgPart P;
-   double *host;   // pinned: rr readback
+   float *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *Minv = nullptr, *d32 = nullptr;
+   double *x64 = nullptr, *r64 = nullptr;
+   CgScal *S = nullptr;
+   CgPart P = {nullptr, 0};
+   double *host = nullptr;   // pinned: rr readback
+   bool minv_ok = false;
+   size_t bytes = 0;
 };
+
+static void FreeCgWork(GmlCtx *g, CgWork *w)
+{
+   if(!w) return;
+   DevFree(g, w->x); DevFree(g, w->r); DevFree(g, w->p); DevFree(g, w->q); DevFree(g, w->z);
+   DevFree(g, w->d32); DevFree(g, w->Minv); DevFree(g, w->x64); DevFree(g, w->r64); DevFree(g, w->S);
+   DevFree(g, w->P.p);
+   if(w->host) (void)hipHostFree(w->host);
+   g->MemSiz -= w->bytes;
+   delete w;
+}
+
+static CgWork *CgWorkspace(GmlCtx *g, Matrix *A)
+{
+   if(A->cg) return A->cg;
+   const int nrow = A->NmbLin, B = A->b;
+   const long long n = (long long)nrow * B;
+   CgWork *w = new CgWork();
+   w->P.np = std::max<unsigned int>(kCgGrid, Blocks(nrow, 64));
+   bool ok = true;
+   auto al = [&](size_t by) { void *p = DevAlloc(g, by); if(!p) ok = false; w->bytes += by; return p; };
+   w->x = (float *)al(n * 4); w->r = (float *)al(n * 4); w->p = (float *)al(n * 4);
+   w->q = (float *)al(n * 4); w->z = (float *)al(n * 4); w->d32 = (float *)al(n * 4);
+   w->Minv = (float *)al((size_t)nrow * B * B * 4);
+   w->x64 = (double *)al(n * 8); w->r64 = (double *)al(n * 8);
+   w->S = (CgScal *)al(sizeof(CgScal));
+   w->P.p = (double *)al((size_t)2 * w->P.np * 8);
+   if(ok && hipHostMalloc((void **)&w->host, 64, hipHostMallocDefault) != hipSuccess) ok = false;
+   g->MemSiz += w->bytes;
+   if(!ok) { FreeCgWork(g, w); return nullptr; }
+   A->cg = w;
+   return w;
+}
 
 } // namespace gml
 
@@ -835,6 +879,7 @@ int GmlxFreeMatrix(size_t GmlIdx, int MatIdx)
    GML_CTX(g, GmlIdx);
    Matrix *m = Mat(g, MatIdx);
    if(!m) return 0;
+   FreeCgWork(g, m->cg);
    DevFree(g, m->deg); DevFree(g, m->col); DevFree(g, m->off); DevFree(g, m->val);
    delete m;
    g->mat[MatIdx] = nullptr;
@@ -866,24 +911,10 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    hipStream_t s = g->stream;
    const Shape sh = BuiltinShape(GmlxBuiltinCgSpMV), su = BuiltinShape(GmlxBuiltinCgUpdate);
    const unsigned int nbs = Blocks(nrow, sh.grp);
-   CgWork w = {};
-   w.P.np = std::max<unsigned int>(kCgGrid, nbs);
-   bool ok = true;
-   auto al = [&](size_t by) { void *p = DevAlloc(g, by); if(!p) ok = false; return p; };
-   w.x = (float *)al(n * 4); w.r = (float *)al(n * 4); w.p = (float *)al(n * 4);
-   w.q = (float *)al(n * 4); w.z = (float *)al(n * 4); w.d32 = (float *)al(n * 4);
-   w.Minv = (float *)al((size_t)nrow * B * B * 4);
-   w.x64 = (double *)al(n * 8); w.r64 = (double *)al(n * 8);
-   w.S = (CgScal *)al(sizeof(CgScal));
-   w.P.p = (double *)al((size_t)2 * w.P.np * 8);
-   if(ok && hipHostMalloc((void **)&w.host, 64, hipHostMallocDefault) != hipSuccess) ok = false;
-   auto cleanup = [&]() {
-      DevFree(g, w.x); DevFree(g, w.r); DevFree(g, w.p); DevFree(g, w.q); DevFree(g, w.z);
-      DevFree(g, w.d32); DevFree(g, w.Minv); DevFree(g, w.x64); DevFree(g, w.r64); DevFree(g, w.S);
-      DevFree(g, w.P.p);
-      if(w.host) (void)hipHostFree(w.host);
-   };
-   if(!ok) { cleanup(); return -5; }
+   CgWork *ws = CgWorkspace(g, A);
+   if(!ws) return -5;
+   CgWork &w = *ws;
+   auto cleanup = []() {};
    HIPCHK(hipMemsetAsync(w.S, 0, sizeof(CgScal), s));
    HIPCHK(hipMemsetAsync(w.x64, 0, n * 8, s));
    unsigned int nbr = Blocks(nrow), nbn = Blocks(n);
@@ -892,7 +923,11 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    unsigned int gn = std::min<unsigned int>(kCgGrid, Blocks(n, kCgBlock));
 #define GML_B(K, ...) do { switch(B) { case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break; \
    case 3: hipLaunchKernelGGL(K<3>, __VA_ARGS__); break; default: hipLaunchKernelGGL(K<4>, __VA_ARGS__); } } while(0)
-   GML_B(k_block_jacobi, dim3(nbr), dim3(256), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val, w.Minv);
+   if(!w.minv_ok)
+   {
+      GML_B(k_block_jacobi, dim3(nbr), dim3(256), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val, w.Minv);
+      w.minv_ok = true;
+   }
 
    // |rhs|^2
    hipLaunchKernelGGL(k_dot, dim3(gn), dim3(kCgBlock), 0, s, n, (const float *)b->dev, (const float *)b->dev, w.P);

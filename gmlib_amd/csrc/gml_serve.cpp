@@ -1,0 +1,416 @@
+// Serving loop for mesh jobs (the request side of the engine).
+//
+// A server owns a cache of device-resident meshes, each in its own GMlib
+// handle (own HIP stream), and a pool of slots (worker threads).  Jobs name
+// a seeded cube mesh (cells per side, seed) and a kind of work:
+//   quality:  tet mean ratio + ball minimum + min / mean   (GmlxTetQuality,
+//             GmlxBallMin, GmlReduceVector)
+//   smooth:   nsweep Jacobi barycentre sweeps from the mesh's original
+//             coordinates, then min / mean quality          (GmlxSmoothSweeps)
+//   cg:       block-Jacobi CG with FP64 refinement on the mesh's edge
+//             Laplacian (GmlxEdgeLaplacian3), hashed N(0,1) right-hand side
+//                                                           (GmlxSolveCG)
+// Admission is continuous: whenever a slot is free, the oldest pending job
+// that can run is admitted -- its mesh resident and idle, or loadable within
+// the HBM budget after evicting least-recently-used idle meshes -- so jobs
+// arriving while others run start as soon as a slot and memory free up, and
+// their kernels overlap on the device (one stream per resident mesh).  A
+// mesh serves one job at a time; a smoothing job restores the original
+// coordinates first (and so does any job after one), so every result is
+// independent of what else ran: the same alone and in the mix.
+//
+// Resident = 0 gives the plain per-job path for comparison: every job loads
+// its mesh into a fresh handle, runs, and frees it, one job at a time.
+
+#include "gml_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <thread>
+
+namespace gml {
+
+struct Resident {
+   int n = 0;
+   unsigned long long seed = 0;
+   size_t h = 0;
+   int nver = 0, ntet = 0;
+   int VerIdx = 0, TetIdx = 0, qal = 0, vqal = 0, bary = 0, crd0 = 0;
+   int mat = 0, rhs = 0, sol = 0;
+   std::vector<int> tet;          // host copy of the tets (CG matrix build)
+   size_t bytes = 0;              // device bytes in use (or reserved before the job ends)
+   bool busy = false, dirty = false;
+   long long last = 0;
+};
+
+struct ServeJob {
+   long long id = 0;
+   int type = 0, n = 0;
+   unsigned long long seed = 0;
+   double param = 0.;
+   double t_submit = 0., t_admit = 0., t_done = 0.;
+   double res[4] = {0., 0., 0., 0.};
+   int rc = 0;
+   bool loaded = false, done = false;
+};
+
+struct Server {
+   int dev = 0, nslots = 1;
+   bool resident = true;
+   size_t budget = 0, used = 0, peak = 0;
+   std::mutex mtx;
+   std::condition_variable cv_work, cv_done;
+   std::deque<ServeJob *> pending;
+   std::map<long long, ServeJob *> jobs;
+   std::vector<Resident *> cache;
+   std::vector<std::thread> workers;
+   bool stop = false;
+   long long next_id = 1, clock = 0, loads = 0, evictions = 0, ndone = 0;
+   int running = 0, max_running = 0;
+   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+   double Now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+// device bytes a job's mesh may need (upper estimate, before loading)
+static size_t Estimate(int n, int type)
+{
+   const double V = std::pow(n + 1.0, 3), T = 6.0 * std::pow((double)n, 3);
+   double b = 44 * T + 48 * V + (1 << 20);
+   if(type == GmlxJobSmooth) b += 30 * T + 16 * V;
+   if(type == GmlxJobCG) b += 900 * V;
+   return (size_t)b;
+}
+
+static size_t DeviceBytes(const Resident *r) { return r->h ? GmlGetMemoryUsage(r->h) + (2 << 20) : 0; }
+
+static void Unload(Resident *r)
+{
+   if(r->h) GmlStop(r->h);
+   r->h = 0;
+}
+
+// load the mesh of r into a fresh handle; false on failure
+static bool Load(Server *s, Resident *r)
+{
+   const int n = r->n;
+   const long long V = (long long)(n + 1) * (n + 1) * (n + 1), T = 6LL * n * n * n;
+   if(V >= (1LL << 31) || T >= (1LL << 31)) return false;
+   std::vector<float> crd((size_t)V * 4);
+   std::vector<int> ref((size_t)V), tref((size_t)T, 0);
+   r->tet.resize((size_t)T * 4);
+   if(!GmlxCubeMesh(n, 0.2, r->seed, 1, crd.data(), ref.data(), r->tet.data())) return false;
+   r->nver = (int)V;
+   r->ntet = (int)T;
+   r->h = GmlInit(s->dev);
+   if(!r->h) return false;
+   const size_t h = r->h;
+   r->VerIdx = GmlNewMeshData(h, GmlVertices, (int)V);
+   r->TetIdx = GmlNewMeshData(h, GmlTetrahedra, (int)T);
+   if(!r->VerIdx || !r->TetIdx
+   || !GmlSetDataBlock(h, GmlVertices, 0, (int)V - 1, crd.data(), crd.data() + (V - 1) * 4, ref.data(), ref.data() + V - 1)
+   || !GmlSetDataBlock(h, GmlTetrahedra, 0, (int)T - 1, r->tet.data(), r->tet.data() + (T - 1) * 4, tref.data(),
+                       tref.data() + T - 1))
+      return false;
+   r->qal = GmlNewSolutionData(h, GmlTetrahedra, 1, GmlFlt, (char *)"qal");
+   r->vqal = GmlNewSolutionData(h, GmlVertices, 1, GmlFlt, (char *)"vqal");
+   r->crd0 = GmlNewSolutionData(h, GmlVertices, 1, GmlFlt4, (char *)"crd0");
+   if(!r->qal || !r->vqal || !r->crd0) return false;
+   // pristine copy of the coordinates, restored before a job that needs it
+   size_t nb = 0, nb0 = 0;
+   void *src = GmlxGetDevicePointer(h, r->VerIdx, &nb), *dst = GmlxGetDevicePointer(h, r->crd0, &nb0);
+   if(!src || !dst || nb0 < (size_t)V * 16) return false;
+   if(hipMemcpyAsync(dst, src, (size_t)V * 16, hipMemcpyDeviceToDevice, (hipStream_t)GmlxGetStream(h)) != hipSuccess)
+      return false;
+   return GmlxSync(h) == 1;
+}
+
+static bool Restore(Resident *r)
+{
+   size_t nb = 0, nb0 = 0;
+   void *dst = GmlxGetDevicePointer(r->h, r->VerIdx, &nb), *src = GmlxGetDevicePointer(r->h, r->crd0, &nb0);
+   if(!src || !dst) return false;
+   if(hipMemcpyAsync(dst, src, (size_t)r->nver * 16, hipMemcpyDeviceToDevice, (hipStream_t)GmlxGetStream(r->h)) != hipSuccess)
+      return false;
+   GmlxInvalidateHost(r->h, r->VerIdx);
+   r->dirty = false;
+   return true;
+}
+
+static int QualityStats(Resident *r, double *res)
+{
+   double mn = 0., sm = 0.;
+   if(GmlxTetQuality(r->h, r->qal) != 1 || GmlxBallMin(r->h, r->qal, r->vqal) != 1) return -6;
+   if(GmlReduceVector(r->h, r->qal, GmlMin, &mn) != 1 || GmlReduceVector(r->h, r->qal, GmlSum, &sm) != 1) return -6;
+   res[0] = mn;
+   res[1] = sm / (double)r->ntet;
+   return 1;
+}
+
+// hashed N(0,1) entries (Box-Muller on two splitmix draws)
+static double HashNormal(unsigned long long seed, long long i)
+{
+   auto mix = [](unsigned long long z) {
+      z += 0x9e3779b97f4a7c15ULL;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      return z ^ (z >> 31);
+   };
+   const unsigned long long k = mix(seed ^ mix((unsigned long long)i));
+   const double u1 = ((double)(mix(k + 1) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+   const double u2 = (double)(mix(k + 2) >> 11) * (1.0 / 9007199254740992.0);
+   return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+static int RunJob(Resident *r, ServeJob *j)
+{
+   if(j->type == GmlxJobQuality)
+   {
+      if(r->dirty && !Restore(r)) return -6;
+      return QualityStats(r, j->res);
+   }
+   if(j->type == GmlxJobSmooth)
+   {
+      if(!Restore(r)) return -6;
+      if(!r->bary && !(r->bary = GmlNewSolutionData(r->h, GmlTetrahedra, 1, GmlFlt4, (char *)"bary"))) return -5;
+      r->dirty = true;
+      const int nsweep = j->param > 0 ? (int)j->param : 20;
+      if(GmlxSmoothSweeps(r->h, r->bary, nsweep, 0.2) != 1) return -6;
+      return QualityStats(r, j->res);
+   }
+   if(j->type == GmlxJobCG)
+   {
+      if(!r->mat)
+      {
+         const int V = r->nver;
+         std::vector<int> rowptr((size_t)V + 1);
+         const long long nnz = GmlxEdgeLaplacian3(V, r->ntet, r->tet.data(), r->seed, 0.01, rowptr.data(), nullptr,
+                                                  nullptr, 0);
+         if(nnz <= 0) return -5;
+         std::vector<int> col((size_t)nnz);
+         std::vector<float> val((size_t)nnz * 9);
+         if(GmlxEdgeLaplacian3(V, r->ntet, r->tet.data(), r->seed, 0.01, rowptr.data(), col.data(), val.data(), nnz) != nnz)
+            return -5;
+         std::vector<float> b((size_t)V * 3);
+         for(long long i = 0; i < (long long)V * 3; i++) b[i] = (float)HashNormal(r->seed, i);
+         r->mat = GmlNewMatrix(r->h, V, (int)nnz, 3, val.data(), col.data(), rowptr.data(), GmlFlt);
+         r->rhs = GmlNewVector(r->h, V, 3, b.data(), GmlFlt);
+         r->sol = GmlNewVector(r->h, V, 3, nullptr, GmlDbl);
+         if(!r->mat || !r->rhs || !r->sol) return -5;
+      }
+      const double tol = j->param > 0 ? j->param : 1e-8;
+      double rel = 0.;
+      int it = 0;
+      const int rc = GmlxSolveCG(r->h, r->mat, r->rhs, r->sol, tol, 20000, 1, &rel, &it);
+      j->res[0] = rel;
+      j->res[1] = it;
+      return rc;
+   }
+   return -1;
+}
+
+// evict least-recently-used idle meshes (never keep) until need more bytes
+// fit the budget (lock held)
+static bool MakeRoom(Server *s, size_t need, const Resident *keep)
+{
+   while(s->used + need > s->budget)
+   {
+      Resident *v = nullptr;
+      for(Resident *r : s->cache)
+         if(!r->busy && r != keep && (!v || r->last < v->last)) v = r;
+      if(!v) return false;
+      s->used -= v->bytes;
+      s->cache.erase(std::find(s->cache.begin(), s->cache.end(), v));
+      Unload(v);
+      delete v;
+      s->evictions++;
+   }
+   return true;
+}
+
+static void Finish(Server *s, ServeJob *j, int rc)
+{
+   j->rc = rc;
+   j->done = true;
+   j->t_done = s->Now();
+   s->ndone++;
+   s->cv_done.notify_all();
+}
+
+// the oldest pending job that can run now, its mesh reserved (lock held)
+static ServeJob *Admit(Server *s, Resident **out)
+{
+   for(auto it = s->pending.begin(); it != s->pending.end();)
+   {
+      ServeJob *j = *it;
+      const size_t est = Estimate(j->n, j->type);
+      if(est > s->budget)
+      {
+         it = s->pending.erase(it);
+         j->t_admit = s->Now();
+         Finish(s, j, -7);
+         continue;
+      }
+      if(!s->resident && s->running > 0) return nullptr;
+      Resident *r = nullptr;
+      if(s->resident)
+         for(Resident *c : s->cache)
+            if(c->n == j->n && c->seed == j->seed) { r = c; break; }
+      if(r && r->busy) { ++it; continue; }
+      const size_t need = r ? (est > r->bytes ? est - r->bytes : 0) : est;
+      if(need && !MakeRoom(s, need, r)) { ++it; continue; }
+      if(!r)
+      {
+         r = new Resident();
+         r->n = j->n;
+         r->seed = j->seed;
+         s->cache.push_back(r);
+         j->loaded = true;
+         s->loads++;
+      }
+      // the reservation joins the mesh's bytes until its real size is known
+      s->used += need;
+      r->bytes += need;
+      s->peak = std::max(s->peak, s->used);
+      r->busy = true;
+      r->last = ++s->clock;
+      s->pending.erase(it);
+      *out = r;
+      return j;
+   }
+   return nullptr;
+}
+
+static void Worker(Server *s)
+{
+   (void)hipSetDevice(s->dev);
+   for(;;)
+   {
+      std::unique_lock<std::mutex> lk(s->mtx);
+      ServeJob *j = nullptr;
+      Resident *r = nullptr;
+      s->cv_work.wait(lk, [&]() { return s->stop || (j = Admit(s, &r)) != nullptr; });
+      if(!j) return;
+      s->running++;
+      s->max_running = std::max(s->max_running, s->running);
+      j->t_admit = s->Now();
+      lk.unlock();
+
+      int rc = 1;
+      if(!r->h && !Load(s, r)) rc = -5;
+      if(rc == 1) rc = RunJob(r, j);
+
+      lk.lock();
+      const size_t real = DeviceBytes(r);
+      if(rc == -5 || !s->resident)
+      {
+         // failed load, or the plain path: the mesh goes
+         s->used -= r->bytes;
+         s->cache.erase(std::find(s->cache.begin(), s->cache.end(), r));
+         Unload(r);
+         delete r;
+      }
+      else
+      {
+         s->used = s->used - r->bytes + real;
+         r->bytes = real;
+         r->busy = false;
+         r->last = ++s->clock;
+         s->peak = std::max(s->peak, s->used);
+         MakeRoom(s, 0, nullptr);
+      }
+      s->running--;
+      Finish(s, j, rc);
+      s->cv_work.notify_all();
+   }
+}
+
+} // namespace gml
+
+using namespace gml;
+
+extern "C" {
+
+size_t GmlxServeStart(int dev, double BudgetGB, int NmbSlot, int Resident)
+{
+   if(BudgetGB <= 0. || NmbSlot < 1 || NmbSlot > 64) return 0;
+   Server *s = new Server();
+   s->dev = dev;
+   s->budget = (size_t)(BudgetGB * 1e9);
+   s->resident = Resident != 0;
+   s->nslots = s->resident ? NmbSlot : 1;
+   for(int i = 0; i < s->nslots; i++) s->workers.emplace_back(Worker, s);
+   return (size_t)s;
+}
+
+long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long seed, double param)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || type < GmlxJobQuality || type > GmlxJobCG || n < 1 || n > 1000) return -1;
+   ServeJob *j = new ServeJob();
+   j->type = type;
+   j->n = n;
+   j->seed = seed;
+   j->param = param;
+   std::lock_guard<std::mutex> lk(s->mtx);
+   j->id = s->next_id++;
+   j->t_submit = s->Now();
+   s->jobs[j->id] = j;
+   s->pending.push_back(j);
+   s->cv_work.notify_one();
+   return j->id;
+}
+
+int GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s) return -1;
+   std::unique_lock<std::mutex> lk(s->mtx);
+   auto it = s->jobs.find(JobIdx);
+   if(it == s->jobs.end()) return -1;
+   ServeJob *j = it->second;
+   s->cv_done.wait(lk, [&]() { return j->done; });
+   if(res) for(int i = 0; i < 4; i++) res[i] = j->res[i];
+   if(tim)
+   {
+      tim[0] = j->t_submit;
+      tim[1] = j->t_admit;
+      tim[2] = j->t_done;
+      tim[3] = j->loaded ? 1. : 0.;
+   }
+   const int rc = j->rc;
+   s->jobs.erase(it);
+   delete j;
+   return rc;
+}
+
+int GmlxServeInfo(size_t SrvIdx, double *info)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || !info) return 0;
+   std::lock_guard<std::mutex> lk(s->mtx);
+   const double v[8] = {(double)s->cache.size(), (double)s->used, (double)s->loads, (double)s->evictions,
+                        (double)s->ndone, (double)s->max_running, (double)s->budget, (double)s->peak};
+   for(int i = 0; i < 8; i++) info[i] = v[i];
+   return 1;
+}
+
+void GmlxServeStop(size_t SrvIdx)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s) return;
+   {
+      std::lock_guard<std::mutex> lk(s->mtx);
+      s->stop = true;
+      s->cv_work.notify_all();
+   }
+   for(auto &t : s->workers) t.join();
+   for(Resident *r : s->cache) { Unload(r); delete r; }
+   for(auto &p : s->jobs) delete p.second;
+   delete s;
+}
+
+} // extern "C"

@@ -147,6 +147,40 @@ double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches);
 int    GmlxCubeMesh(int n, double jitter, unsigned long long seed, int order,
                     float *crd, int *ref, int *tet);
 
+/* Edge-graph 3x3 vector Laplacian of a tet mesh (host, no handle): block
+   CSR with one block row per vertex, columns ascending, off-diagonal block
+   (i, j) = -w (I + 0.25 s s^T) with w in [0.5, 2] and s a unit vector drawn
+   from a hash of (seed, min(i,j), max(i,j)), diagonal = sum of the
+   incident edge blocks + shift I (SPD).  rowptr (NmbVer + 1 ints) is always
+   filled; col (cap ints) and val (9 cap floats) when cap >= the block
+   count.  Returns the block count, or < 0 on invalid input. */
+long long GmlxEdgeLaplacian3(int NmbVer, int NmbTet, const int *tet, unsigned long long seed, double shift,
+                             int *rowptr, int *col, float *val, long long cap);
+
+/* ---- serving: a queue of mesh jobs run against device-resident meshes.
+   A job names a seeded cube mesh (GmlxCubeMesh(n, 0.2, seed, Hilbert)) and a
+   kind of work; up to NmbSlot jobs run at once, each on its mesh's own
+   stream; meshes stay resident under an HBM budget of BudgetGB (least
+   recently used idle meshes are evicted when a new one needs room); a job
+   is admitted as soon as a slot is free and its mesh is resident and idle or
+   fits.  Resident = 0: the plain path, every job loads its mesh into a fresh
+   handle, runs and frees it, one job at a time. ---- */
+enum gmlx_job { GmlxJobQuality, GmlxJobSmooth, GmlxJobCG };
+size_t    GmlxServeStart(int dev, double BudgetGB, int NmbSlot, int Resident);
+/* param: smooth = sweep count (20 if <= 0), cg = relative tolerance (1e-8
+   if <= 0), unused for quality.  Returns a job id > 0, or -1. */
+long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long seed, double param);
+/* block until the job is done.  res[4]: quality and smooth = (min, mean
+   quality), cg = (FP64 relative residual, iterations); tim[4] = (submitted,
+   admitted, done) in seconds since GmlxServeStart, and 1 if the job loaded
+   its mesh.  Returns 1, or a negative error (-7: the mesh cannot fit the
+   budget). */
+int       GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim);
+/* 8 values: resident meshes, bytes in use, mesh loads, evictions, jobs done,
+   most jobs running at once, budget bytes, peak bytes */
+int       GmlxServeInfo(size_t SrvIdx, double *info);
+void      GmlxServeStop(size_t SrvIdx);
+
 /* ---- batches of meshes: many independent tet meshes kept on the device and
    processed by one persistent kernel replayed from a HIP graph.  A run
    computes the mean-ratio quality of every tet (as GmlxTetQuality), its
