@@ -119,6 +119,7 @@ _SIGS = {
     "GmlxServeWait": (_I, [_S, C.c_longlong, _DP, _DP]),
     "GmlxServeInfo": (_I, [_S, _DP]),
     "GmlxServeStop": (None, [_S]),
+    "GmlxSetStaging": (_I, [_I]),
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)

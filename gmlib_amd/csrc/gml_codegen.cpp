@@ -220,11 +220,60 @@ static std::string ArgDecl(const Arg &a)
 
 // Emit the generated kernel.  lazy = 1 makes multi-item uplink data lazy
 // accessors instead of eagerly filled local arrays.
+// ---------------------------------------------------------------------------
+// LDS-staged uplink data.  Read-only, single-item data reached through a
+// vertex ball (or any uplink) are gathered per work-group: the tile's
+// distinct destination entities (ascending ids, so the loads coalesce) are
+// loaded once into LDS, and every ball slot then reads LDS through its
+// tile-local index (StagePlan).  Values and their order are those of the
+// direct gather; only where the bytes come from changes.
+// ---------------------------------------------------------------------------
+int g_stage_uplinks = 1;
+
+// the ball whose data are staged (the first ball with stageable data), or
+// -1; *staged receives the staged data arguments
+static int StagedArgs(const std::vector<Arg> &args, std::vector<int> *staged)
+{
+   int ball = -1;
+   staged->clear();
+   for(size_t i = 0; i < args.size(); i++)
+   {
+      const Arg &a = args[i];
+      if(a.kind != AkData || a.CntIdx < 0 || a.LnkIdx < 0 || args[a.LnkIdx].kind != AkBall) continue;
+      if(!(a.FlgTab & GmlReadMode) || (a.FlgTab & (GmlWriteMode | GmlManual)) || a.NmbItm != 1) continue;
+      if(ball < 0) ball = a.LnkIdx;
+      if(a.LnkIdx == ball) staged->push_back((int)i);
+   }
+   return ball;
+}
+
+static bool IsIdent(char c) { return isalnum((unsigned char)c) || c == '_'; }
+
+// does the body name this identifier?
+static bool BodyUses(const char *body, const std::string &id)
+{
+   for(const char *p = strstr(body, id.c_str()); p; p = strstr(p + 1, id.c_str()))
+      if((p == body || !IsIdent(p[-1])) && !IsIdent(p[id.size()])) return true;
+   return false;
+}
+
 static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam, int MshTyp,
-                              const std::vector<Arg> &args, const char *ParSrc, bool lazy)
+                              const std::vector<Arg> &args, const char *ParSrc, bool lazy, bool stage)
 {
    std::string s;
    s.reserve(16384);
+   std::vector<int> staged;
+   const int sball = stage ? StagedArgs(args, &staged) : -1;
+   // the ball's own entries are read only when something needs them
+   bool ball_ids = true;
+   if(sball >= 0)
+   {
+      const Arg &B = args[sball];
+      ball_ids = (B.FlgTab & GmlVoyeurs) || BodyUses(KrnSrc, B.nam);
+      for(size_t i = 0; i < args.size(); i++)
+         if(args[i].LnkIdx == sball && std::find(staged.begin(), staged.end(), (int)i) == staged.end())
+            ball_ids = true;
+   }
    s += "// GMlib MESHING AND GEOMETRICAL TOOLKIT\n";
    s += kPrelude;
    if(lazy)
@@ -255,6 +304,9 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    s += Fmt("extern \"C\" __global__ void __launch_bounds__(256) %s(", PrcNam);
    for(auto &a : args)
       s += "\n   " + ArgDecl(a) + ",";
+   if(sball >= 0)
+      s += "\n   const unsigned int *gml_tofs,\n   const int *gml_tlist,\n   const unsigned short *gml_lidx,"
+           "\n   const int gml_stage_n,";
    s += "\n   GmlParSct *GmlPar,\n   const int2 count,\n   const int gml_remap)\n{\n";
 
    // --- locals (gmlib3.c:2670-2732)
@@ -269,6 +321,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       bool lz = lazy && C && a.NmbItm > 1;
       if(a.kind == AkBall)
       {
+         if((int)i == sball) s += Fmt("   unsigned short gml_l_%s[%d];\n", a.nam.c_str(), a.NmbItm * a.ItmLen);
          s += Fmt("   int gml_%s[%d];\n", a.nam.c_str(), a.NmbItm * a.ItmLen);  // shifted entries, one per slot
          if(a.NmbItm > 1) s += Fmt("   int16 %s[%d];\n", a.nam.c_str(), a.NmbItm);
          else             s += Fmt("   %s %s;\n", OclTypStr[a.ItmTyp], a.nam.c_str());
@@ -301,6 +354,27 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    s += "// KERNEL COUNTERS\n";
    s += "   int       cnt = gml_gid(gml_remap);\n";
    s += Fmt("   int       %sIdx = cnt + count.s1;\n\n", BalTypStr[MshTyp]);
+   if(sball >= 0)
+   {
+      s += "// STAGED UPLINK DATA: the work-group's tile, coalesced loads into LDS\n"
+           "   extern __shared__ __align__(16) char gml_lds[];\n"
+           "   size_t gml_lo = 0;\n";
+      for(int i : staged)
+      {
+         const Arg &a = args[i];
+         const char *T = OclTypStr[a.ItmTyp];
+         s += Fmt("   %s *gml_s_%s = (%s *)(gml_lds + gml_lo);\n", T, a.nam.c_str(), T);
+         s += Fmt("   gml_lo += ((size_t)gml_stage_n * sizeof(%s) + 15) & ~(size_t)15;\n", T);
+      }
+      s += "   {\n"
+           "      const unsigned int gml_t = (unsigned int)(cnt - (int)threadIdx.x) / blockDim.x;\n"
+           "      const unsigned int gml_t0 = gml_tofs[gml_t], gml_nt = gml_tofs[gml_t + 1] - gml_t0;\n"
+           "      for(unsigned int gml_i = threadIdx.x; gml_i < gml_nt; gml_i += blockDim.x)\n      {\n"
+           "         const int gml_x = gml_tlist[gml_t0 + gml_i];\n";
+      for(int i : staged)
+         s += Fmt("         gml_s_%s[gml_i] = %sTab[gml_x];\n", args[i].nam.c_str(), args[i].nam.c_str());
+      s += "      }\n      __syncthreads();\n   }\n\n";
+   }
    s += "   if(cnt >= count.s0)\n      return;\n\n";
 
    // --- reads (gmlib3.c:2755-2885)
@@ -320,14 +394,24 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
          int w = a.NmbItm * a.ItmLen;
          const Arg *D = nullptr;
          for(auto &b : args) if(b.kind == AkDeg && b.SrcTyp == a.SrcTyp && b.DstTyp == a.DstTyp) D = &b;
+         const bool st = (int)i == sball;
          s += Fmt("   {\n      const int gml_e = cnt + count.s1;\n"
                   "      const int *gml_row = %sTab + (size_t)%sOff[gml_e >> 6] * 64 + (gml_e & 63);\n"
                   "      const int gml_d = %sTab[gml_e];\n", n, n, D ? D->nam.c_str() : "");
-         s += Fmt("#pragma unroll\n      for(int k = 0; k < %d; k++)\n      {\n"
-                  "         int v = k < gml_d ? gml_row[k * 64] : 0;\n", w);
-         if(a.FlgTab & GmlVoyeurs)
-            s += Fmt("         %s[k] = (char)(v & 7);\n", a.VoyNam.c_str());
-         s += Fmt("         gml_%s[k] = v >> 4;\n      }\n   }\n", n);
+         if(st)
+            s += Fmt("      const unsigned short *gml_lrow = gml_lidx + (size_t)%sOff[gml_e >> 6] * 64 + (gml_e & 63);\n", n);
+         s += Fmt("#pragma unroll\n      for(int k = 0; k < %d; k++)\n      {\n", w);
+         if(st)
+            s += Fmt("         gml_l_%s[k] = k < gml_d ? gml_lrow[k * 64] : 0;\n", n);
+         if(!st || ball_ids)
+         {
+            s += "         int v = k < gml_d ? gml_row[k * 64] : 0;\n";
+            if(a.FlgTab & GmlVoyeurs)
+               s += Fmt("         %s[k] = (char)(v & 7);\n", a.VoyNam.c_str());
+            s += Fmt("         gml_%s[k] = v >> 4;\n", n);
+         }
+         s += "      }\n   }\n";
+         if(st && !ball_ids) { s += "\n"; continue; }
          for(int v = 0; v < a.NmbItm; v++)
          {
             if(a.NmbItm > 1) s += Fmt("   %s[%d] = (int16){", n, v);
@@ -378,7 +462,11 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
                std::string l2 = L->ItmLen > 1 ? Fmt(".s%c", HexDig[k & 15]) : "";
                lnk = L->nam + l1 + l2;
             }
-            if(C)
+            const bool sd = std::find(staged.begin(), staged.end(), (int)i) != staged.end();
+            if(C && sd)
+               s += Fmt("   %s%s = (%sDeg >= %d) ? gml_s_%s[ gml_l_%s[%d] ] : %sNul;\n",
+                        n, td2.c_str(), n, k + 1, n, L->nam.c_str(), k, n);
+            else if(C)
                s += Fmt("   %s%s%s = (%sDeg >= %d) ? %sTab[ %s ]%s : %sNul;\n",
                         n, td2.c_str(), td1.c_str(), n, k + 1, n, lnk.c_str(), td1.c_str(), n);
             else if(L && L->LnkDir == 0)
@@ -513,9 +601,56 @@ int RunKernel(GmlCtx *g, Kernel *k)
       remap = kShapes[k->tune_step][1];
       tuning = 1;
    }
+   // staged uplink data: the tile plan of this work-group size; the tuner
+   // keeps the largest size whose LDS fits 64 KB and varies the remap only
+   StagePlan *sp = nullptr;
+   unsigned int lds = 0;
+   int stage_n = 0;
+   if(k->stage_src >= 0)
+   {
+      auto plan_for = [&](int gs) -> StagePlan * {
+         StagePlan *&slot = g->up[k->stage_src][k->stage_dst].stage[gs == 64 ? 0 : (gs == 128 ? 1 : 2)];
+         if(slot && slot->nrows != n) { FreeStagePlan(g, slot); slot = nullptr; }
+         if(!slot) slot = NewStagePlan(g, k->stage_src, k->stage_dst, gs, n);
+         return slot;
+      };
+      auto lds_of = [&](const StagePlan *p) {
+         size_t b = 0;
+         for(int z : k->stage_siz) b += ((size_t)p->max_nt * z + 15) & ~(size_t)15;
+         return b;
+      };
+      if(!k->stage_grp)
+         for(int gs : {256, 128, 64})
+         {
+            StagePlan *p = plan_for(gs);
+            if(p && lds_of(p) <= 64 * 1024) { k->stage_grp = gs; break; }
+         }
+      if(!k->stage_grp)
+      {
+         fprintf(stderr, "gml: kernel %s: no tile of its staged uplink data fits the LDS\n", k->name.c_str());
+         return -7;
+      }
+      if(tuning || !k->grp) grp = k->stage_grp;
+      sp = plan_for(grp);
+      if(!sp || lds_of(sp) > 64 * 1024)
+      {
+         fprintf(stderr, "gml: kernel %s: work-group size %d leaves its staged data too large for the LDS\n",
+                 k->name.c_str(), grp);
+         return -7;
+      }
+      lds = (unsigned int)lds_of(sp);
+      stage_n = sp->max_nt;
+   }
    int remap_arg = remap;
    std::vector<void *> kp;
    for(auto &p : ptrs) kp.push_back((void *)&p);
+   void *stage_ptr[3] = {sp ? (void *)sp->tofs : nullptr, sp ? (void *)sp->tlist : nullptr,
+                         sp ? (void *)sp->lidx : nullptr};
+   if(sp)
+   {
+      for(auto &p : stage_ptr) kp.push_back((void *)&p);
+      kp.push_back((void *)&stage_n);
+   }
    kp.push_back((void *)&par);
    kp.push_back((void *)count);
    kp.push_back((void *)&remap_arg);
@@ -526,7 +661,7 @@ int RunKernel(GmlCtx *g, Kernel *k)
    if(have) (void)hipEventRecord(rec.beg, g->stream);
    if(tun) (void)hipEventRecord(tr.beg, g->stream);
    unsigned int nb = (unsigned int)((n + grp - 1) / grp);
-   hipError_t e = hipModuleLaunchKernel(k->fn, nb, 1, 1, grp, 1, 1, 0, g->stream, kp.data(), nullptr);
+   hipError_t e = hipModuleLaunchKernel(k->fn, nb, 1, 1, grp, 1, 1, lds, g->stream, kp.data(), nullptr);
    if(e != hipSuccess)
    {
       (void)hipGetLastError();
@@ -564,7 +699,7 @@ int RunKernel(GmlCtx *g, Kernel *k)
          (void)hipGetLastError();
          for(auto &t : k->tune_ev) { (void)hipEventDestroy(t.beg); (void)hipEventDestroy(t.end); }
          k->tune_ev.clear();
-         k->grp = kShapes[bi][0];
+         k->grp = k->stage_src >= 0 ? k->stage_grp : kShapes[bi][0];
          k->remap = kShapes[bi][1];
          k->tuned = 1;
       }
@@ -737,18 +872,30 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
    //    body needs them as lvalues
    bool any_lazy = false;
    for(auto &a : args) if(a.CntIdx >= 0 && a.NmbItm > 1) any_lazy = true;
+   std::vector<int> staged;
+   const int sball = g_stage_uplinks ? StagedArgs(args, &staged) : -1;
    std::string log, src;
    int KrnIdx = 0;
-   if(any_lazy)
+   for(int st = sball >= 0 ? 1 : 0; st >= 0 && !KrnIdx; st--)
    {
-      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true);
-      KrnIdx = NewKernel(g, src, PrcNam, &log);
-   }
-   if(!KrnIdx)
-   {
-      log.clear();
-      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false);
-      KrnIdx = NewKernel(g, src, PrcNam, &log);
+      if(any_lazy)
+      {
+         src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true, st);
+         KrnIdx = NewKernel(g, src, PrcNam, &log);
+      }
+      if(!KrnIdx)
+      {
+         log.clear();
+         src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false, st);
+         KrnIdx = NewKernel(g, src, PrcNam, &log);
+      }
+      if(KrnIdx && st)
+      {
+         Kernel *k = &g->krn[KrnIdx];
+         k->stage_src = args[sball].SrcTyp;
+         k->stage_dst = args[sball].DstTyp;
+         for(int i : staged) k->stage_siz.push_back(OclTypSiz[args[i].ItmTyp]);
+      }
    }
    if(!KrnIdx)
    {
@@ -787,12 +934,12 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
    int HghIdx = 0;
    if(any_lazy)
    {
-      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true);
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true, false);
       HghIdx = NewKernel(g, src, PrcNam, &log);
    }
    if(!HghIdx)
    {
-      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false);
+      src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false, false);
       HghIdx = NewKernel(g, src, PrcNam, &log);
    }
    if(!HghIdx)
@@ -873,6 +1020,13 @@ double GmlGetKernelRunTime(size_t GmlIdx, int idx)
       t += h->run_time;
    }
    return t;
+}
+
+int GmlxSetStaging(int on)
+{
+   const int old = g_stage_uplinks;
+   g_stage_uplinks = on ? 1 : 0;
+   return old;
 }
 
 const char *GmlxGetKernelSource(size_t GmlIdx, int idx)

@@ -58,10 +58,24 @@ struct Data {
 // Vertex/entity -> element uplink in SELL-64 layout: rows are grouped in
 // chunks of 64 consecutive source entities; chunk c stores its entries
 // slab-major at ent[(off[c] + k) * 64 + lane], k < width of the chunk.
+// Tile plan of an LDS-staged uplink gather in generated kernels: for tile t
+// (rows t*grp .. t*grp+grp-1 of the source kind), the distinct destination
+// entities of its ball rows, ascending, and for every ball slot (SELL-64
+// layout of the ball) the tile-local index of its entity.
+struct StagePlan {
+   int grp = 0, max_nt = 0;
+   long long nrows = 0, ntile = 0;
+   unsigned int *tofs = nullptr;    // [ntile+1]
+   int *tlist = nullptr;            // destination ids, tile by tile
+   unsigned short *lidx = nullptr;  // per ball slot
+   size_t bytes = 0;
+};
+
 struct Uplink {
    int   DegIdx = 0, BalIdx = 0, OffIdx = 0;
    int   BasWid = 0, HghWid = 0, MaxPos = 0, MaxDeg = 0;
    long long nslab = 0;
+   StagePlan *stage[3] = {nullptr, nullptr, nullptr};   // work-group size 64, 128, 256
 };
 
 struct LaunchRec { hipEvent_t beg, end; };
@@ -78,6 +92,10 @@ struct Kernel {
    bool          has_par = true;
    // launch shape autotuning (group size x XCD remap)
    int           grp = 256, remap = 1, tuned = 0, tune_step = 0;
+   // LDS-staged uplink data (stage_src >= 0): the uplink, the byte size of
+   // each staged item, the work-group size its plan was built for
+   int           stage_src = -1, stage_dst = -1, stage_grp = 0;
+   std::vector<int> stage_siz;
    std::vector<LaunchRec> tune_ev;
    // profiling
    std::vector<LaunchRec> evts;
@@ -236,6 +254,9 @@ void FreeBuiltinStats(GmlCtx *g);
 void FreeBatches(GmlCtx *g);
 SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
+StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows);
+void FreeStagePlan(GmlCtx *g, StagePlan *p);
+extern int g_stage_uplinks;   // GmlxSetStaging
 
 // launcher (gml_codegen.cpp)
 int  RunKernel(GmlCtx *g, Kernel *k);

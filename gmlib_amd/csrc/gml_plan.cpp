@@ -202,6 +202,87 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
    return p;
 }
 
+StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows)
+{
+   Uplink *u = &g->up[Src][Dst];
+   if(!u->BalIdx || grp < 64 || grp > 256 || (grp & 63) || nrows < 0 || nrows > g->NmbEle[Src]) return nullptr;
+   const long long nsrc = g->NmbEle[Src], nchunk = (nsrc + 63) / 64;
+   std::vector<int> deg((size_t)nsrc);
+   std::vector<unsigned int> off((size_t)nchunk + 1);
+   std::vector<int> ent((size_t)u->nslab * 64 + 64);
+   if(!DevCopy(g, deg.data(), g->dat[u->DegIdx].dev, (size_t)nsrc * 4, 2)
+   || !DevCopy(g, off.data(), g->dat[u->OffIdx].dev, off.size() * 4, 2)
+   || !DevCopy(g, ent.data(), g->dat[u->BalIdx].dev, ent.size() * 4, 2))
+      return nullptr;
+   const long long ntile = (nrows + grp - 1) / grp;
+   std::vector<std::vector<int>> tl((size_t)ntile);
+   std::vector<unsigned short> lidx((size_t)u->nslab * 64 + 64, 0);
+   std::atomic<bool> fits(true);
+   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   std::vector<std::thread> th;
+   for(unsigned w = 0; w < nth; w++)
+      th.emplace_back([&, w]() {
+         std::vector<int> ts;
+         for(long long t = w; t < ntile; t += nth)
+         {
+            const long long r0 = t * grp, r1 = std::min(nrows, r0 + grp);
+            ts.clear();
+            for(long long r = r0; r < r1; r++)
+               for(int k = 0; k < deg[r]; k++)
+                  ts.push_back(ent[((size_t)off[r >> 6] + k) * 64 + (r & 63)] >> 4);
+            std::sort(ts.begin(), ts.end());
+            ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+            if(ts.size() > 65535) { fits = false; continue; }
+            for(long long r = r0; r < r1; r++)
+               for(int k = 0; k < deg[r]; k++)
+               {
+                  const size_t sl = ((size_t)off[r >> 6] + k) * 64 + (r & 63);
+                  lidx[sl] = (unsigned short)(std::lower_bound(ts.begin(), ts.end(), ent[sl] >> 4) - ts.begin());
+               }
+            tl[t] = ts;
+         }
+      });
+   for(auto &x : th) x.join();
+   if(!fits) return nullptr;
+   StagePlan *p = new StagePlan();
+   p->grp = grp;
+   p->nrows = nrows;
+   p->ntile = ntile;
+   std::vector<unsigned int> tofs((size_t)ntile + 1, 0);
+   for(long long t = 0; t < ntile; t++)
+   {
+      tofs[t + 1] = tofs[t] + (unsigned int)tl[t].size();
+      p->max_nt = std::max(p->max_nt, (int)tl[t].size());
+   }
+   std::vector<int> tlist((size_t)tofs[ntile] + 1);
+   for(long long t = 0; t < ntile; t++) std::copy(tl[t].begin(), tl[t].end(), tlist.begin() + tofs[t]);
+   p->tofs = (unsigned int *)DevAlloc(g, tofs.size() * 4);
+   p->tlist = (int *)DevAlloc(g, tlist.size() * 4);
+   p->lidx = (unsigned short *)DevAlloc(g, lidx.size() * 2);
+   if(!p->tofs || !p->tlist || !p->lidx
+   || !DevCopy(g, p->tofs, tofs.data(), tofs.size() * 4, 1)
+   || !DevCopy(g, p->tlist, tlist.data(), tlist.size() * 4, 1)
+   || !DevCopy(g, p->lidx, lidx.data(), lidx.size() * 2, 1))
+   {
+      FreeStagePlan(g, p);
+      return nullptr;
+   }
+   p->bytes = tofs.size() * 4 + tlist.size() * 4 + lidx.size() * 2;
+   g->MemSiz += p->bytes;
+   DbgPrint(g, "Staging plan %d->%d: %lld tiles of %d rows, %u tile entities (%.2f per entity), max %d per tile\n",
+            Src, Dst, ntile, grp, tofs[ntile], g->NmbEle[Dst] ? (double)tofs[ntile] / (double)g->NmbEle[Dst] : 0.,
+            p->max_nt);
+   return p;
+}
+
+void FreeStagePlan(GmlCtx *g, StagePlan *p)
+{
+   if(!p) return;
+   DevFree(g, p->tofs); DevFree(g, p->tlist); DevFree(g, p->lidx);
+   g->MemSiz -= p->bytes;
+   delete p;
+}
+
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
 {
    if(!p) return;

@@ -312,6 +312,9 @@ void GmlStop(size_t GmlIdx)
    g->plan = g->oplan = nullptr;
    if(g->xalt) { DevFree(g, g->xalt); g->xalt = nullptr; }
    FreeBatches(g);
+   for(int a = 0; a < GmlMaxEleTyp; a++)
+      for(int b = 0; b < GmlMaxEleTyp; b++)
+         for(auto &sp : g->up[a][b].stage) { FreeStagePlan(g, sp); sp = nullptr; }
    FreeAlgebra(g);
    for(int i = 1; i <= GmlMaxDat; i++)
       FreeDataSlot(g, i);

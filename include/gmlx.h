@@ -31,6 +31,12 @@ int    GmlxCompileKernelv(size_t GmlIdx, const char *KrnSrc, const char *PrcNam,
 
 /* Generated HIP source of a compiled kernel (NUL-terminated, owned by the
    library, valid until GmlStop); NULL if the index is invalid. */
+/* LDS staging of uplink data in generated kernels (default on): read-only,
+   single-item data reached through a ball (e.g. tet data read by a vertex
+   kernel) are loaded once per work-group tile into LDS with coalesced loads
+   and read from there; results are identical either way.  Applies to
+   kernels compiled afterwards; returns the previous setting. */
+int    GmlxSetStaging(int on);
 const char *GmlxGetKernelSource(size_t GmlIdx, int KrnIdx);
 
 /* Group size and XCD remap flag the launch-shape tuner settled on for a

@@ -48,23 +48,40 @@ def test_downlink_kernel_contract(mesh):
         assert "float4 VerCrd[4];" in hdr
 
 
-def test_uplink_kernel_contract_with_ref_and_params(mesh):
+@pytest.mark.parametrize("staged", [0, 1])
+def test_uplink_kernel_contract_with_ref_and_params(mesh, staged):
     crd, ref, tet = mesh
-    with G.Gml(host_only=True) as g:
-        vi, ti = g.set_mesh(crd, ref, tet)
-        g.new_parameters(16, body("smooth_params"))
-        b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
-        k = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
-                      [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
-        s = src_of(g, k)
-        assert "#define   BaryDegMax 32" in s
-        assert "int16 VerTetBal[2];" in s and "int VerTetDeg;" in s and "int BaryDeg;" in s
+    old = G.lib().GmlxSetStaging(staged)
+    try:
+        with G.Gml(host_only=True) as g:
+            vi, ti = g.set_mesh(crd, ref, tet)
+            g.new_parameters(16, body("smooth_params"))
+            b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+            k = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
+                          [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+            s = src_of(g, k)
+            linkinfo = g.link_info(G.GmlVertices, G.GmlTetrahedra)
+    finally:
+        G.lib().GmlxSetStaging(old)
+    assert "#define   BaryDegMax 32" in s
+    assert "int16 VerTetBal[2];" in s and "int VerTetDeg;" in s and "int BaryDeg;" in s
+    if staged:
+        # tile loads into LDS before the bounds return, reads through tile-local indices;
+        # the body does not name the ball, so its entries are not read at all
+        h = s[s.index("// KERNEL HEADER"):]
+        assert h.index("gml_s_Bary[gml_i] = BaryTab[gml_x];") < h.index("__syncthreads();") \
+            < h.index("if(cnt >= count.s0)")
+        assert "Bary[23] = (BaryDeg >= 24) ? gml_s_Bary[ gml_l_VerTetBal[23] ] : BaryNul;" in s
+        assert "gml_VerTetBal[k] = v >> 4;" not in s
+    else:
         assert "Bary[23] = (BaryDeg >= 24) ? BaryTab[ gml_VerTetBal[23] ] : BaryNul;" in s
+        assert "gml_lds" not in s
+    for s in [s]:
         assert "VerRef = VerRefTab[ cnt + count.s1 ];" in s
         assert "VerCrdTab[ cnt + count.s1 ] = VerCrd;" in s
         assert "VerRefTab[ cnt + count.s1 ] =" not in s
         assert "float omega;" in s
-        assert g.link_info(G.GmlVertices, G.GmlTetrahedra) == (len(crd), 32, 0, 0)
+    assert linkinfo == (len(crd), 32, 0, 0)
 
 
 def test_host_uplink_matches_oracle(mesh):
