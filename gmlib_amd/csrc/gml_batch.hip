@@ -5,22 +5,26 @@
 // examples/MeshQuality and examples/BallQuality, applied to every mesh of
 // the batch at once):
 //   - the mean-ratio quality of every tet (the expression of
-//     tet_meanratio.cl, the same device function as GmlxTetQuality),
+//     tet_meanratio.cl; coordinates are stored with a zero fourth component,
+//     so meanratio3 gives GmlxTetQuality's values bit for bit),
 //   - the minimum of it over every vertex ball (as GmlxBallMin: an empty
 //     ball gives 0),
 //   - per mesh, the minimum and the sum of the tet qualities (FP64 sum).
 //
 // Layout.  Each mesh is cut into tiles of TV consecutive vertices (tiles
-// never straddle two meshes).  A tile lists its distinct ball tets (as 4
-// tile-local vertex indices, uint16), the distinct vertices of those tets
-// (global indices, ascending), and, for each of its own vertices, its ball
-// as tile-local tet indices in SELL-64 slabs whose rows are padded to a
-// multiple of Pad with a sentinel that points at +inf.  One work-group
-// processes a tile: stage the vertex coordinates in LDS, compute every
-// distinct tet's quality once into LDS (the tile that owns a tet's node 0
-// also writes it out and counts it for its mesh), take every ball minimum
-// from LDS, write one (min, sum) partial per tile.  A second kernel adds
-// each mesh's tile partials in tile order (deterministic).
+// never straddle two meshes).  A tet belongs to the tile that owns its node
+// 0, and tets are numbered tile by tile in that order (the plan keeps the
+// permutation back to the user's order), so a tile's own tets are one
+// contiguous range.  A tile lists its distinct ball tets -- its own first,
+// then the halo -- as 4 tile-local vertex indices (uint16), the distinct
+// vertices of those tets (global indices, ascending), and, for each of its
+// own vertices, its ball as tile-local tet indices in SELL-64 slabs whose
+// rows are padded to a multiple of Pad with a sentinel that points at +inf.
+// One work-group processes a tile: stage the vertex coordinates in LDS,
+// compute every distinct tet's quality once into LDS (its own tets are also
+// written out, coalesced, and counted for the mesh), take every ball
+// minimum from LDS, write one (min, sum) partial per tile.  A second kernel
+// adds each mesh's tile partials in tile order (deterministic).
 //
 // The work-group count is the device's resident capacity (persistent
 // kernel); each XCD walks a contiguous range of tiles so that neighbouring
@@ -52,14 +56,15 @@ struct BatchPlan {
    unsigned int *vofs = nullptr;    // [ntile+1] tile vertex list ranges
    int *vlist = nullptr;            // tile vertex lists (global vertex ids)
    unsigned int *tofs = nullptr;    // [ntile+1] tile tet ranges
-   v4u16 *tloc = nullptr;           // tile tets: 4 tile-local vertex indices
-   int *tid = nullptr;              // tile tets: global id if owned, else -1 - id
+   v4u16 *tloc = nullptr;           // tile tets (own first): 4 tile-local vertex indices
+   unsigned int *tbeg = nullptr;    // [ntile+1] first own tet of each tile (plan numbering)
    unsigned int *vbeg = nullptr;    // [ntile+1] own vertex ranges (global)
    unsigned int *cofs = nullptr;    // [ntile+1] first SELL-64 chunk of each tile
    unsigned int *loff = nullptr;    // [nchunk+1] slab offset of each chunk
    unsigned short *lball = nullptr; // ball slots: tile-local tet index (nt = sentinel)
    unsigned short *ldeg = nullptr;  // [V] padded ball length of each vertex
    unsigned int *mtile = nullptr;   // [nmesh+1] tile range of each mesh
+   std::vector<int> perm;           // host: plan tet number -> batch tet number
    size_t bytes = 0;
 };
 
@@ -88,7 +93,7 @@ struct Batch {
 template<int PAD>
 __global__ void __launch_bounds__(kBatBlock) k_batch_quality(
    long long ntile, const unsigned int *__restrict__ vofs, const int *__restrict__ vlist,
-   const unsigned int *__restrict__ tofs, const v4u16 *__restrict__ tloc, const int *__restrict__ tid,
+   const unsigned int *__restrict__ tofs, const v4u16 *__restrict__ tloc, const unsigned int *__restrict__ tbeg,
    const unsigned int *__restrict__ vbeg, const unsigned int *__restrict__ cofs, const unsigned int *__restrict__ loff,
    const unsigned short *__restrict__ lball, const unsigned short *__restrict__ ldeg,
    const v4f *__restrict__ crd, float *__restrict__ tq, float *__restrict__ vmin, double *__restrict__ part, int max_nv)
@@ -113,15 +118,15 @@ __global__ void __launch_bounds__(kBatBlock) k_batch_quality(
 
       float mn = FLT_MAX;
       double sm = 0.;
+      const unsigned int q0 = tbeg[t], nown = tbeg[t + 1] - q0;
       for(unsigned int j = threadIdx.x; j < nt; j += kBatBlock)
       {
          const v4u16 a = tloc[t0 + j];
-         const float q = meanratio(X[a.x], X[a.y], X[a.z], X[a.w]);
+         const float q = meanratio3(X[a.x], X[a.y], X[a.z], X[a.w]);
          Q[j] = q;
-         const int id = tid[t0 + j];
-         if(id >= 0)
+         if(j < nown)
          {
-            tq[id] = q;
+            tq[q0 + j] = q;
             mn = fminf(mn, q);
             sm += (double)q;
          }
@@ -192,10 +197,10 @@ __global__ void __launch_bounds__(256) k_batch_finish(int nmesh, const unsigned 
 // tile plan (host)
 // ---------------------------------------------------------------------------
 struct MeshTiles {
-   std::vector<unsigned int> nvt, ntt, nownt, nck;   // per tile: vertices, tets, own vertices, chunks
+   std::vector<unsigned int> nvt, ntt, nownt, nck, nownt_t;  // per tile: vertices, tets, own vertices, chunks, own tets
    std::vector<int> vlist;                           // global vertex ids
    std::vector<unsigned short> tloc;                 // 4 per tile tet
-   std::vector<int> tid;
+   std::vector<int> perm;                            // own tets in plan order: mesh-local tet number
    std::vector<unsigned int> ckw;                    // chunk widths (slabs)
    std::vector<unsigned short> lball;                // chunk slabs, 64 lanes each
    std::vector<unsigned short> ldeg;                 // per own vertex (mesh order)
@@ -207,7 +212,7 @@ static void MeshPlan(const Batch *b, int m, int TV, int Pad, MeshTiles &o)
 {
    const long long V = b->vstart[m + 1] - b->vstart[m], T = b->tstart[m + 1] - b->tstart[m];
    const int *tet = b->htet.data() + b->tstart[m] * 4;
-   const long long vg = b->vstart[m], tg = b->tstart[m];
+   const long long vg = b->vstart[m];
    // vertex balls, ascending tet index
    std::vector<unsigned int> bo((size_t)V + 1, 0);
    for(long long e = 0; e < T * 4; e++) bo[tet[e] + 1]++;
@@ -219,7 +224,7 @@ static void MeshPlan(const Batch *b, int m, int TV, int Pad, MeshTiles &o)
          for(int j = 0; j < 4; j++) ball[fill[tet[e * 4 + j]]++] = (int)e;
    }
    std::vector<int> mapv((size_t)V, -1), mapt((size_t)T, -1);
-   std::vector<int> ts, vs;
+   std::vector<int> ts, vs, th;
    for(long long v0 = 0; v0 < V; v0 += TV)
    {
       const long long v1 = std::min(V, v0 + TV);
@@ -227,7 +232,14 @@ static void MeshPlan(const Batch *b, int m, int TV, int Pad, MeshTiles &o)
       for(long long v = v0; v < v1; v++)
          for(unsigned int k = bo[v]; k < bo[v + 1]; k++)
             if(mapt[ball[k]] < 0) { mapt[ball[k]] = 0; ts.push_back(ball[k]); }
+      // own tets (node 0 in the tile) first, then the halo, each ascending
       std::sort(ts.begin(), ts.end());
+      th.clear();
+      size_t no = 0;
+      for(int e : ts)
+         if(tet[e * 4] >= v0 && tet[e * 4] < v1) ts[no++] = e;
+         else th.push_back(e);
+      std::copy(th.begin(), th.end(), ts.begin() + no);
       vs.clear();
       for(int e : ts)
          for(int j = 0; j < 4; j++)
@@ -240,9 +252,9 @@ static void MeshPlan(const Batch *b, int m, int TV, int Pad, MeshTiles &o)
          const int e = ts[i];
          mapt[e] = (int)i;
          for(int j = 0; j < 4; j++) o.tloc.push_back((unsigned short)mapv[tet[e * 4 + j]]);
-         const bool own = tet[e * 4] >= v0 && tet[e * 4] < v1;
-         o.tid.push_back(own ? (int)(tg + e) : -1 - (int)(tg + e));
+         if(i < no) o.perm.push_back(e);
       }
+      o.nownt_t.push_back((unsigned int)no);
       // own vertices' balls in SELL-64 chunks, rows padded to Pad with the sentinel
       const unsigned short sent = (unsigned short)ts.size();
       unsigned int nck = 0;
@@ -276,7 +288,7 @@ static void MeshPlan(const Batch *b, int m, int TV, int Pad, MeshTiles &o)
 static void FreePlan(GmlCtx *g, BatchPlan *p)
 {
    if(!p) return;
-   DevFree(g, p->vofs); DevFree(g, p->vlist); DevFree(g, p->tofs); DevFree(g, p->tloc); DevFree(g, p->tid);
+   DevFree(g, p->vofs); DevFree(g, p->vlist); DevFree(g, p->tofs); DevFree(g, p->tloc); DevFree(g, p->tbeg);
    DevFree(g, p->vbeg); DevFree(g, p->cofs); DevFree(g, p->loff); DevFree(g, p->lball); DevFree(g, p->ldeg);
    DevFree(g, p->mtile);
    g->MemSiz -= p->bytes;
@@ -301,7 +313,7 @@ static BatchPlan *BuildPlan(GmlCtx *g, const Batch *b, const std::vector<int> &m
 
    BatchPlan *p = new BatchPlan();
    p->TV = TV; p->Pad = Pad;
-   std::vector<unsigned int> vofs(1, 0), tofs(1, 0), vbeg(1, 0), cofs(1, 0), loff(1, 0), mtile(1, 0);
+   std::vector<unsigned int> vofs(1, 0), tofs(1, 0), vbeg(1, 0), cofs(1, 0), loff(1, 0), mtile(1, 0), tbeg(1, 0);
    size_t nl = 0, ntl = 0, nb = 0, nd = 0;
    for(int i = 0; i < nm; i++)
    {
@@ -312,12 +324,13 @@ static BatchPlan *BuildPlan(GmlCtx *g, const Batch *b, const std::vector<int> &m
          tofs.push_back(tofs.back() + o.ntt[k]);
          vbeg.push_back(vbeg.back() + o.nownt[k]);
          cofs.push_back(cofs.back() + o.nck[k]);
+         tbeg.push_back(tbeg.back() + o.nownt_t[k]);
          p->max_nv = std::max(p->max_nv, (int)o.nvt[k]);
          p->max_nt = std::max(p->max_nt, (int)o.ntt[k]);
       }
       for(unsigned int w : o.ckw) loff.push_back(loff.back() + w);
       mtile.push_back((unsigned int)(vofs.size() - 1));
-      nl += o.vlist.size(); ntl += o.tid.size(); nb += o.lball.size(); nd += o.ldeg.size();
+      nl += o.vlist.size(); ntl += o.tloc.size() / 4; nb += o.lball.size(); nd += o.ldeg.size();
    }
    p->ntile = (long long)vofs.size() - 1;
    p->nchunk = (long long)loff.size() - 1;
@@ -325,23 +338,21 @@ static BatchPlan *BuildPlan(GmlCtx *g, const Batch *b, const std::vector<int> &m
    p->sum_nv = vofs.back();
    p->sum_nt = tofs.back();
    // concatenate (vertex ids of a sample are renumbered to its own prefix)
-   std::vector<int> vlist(nl + 1), tid(ntl + 1);
+   std::vector<int> vlist(nl + 1);
+   p->perm.resize(tbeg.back());
    std::vector<unsigned short> tloc(ntl * 4 + 4), lball(nb + 64), ldeg(nd + 1);
    {
-      size_t a = 0, c = 0, e = 0, f = 0;
+      size_t a = 0, c = 0, e = 0, f = 0, q = 0;
       long long vshift = 0, tshift = 0;
       for(int i = 0; i < nm; i++)
       {
          auto &o = mt[i];
-         const long long dv = b->vstart[ms[i]] - vshift, dt = b->tstart[ms[i]] - tshift;
+         const long long dv = b->vstart[ms[i]] - vshift;
          for(int v : o.vlist) vlist[a++] = (int)(v - dv);
-         for(size_t k = 0; k < o.tid.size(); k++)
-         {
-            const int id = o.tid[k];
-            tid[c + k] = id >= 0 ? (int)(id - dt) : (int)(id + dt);
-         }
+         // own tets in plan order -> their number in the (sample's) batch order
+         for(int x : o.perm) p->perm[q++] = (int)(tshift + x);
          std::copy(o.tloc.begin(), o.tloc.end(), tloc.begin() + c * 4);
-         c += o.tid.size();
+         c += o.tloc.size() / 4;
          std::copy(o.lball.begin(), o.lball.end(), lball.begin() + e);
          e += o.lball.size();
          std::copy(o.ldeg.begin(), o.ldeg.end(), ldeg.begin() + f);
@@ -363,7 +374,7 @@ static BatchPlan *BuildPlan(GmlCtx *g, const Batch *b, const std::vector<int> &m
    up((void **)&p->loff, loff.data(), loff.size() * 4);
    up((void **)&p->mtile, mtile.data(), mtile.size() * 4);
    up((void **)&p->vlist, vlist.data(), vlist.size() * 4);
-   up((void **)&p->tid, tid.data(), tid.size() * 4);
+   up((void **)&p->tbeg, tbeg.data(), tbeg.size() * 4);
    up((void **)&p->tloc, tloc.data(), tloc.size() * 2);
    up((void **)&p->lball, lball.data(), lball.size() * 2);
    up((void **)&p->ldeg, ldeg.data(), ldeg.size() * 2);
@@ -409,7 +420,7 @@ static bool LaunchQuality(GmlCtx *g, const BatchPlan *p, const v4f *crd, float *
 {
    const size_t lds = PlanLds(p);
 #define GML_Q(P) hipLaunchKernelGGL(k_batch_quality<P>, dim3(grid), dim3(kBatBlock), lds, g->stream, p->ntile, p->vofs, \
-      p->vlist, p->tofs, p->tloc, p->tid, p->vbeg, p->cofs, p->loff, p->lball, p->ldeg, crd, tq, vmin, part, p->max_nv)
+      p->vlist, p->tofs, p->tloc, p->tbeg, p->vbeg, p->cofs, p->loff, p->lball, p->ldeg, crd, tq, vmin, part, p->max_nv)
    if(p->Pad == 1) GML_Q(1);
    else if(p->Pad == 2) GML_Q(2);
    else GML_Q(4);
@@ -648,9 +659,15 @@ int GmlxGetBatchField(size_t GmlIdx, int BatIdx, float *TetQal, float *VerMin)
    GML_CTX(g, GmlIdx);
    Batch *b = Bat(g, BatIdx);
    if(!b) return -1;
-   if((TetQal && !DevCopy(g, TetQal, b->tq, (size_t)b->T * 4, 2))
-   || (VerMin && !DevCopy(g, VerMin, b->vmin, (size_t)b->V * 4, 2)))
-      return -6;
+   if(TetQal)
+   {
+      if(!b->plan) return -1;
+      std::vector<float> q((size_t)b->T);
+      if(!DevCopy(g, q.data(), b->tq, (size_t)b->T * 4, 2)) return -6;
+      const std::vector<int> &pm = b->plan->perm;
+      for(long long i = 0; i < b->T; i++) TetQal[pm[i]] = q[i];
+   }
+   if(VerMin && !DevCopy(g, VerMin, b->vmin, (size_t)b->V * 4, 2)) return -6;
    return 1;
 }
 

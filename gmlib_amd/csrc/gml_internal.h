@@ -118,10 +118,11 @@ struct SweepPlan {
    unsigned int *tofs = nullptr;            // [ntile+1] offsets into tloc
    unsigned short *tloc = nullptr;          // 4 tile-local vertex indices per distinct tet
    unsigned short *lidx = nullptr;          // ball slot -> tile-local tet index (SELL-64)
-   // operator form: per vertex, its 1-ring as (tile-local index << 8 | count
-   // of ball tets containing the neighbour), SELL-64 over the vertices
-   unsigned int *nbr = nullptr, *noff = nullptr;
-   int *ncnt = nullptr;
+   // operator form: per vertex, its 1-ring without itself as 16-bit
+   // (tile-local index << 4 | count of ball tets containing the neighbour),
+   // SELL-64 over the vertices, and (deg << 16 | ring length)
+   unsigned short *nbr = nullptr;
+   unsigned int *noff = nullptr, *meta = nullptr;
    long long nslab = 0;
    size_t bytes = 0;
 };

@@ -2,8 +2,10 @@
 // ratio quality, per-vertex ball minimum, and Jacobi barycentre smoothing
 // (tet scatter + vertex-ball gather).  They compute exactly what the
 // generated kernels built from gmlib_amd/kernels/*.cl compute, expression by
-// expression, so their results are bit-identical to the drop-in path; what
-// differs is how they move data:
+// expression, so their results are bit-identical to the drop-in path (the
+// quality kernel drops the terms of the coordinates' fourth component, which
+// the C API always stores as 0, see meanratio3); what differs is how they
+// move data:
 //   - element rows are streamed with non-temporal loads (read once),
 //   - the vertex ball is read from its SELL-64 slabs with one 256-byte load
 //     per slot and wave, and the barycentre gathers of a row are issued in
@@ -52,7 +54,7 @@ __global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *_
    long long t = (long long)xcd_block(remap) * blockDim.x + threadIdx.x;
    if(t >= nt) return;
    v4i v = ld_nt(&tet[t]);
-   q[t] = meanratio(crd[v.x], crd[v.y], crd[v.z], crd[v.w]);
+   q[t] = meanratio3(crd[v.x], crd[v.y], crd[v.z], crd[v.w]);
 }
 
 __global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
@@ -184,15 +186,29 @@ __global__ void __launch_bounds__(256) k_sweep_fused(long long nv, const unsigne
 // Operator-form Jacobi sweep over one tile: stage the tile's 1-ring
 // coordinates in LDS, then x_v += omega (sum_u c_vu x_u / (4 deg v) - x_v)
 // with the 1-ring read as (tile-local index << 8 | c_vu) slots.
+// The vertex's own data (degree word, ref, coordinates, first 16 ring
+// slots) are loaded before the tile staging so that both latencies overlap.
 __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned int *__restrict__ vofs,
-                                                   const int *__restrict__ vlist, const int *__restrict__ deg,
-                                                   const int *__restrict__ ncnt, const unsigned int *__restrict__ noff,
-                                                   const unsigned int *__restrict__ nbr, const int *__restrict__ ref,
+                                                   const int *__restrict__ vlist, const unsigned int *__restrict__ meta,
+                                                   const unsigned int *__restrict__ noff,
+                                                   const unsigned short *__restrict__ nbr, const int *__restrict__ ref,
                                                    const v4f *__restrict__ xo, v4f *__restrict__ xn, float omega, int remap)
 {
    extern __shared__ v4f lds[];
    const unsigned int tile = xcd_block(remap);
    const int tid = threadIdx.x, TV = blockDim.x;
+   const long long v = (long long)tile * TV + tid;
+   const bool act = v < nv;
+   const unsigned int mt = act ? meta[v] : 0u;
+   const int d = (int)(mt >> 16), m = (int)(mt & 0xffffu);
+   const unsigned short *row = nbr + (act ? (size_t)noff[v >> 6] * 64 + (v & 63) : 0);
+   unsigned int e0[16];
+#pragma unroll
+   for(int u = 0; u < 16; u++)
+      e0[u] = u < m ? __builtin_nontemporal_load(&row[u * 64]) : 0u;
+   v4f p = act ? xo[v] : (v4f){0.f, 0.f, 0.f, 0.f};
+   const int rf = act ? ref[v] : 1;
+
    const unsigned int v0 = vofs[tile], nvt = vofs[tile + 1] - v0;
    v4f *X = lds;
    for(unsigned int i = tid; i < nvt; i += 4 * TV)
@@ -210,12 +226,12 @@ __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned i
          if(vi[u] >= 0) X[i + u * TV] = x[u];
    }
    __syncthreads();
-   const long long v = (long long)tile * TV + tid;
-   if(v >= nv) return;
-   const int d = deg[v], m = ncnt[v];
-   const unsigned int *row = nbr + (size_t)noff[v >> 6] * 64 + (v & 63);
-   v4f s = {0.f, 0.f, 0.f, 0.f};
-   for(int k = 0; k < m; k += 8)
+   if(!act) return;
+   v4f s = p * (float)d;                       // c_vv x_v, c_vv = deg v
+#pragma unroll
+   for(int u = 0; u < 16; u++)
+      if(u < m) s += X[e0[u] >> 4] * (float)(e0[u] & 15u);
+   for(int k = 16; k < m; k += 8)
    {
       unsigned int e[8];
 #pragma unroll
@@ -223,10 +239,9 @@ __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned i
          e[u] = k + u < m ? __builtin_nontemporal_load(&row[(k + u) * 64]) : 0u;
 #pragma unroll
       for(int u = 0; u < 8; u++)
-         if(k + u < m) s += X[e[u] >> 8] * (float)(e[u] & 255u);
+         if(k + u < m) s += X[e[u] >> 4] * (float)(e[u] & 15u);
    }
-   v4f p = xo[v];
-   if(!ref[v] && d > 0)
+   if(!rf && d > 0)
    {
       s *= 0.25f / (float)d;
       p = p + (s - p) * omega;
@@ -325,8 +340,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
          v4f *xn = (v4f *)g->xalt;
          int rc = TimedLaunch(g, GmlxBuiltinSweep, [&]() {
             hipLaunchKernelGGL(k_sweep_op, dim3((unsigned int)p->ntile), dim3(p->TV), lds, g->stream, nv,
-                               p->vofs, p->vlist, (const int *)g->dat[u->DegIdx].dev, p->ncnt, p->noff, p->nbr,
-                               ref, xo, xn, (float)omega, sf.remap);
+                               p->vofs, p->vlist, p->meta, p->noff, p->nbr, ref, xo, xn, (float)omega, sf.remap);
          });
          if(rc != 1) return rc;
          void *t = vd->dev; vd->dev = g->xalt; g->xalt = t;

@@ -15,8 +15,10 @@
 // The operator form (op = 1) keeps the tile vertex lists and replaces the
 // tet lists by each vertex's 1-ring: the mean ball barycentre of v is
 // sum_u c_vu x_u / (4 deg v), c_vu = number of ball tets of v that contain
-// u (c_vv = deg v), so a sweep applies a fixed sparse operator; the 1-ring
-// is stored as (tile-local index << 8 | c_vu), ascending local index.
+// u (c_vv = deg v), so a sweep applies a fixed sparse operator.  The ring
+// of v without v itself is stored as 16-bit entries (tile-local index << 4
+// | c_vu), ascending local index (a count above 15 continues in the next
+// entry); one 32-bit word per vertex holds (deg v << 16 | ring length).
 
 #include "gml_internal.h"
 
@@ -48,7 +50,7 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
    std::vector<std::vector<int>> tv((size_t)ntile);
    std::vector<std::vector<unsigned short>> tl((size_t)ntile);
    std::vector<unsigned short> lidx(op ? 1 : (size_t)u->nslab * 64 + 64, 0);
-   std::vector<std::vector<unsigned int>> ring(op ? (size_t)nv : 0);
+   std::vector<std::vector<unsigned short>> ring(op ? (size_t)nv : 0);
    std::atomic<bool> fits(true);
 
    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -87,14 +89,18 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
                   std::sort(cnt.begin(), cnt.end());
                   auto &R = ring[v];
                   R.clear();
+                  const unsigned int self = (unsigned int)(std::lower_bound(vs.begin(), vs.end(), (int)v) - vs.begin());
                   for(size_t i = 0; i < cnt.size();)
                   {
                      size_t j = i;
                      while(j < cnt.size() && cnt[j] == cnt[i]) j++;
-                     if(j - i > 255) fits = false;
-                     R.push_back((cnt[i] << 8) | (unsigned int)(j - i));
+                     if(cnt[i] > 4095) fits = false;
+                     if(cnt[i] != self)
+                        for(size_t c = j - i; c > 0; c -= std::min<size_t>(c, 15))
+                           R.push_back((unsigned short)((cnt[i] << 4) | std::min<size_t>(c, 15)));
                      i = j;
                   }
+                  if(R.size() > 65535 || deg[v] > 65535) fits = false;
                }
                tv[t] = vs;
                continue;
@@ -133,7 +139,7 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
       for(long long t = 0; t < ntile; t++)
          std::copy(tv[t].begin(), tv[t].end(), vlist.begin() + vofs[t]);
       std::vector<unsigned int> noff((size_t)nchunk + 1, 0);
-      std::vector<int> ncnt((size_t)nv);
+      std::vector<unsigned int> meta((size_t)nv);
       for(long long c = 0; c < nchunk; c++)
       {
          unsigned int w = 0;
@@ -141,30 +147,30 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
          noff[c + 1] = noff[c] + w;
       }
       p->nslab = noff[nchunk];
-      std::vector<unsigned int> nbr((size_t)p->nslab * 64 + 64, 0);
+      std::vector<unsigned short> nbr((size_t)p->nslab * 64 + 64, 0);
       for(long long v = 0; v < nv; v++)
       {
-         ncnt[v] = (int)ring[v].size();
-         p->sum_ring += ncnt[v];
+         meta[v] = ((unsigned int)deg[v] << 16) | (unsigned int)ring[v].size();
+         p->sum_ring += (long long)ring[v].size();
          for(size_t k = 0; k < ring[v].size(); k++)
             nbr[((size_t)noff[v >> 6] + k) * 64 + (v & 63)] = ring[v][k];
       }
       p->vofs = (unsigned int *)DevAlloc(g, vofs.size() * 4);
       p->vlist = (int *)DevAlloc(g, vlist.size() * 4);
       p->noff = (unsigned int *)DevAlloc(g, noff.size() * 4);
-      p->ncnt = (int *)DevAlloc(g, ncnt.size() * 4);
-      p->nbr = (unsigned int *)DevAlloc(g, nbr.size() * 4);
-      if(!p->vofs || !p->vlist || !p->noff || !p->ncnt || !p->nbr
+      p->meta = (unsigned int *)DevAlloc(g, meta.size() * 4);
+      p->nbr = (unsigned short *)DevAlloc(g, nbr.size() * 2);
+      if(!p->vofs || !p->vlist || !p->noff || !p->meta || !p->nbr
       || !DevCopy(g, p->vofs, vofs.data(), vofs.size() * 4, 1)
       || !DevCopy(g, p->vlist, vlist.data(), vlist.size() * 4, 1)
       || !DevCopy(g, p->noff, noff.data(), noff.size() * 4, 1)
-      || !DevCopy(g, p->ncnt, ncnt.data(), ncnt.size() * 4, 1)
-      || !DevCopy(g, p->nbr, nbr.data(), nbr.size() * 4, 1))
+      || !DevCopy(g, p->meta, meta.data(), meta.size() * 4, 1)
+      || !DevCopy(g, p->nbr, nbr.data(), nbr.size() * 2, 1))
       {
          FreeSweepPlan(g, p);
          return nullptr;
       }
-      p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + ncnt.size() * 4 + nbr.size() * 4;
+      p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + meta.size() * 4 + nbr.size() * 2;
       p->sum_nv = vofs[ntile];
       g->MemSiz += p->bytes;
       DbgPrint(g, "Operator sweep plan: %lld tiles of %d vertices, %u tile vertices, %lld ring slabs, %zu MB\n",
@@ -288,7 +294,7 @@ void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
    if(!p) return;
    DevFree(g, p->vofs); DevFree(g, p->tofs); DevFree(g, p->vlist);
    DevFree(g, p->tloc); DevFree(g, p->lidx);
-   DevFree(g, p->nbr); DevFree(g, p->noff); DevFree(g, p->ncnt);
+   DevFree(g, p->nbr); DevFree(g, p->noff); DevFree(g, p->meta);
    g->MemSiz -= p->bytes;
    delete p;
 }

@@ -110,7 +110,9 @@ enum gmlx_builtin { GmlxBuiltinQuality, GmlxBuiltinBallMin, GmlxBuiltinScatter, 
                     GmlxBuiltinSweep, GmlxBuiltinSpMV, GmlxBuiltinCgSpMV, GmlxBuiltinCgUpdate,
                     GmlxBuiltinBatch };
 
-/* mean-ratio quality of every tet into QalIdx (a GmlFlt tet solution) */
+/* mean-ratio quality of every tet into QalIdx (a GmlFlt tet solution), from
+   the x, y, z of the vertices (the fourth coordinate slot, 0 whenever the
+   coordinates come through GmlSetDataLine/Block, is not read) */
 int    GmlxTetQuality(size_t GmlIdx, int QalIdx);
 /* per-vertex minimum of a GmlFlt tet field over the vertex ball */
 int    GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx);
