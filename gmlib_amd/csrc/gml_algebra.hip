@@ -80,13 +80,13 @@ __device__ __forceinline__ void row_product(int r, int l, int d, size_t base, co
    for(; k + 1 < d; k += 2)
    {
       const size_t s0 = base + k, s1 = s0 + 1;
-      const int j0 = col[s0 * 64 + l], j1 = col[s1 * 64 + l];
+      const int j0 = __builtin_nontemporal_load(&col[s0 * 64 + l]), j1 = __builtin_nontemporal_load(&col[s1 * 64 + l]);
       T v0[B * B], v1[B * B];
 #pragma unroll
       for(int e = 0; e < B * B; e++)
       {
-         v0[e] = val[(s0 * (B * B) + e) * 64 + l];
-         v1[e] = val[(s1 * (B * B) + e) * 64 + l];
+         v0[e] = __builtin_nontemporal_load(&val[(s0 * (B * B) + e) * 64 + l]);
+         v1[e] = __builtin_nontemporal_load(&val[(s1 * (B * B) + e) * 64 + l]);
       }
       X x0[B], x1[B];
 #pragma unroll

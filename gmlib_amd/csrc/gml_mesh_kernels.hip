@@ -48,13 +48,20 @@ void FreeBuiltinStats(GmlCtx *g)
       }
 }
 
+// two tets per thread (a work-group covers 2 x its size, each half
+// coalesced): eight coordinate gathers in flight per thread
 __global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *__restrict__ tet,
                                                        const v4f *__restrict__ crd, float *__restrict__ q, int remap)
 {
-   long long t = (long long)xcd_block(remap) * blockDim.x + threadIdx.x;
-   if(t >= nt) return;
-   v4i v = ld_nt(&tet[t]);
-   q[t] = meanratio3(crd[v.x], crd[v.y], crd[v.z], crd[v.w]);
+   const long long t0 = (long long)xcd_block(remap) * 2 * blockDim.x + threadIdx.x, t1 = t0 + blockDim.x;
+   if(t0 >= nt) return;
+   const bool two = t1 < nt;
+   const v4i a = ld_nt(&tet[t0]);
+   const v4i b = two ? ld_nt(&tet[t1]) : a;
+   const v4f a0 = crd[a.x], a1 = crd[a.y], a2 = crd[a.z], a3 = crd[a.w];
+   const v4f b0 = crd[b.x], b1 = crd[b.y], b2 = crd[b.z], b3 = crd[b.w];
+   q[t0] = meanratio3(a0, a1, a2, a3);
+   if(two) q[t1] = meanratio3(b0, b1, b2, b3);
 }
 
 __global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
@@ -280,7 +287,7 @@ int GmlxTetQuality(size_t GmlIdx, int QalIdx)
    g->dat[QalIdx].host_valid = false;
    if(!nt) return 1;
    return TimedLaunch(g, GmlxBuiltinQuality, [&]() {
-      hipLaunchKernelGGL(k_tet_quality, dim3((unsigned int)((nt + sh.grp - 1) / sh.grp)), dim3(sh.grp), 0, g->stream,
+      hipLaunchKernelGGL(k_tet_quality, dim3((unsigned int)((nt + 2 * sh.grp - 1) / (2 * sh.grp))), dim3(sh.grp), 0, g->stream,
                          nt, tet, crd, q, sh.remap);
    });
 }

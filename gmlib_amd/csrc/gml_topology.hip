@@ -12,9 +12,10 @@
 // Storage is SELL-64: chunk c of 64 consecutive source entities keeps its
 // entries slab-major, ent[(off[c] + k) * 64 + lane], k < width(c) = the
 // chunk's largest degree, so a wave reads slot k of 64 rows as one 256-byte
-// coalesced load.  Vertex balls are built on the GPU (degree count, chunk
-// widths, offset scan, atomic fill, per-row descending sort); edge and face
-// shells and downlinks are built on the host.
+// coalesced load.  Vertex balls are built on the GPU (degree count, whose
+// atomic returns also give each (element, node) its slot; chunk widths,
+// offset scan, atomic-free fill, per-row descending sort in registers); edge
+// and face shells and downlinks are built on the host.
 
 #include "gml_internal.h"
 
@@ -27,21 +28,24 @@ namespace gml {
 // ---------------------------------------------------------------------------
 // GPU vertex-ball builder
 // ---------------------------------------------------------------------------
+// degree of every vertex; rank[e * nnod + j] = the slot the atomic gave
+// node j of element e in its vertex's row (any order: rows are sorted later)
 __global__ void k_ball_count(const int *__restrict__ ele, int stride, int nnod, long long nele,
-                             long long nver, int *__restrict__ deg, int *__restrict__ bad)
+                             long long nver, int *__restrict__ deg, int *__restrict__ rank, int *__restrict__ bad)
 {
    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(e >= nele) return;
    for(int j = 0; j < nnod; j++)
    {
       int v = ele[e * stride + j];
-      if(v < 0 || v >= nver) { atomicOr(bad, 1); continue; }
-      atomicAdd(&deg[v], 1);
+      if(v < 0 || v >= nver) { atomicOr(bad, 1); rank[e * nnod + j] = -1; continue; }
+      rank[e * nnod + j] = atomicAdd(&deg[v], 1);
    }
 }
 
-// width of each 64-row chunk = its largest degree; global max degree and the
-// first row wider than the base width
+// width of each 64-row chunk = its largest degree, and the first row wider
+// than the base width (the maximum degree is the largest chunk width, taken
+// by the scan)
 __global__ void k_ball_chunks(const int *__restrict__ deg, long long nver, unsigned int *__restrict__ w,
                               int *__restrict__ stats, int baswid)
 {
@@ -54,7 +58,6 @@ __global__ void k_ball_chunks(const int *__restrict__ deg, long long nver, unsig
    {
       long long c = v >> 6;
       if(c * 64 < nver) w[c] = (unsigned int)m;
-      atomicMax(&stats[0], m);
    }
    if(v < nver && d > baswid)
       atomicMin(&stats[1], (int)v);
@@ -62,21 +65,27 @@ __global__ void k_ball_chunks(const int *__restrict__ deg, long long nver, unsig
 
 // exclusive scan of the chunk widths (one block; nchunk is at most a few
 // million, each thread scans a contiguous run)
-__global__ void k_ball_scan(const unsigned int *__restrict__ w, long long nchunk, unsigned int *__restrict__ off)
+__global__ void k_ball_scan(const unsigned int *__restrict__ w, long long nchunk, unsigned int *__restrict__ off,
+                            int *__restrict__ stats)
 {
    __shared__ unsigned long long part[1024];
+   __shared__ unsigned int wmax[1024];
    const int t = threadIdx.x, nt = blockDim.x;
    long long per = (nchunk + nt - 1) / nt;
    long long b = per * t, e = min(b + per, nchunk);
    unsigned long long s = 0;
-   for(long long i = b; i < e; i++) s += w[i];
+   unsigned int mx = 0;
+   for(long long i = b; i < e; i++) { s += w[i]; mx = max(mx, w[i]); }
    part[t] = s;
+   wmax[t] = mx;
    __syncthreads();
    if(t == 0)
    {
       unsigned long long acc = 0;
-      for(int i = 0; i < nt; i++) { unsigned long long x = part[i]; part[i] = acc; acc += x; }
+      unsigned int m = 0;
+      for(int i = 0; i < nt; i++) { unsigned long long x = part[i]; part[i] = acc; acc += x; m = max(m, wmax[i]); }
       off[nchunk] = (unsigned int)acc;
+      stats[0] = (int)m;
    }
    __syncthreads();
    unsigned long long acc = part[t];
@@ -84,20 +93,21 @@ __global__ void k_ball_scan(const unsigned int *__restrict__ w, long long nchunk
 }
 
 __global__ void k_ball_fill(const int *__restrict__ ele, int stride, int nnod, long long nele, long long nver,
-                            const unsigned int *__restrict__ off, int *__restrict__ fill, int *__restrict__ ent)
+                            const unsigned int *__restrict__ off, const int *__restrict__ rank, int *__restrict__ ent)
 {
    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(e >= nele) return;
    for(int j = 0; j < nnod; j++)
    {
-      int v = ele[e * stride + j];
-      if(v < 0 || v >= nver) continue;
-      int p = atomicAdd(&fill[v], 1);
+      int v = ele[e * stride + j], p = rank[e * nnod + j];
+      if(v < 0 || v >= nver || p < 0) continue;
       ent[((size_t)off[v >> 6] + p) * 64 + (v & 63)] = (int)((e << 4) | j);
    }
 }
 
-// descending sort of each row (the reference's chain order)
+// descending sort of each row (the reference's chain order): rows of up to
+// 32 entries with a bitonic network on registers (every index compile-time,
+// so nothing goes to scratch), longer rows in place in memory
 __global__ void k_ball_sort(const int *__restrict__ deg, long long nver, const unsigned int *__restrict__ off,
                             int *__restrict__ ent)
 {
@@ -105,17 +115,31 @@ __global__ void k_ball_sort(const int *__restrict__ deg, long long nver, const u
    if(v >= nver) return;
    int d = deg[v];
    int *row = ent + (size_t)off[v >> 6] * 64 + (v & 63);
-   if(d <= 64)
+   if(d <= 32)
    {
-      int r[64];
-      for(int k = 0; k < d; k++) r[k] = row[(size_t)k * 64];
-      for(int k = 1; k < d; k++)
-      {
-         int x = r[k], i = k - 1;
-         while(i >= 0 && r[i] < x) { r[i + 1] = r[i]; i--; }
-         r[i + 1] = x;
-      }
-      for(int k = 0; k < d; k++) row[(size_t)k * 64] = r[k];
+      int r[32];
+#pragma unroll
+      for(int k = 0; k < 32; k++) r[k] = k < d ? row[(size_t)k * 64] : (int)0x80000000;
+#pragma unroll
+      for(int sz = 2; sz <= 32; sz <<= 1)
+#pragma unroll
+         for(int st = sz >> 1; st > 0; st >>= 1)
+#pragma unroll
+            for(int i = 0; i < 32; i++)
+            {
+               const int j = i ^ st;
+               if(j > i)
+               {
+                  // descending overall: blocks with (i & sz) == 0 descend
+                  const bool desc = (i & sz) == 0;
+                  const int a = r[i], b = r[j];
+                  r[i] = desc ? max(a, b) : min(a, b);
+                  r[j] = desc ? min(a, b) : max(a, b);
+               }
+            }
+#pragma unroll
+      for(int k = 0; k < 32; k++)
+         if(k < d) row[(size_t)k * 64] = r[k];
    }
    else
    {
@@ -136,7 +160,7 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
 {
    long long nchunk = (nver + 63) / 64;
    unsigned int *w = nullptr, *off = nullptr;
-   int *stats = nullptr, *fill = nullptr, *ent = nullptr;
+   int *stats = nullptr, *rank = nullptr, *ent = nullptr;
    int hs[3] = {0, 0, 0};
    const int B = 256;
    unsigned int nbe = (unsigned int)((nele + B - 1) / B), nbv = (unsigned int)((nver + 63 + B - 1) / B);
@@ -144,7 +168,8 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
 
    if(hipMalloc(&w, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
    || hipMalloc(&off, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
-   || hipMalloc(&stats, 4 * sizeof(int)) != hipSuccess)
+   || hipMalloc(&stats, 4 * sizeof(int)) != hipSuccess
+   || hipMalloc(&rank, (size_t)nele * nnod * sizeof(int) + 4) != hipSuccess)
       goto fail;
    HIPCHK(hipMemsetAsync(deg, 0, nver * sizeof(int), s));
    HIPCHK(hipMemsetAsync(w, 0, (nchunk + 1) * sizeof(unsigned int), s));
@@ -153,9 +178,9 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
       HIPCHK(hipMemcpyAsync(stats, init, sizeof(init), hipMemcpyHostToDevice, s));
    }
    if(nele > 0)
-      hipLaunchKernelGGL(k_ball_count, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, deg, stats + 2);
+      hipLaunchKernelGGL(k_ball_count, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, deg, rank, stats + 2);
    hipLaunchKernelGGL(k_ball_chunks, dim3(nbv), dim3(B), 0, s, deg, nver, w, stats, baswid);
-   hipLaunchKernelGGL(k_ball_scan, dim3(1), dim3(1024), 0, s, w, nchunk, off);
+   hipLaunchKernelGGL(k_ball_scan, dim3(1), dim3(1024), 0, s, w, nchunk, off, stats);
    {
       unsigned int nslab = 0;
       HIPCHK(hipMemcpyAsync(&nslab, off + nchunk, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
@@ -167,19 +192,17 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
          goto fail;
       }
       *nslab_out = nslab;
-      if(hipMalloc(&ent, ((size_t)nslab * 64 + 64) * sizeof(int)) != hipSuccess
-      || hipMalloc(&fill, nver * sizeof(int) + 4) != hipSuccess)
+      if(hipMalloc(&ent, ((size_t)nslab * 64 + 64) * sizeof(int)) != hipSuccess)
          goto fail;
       HIPCHK(hipMemsetAsync(ent, 0, ((size_t)nslab * 64 + 64) * sizeof(int), s));
-      HIPCHK(hipMemsetAsync(fill, 0, nver * sizeof(int) + 4, s));
    }
    if(nele > 0)
-      hipLaunchKernelGGL(k_ball_fill, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, off, fill, ent);
+      hipLaunchKernelGGL(k_ball_fill, dim3(nbe), dim3(B), 0, s, ele, stride, nnod, nele, nver, off, rank, ent);
    hipLaunchKernelGGL(k_ball_sort, dim3((unsigned int)((nver + B - 1) / B)), dim3(B), 0, s, deg, nver, off, ent);
    if(hipStreamSynchronize(s) != hipSuccess) goto fail;
    (void)hipFree(w);
    (void)hipFree(stats);
-   (void)hipFree(fill);
+   (void)hipFree(rank);
    *maxdeg_out = hs[0];
    *firsthigh_out = hs[1] == 0x7fffffff ? -1 : hs[1];
    *off_out = off;
@@ -187,7 +210,7 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
    return 1;
 fail:
    (void)hipGetLastError();
-   (void)hipFree(w); (void)hipFree(off); (void)hipFree(stats); (void)hipFree(fill); (void)hipFree(ent);
+   (void)hipFree(w); (void)hipFree(off); (void)hipFree(stats); (void)hipFree(rank); (void)hipFree(ent);
    return 0;
 }
 

@@ -38,7 +38,7 @@ def body(name):
 
 
 def src_of(g, k):
-    return C.cast(g.L.GmlxGetKernelSource(g.h, k), C.c_char_p).value.decode()
+    return g.L.GmlxGetKernelSource(g.h, k).decode()
 
 
 def both(g, src, name, typ, args):
