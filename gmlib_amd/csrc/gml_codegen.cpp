@@ -402,10 +402,10 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
             s += Fmt("      const unsigned short *gml_lrow = gml_lidx + (size_t)%sOff[gml_e >> 6] * 64 + (gml_e & 63);\n", n);
          s += Fmt("#pragma unroll\n      for(int k = 0; k < %d; k++)\n      {\n", w);
          if(st)
-            s += Fmt("         gml_l_%s[k] = k < gml_d ? gml_lrow[k * 64] : 0;\n", n);
+            s += Fmt("         gml_l_%s[k] = k < gml_d ? __builtin_nontemporal_load(&gml_lrow[k * 64]) : 0;\n", n);
          if(!st || ball_ids)
          {
-            s += "         int v = k < gml_d ? gml_row[k * 64] : 0;\n";
+            s += "         int v = k < gml_d ? __builtin_nontemporal_load(&gml_row[k * 64]) : 0;\n";
             if(a.FlgTab & GmlVoyeurs)
                s += Fmt("         %s[k] = (char)(v & 7);\n", a.VoyNam.c_str());
             s += Fmt("         gml_%s[k] = v >> 4;\n", n);
@@ -425,7 +425,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       }
       if(a.kind == AkDeg)
       {
-         s += Fmt("   %s = %sTab[ cnt + count.s1 ];\n\n", n, n);
+         s += Fmt("   %s = __builtin_nontemporal_load(&%sTab[ cnt + count.s1 ]);\n\n", n, n);
          continue;
       }
 
@@ -472,6 +472,9 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
             else if(L && L->LnkDir == 0)
                s += Fmt("   %s%s%s = %s ? %sTab[ %s ]%s : %sNul;\n",
                         n, td2.c_str(), td1.c_str(), lnk.c_str(), n, lnk.c_str(), td1.c_str(), n);
+            else if(!L)   // the entity's own line: read once, streamed past the caches
+               s += Fmt("   %s%s%s = __builtin_nontemporal_load(&%sTab[ %s ]%s);\n", n, td2.c_str(), td1.c_str(), n,
+                        lnk.c_str(), td1.c_str());
             else
                s += Fmt("   %s%s%s = %sTab[ %s ]%s;\n", n, td2.c_str(), td1.c_str(), n, lnk.c_str(), td1.c_str());
          }
@@ -490,10 +493,11 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    {
       if(!(a.FlgTab & GmlWriteMode) || (a.FlgTab & GmlManual)) continue;
       if(a.NmbItm == 1)
-         s += Fmt("   %sTab[ cnt + count.s1 ] = %s;\n", a.nam.c_str(), a.nam.c_str());
+         s += Fmt("   __builtin_nontemporal_store(%s, &%sTab[ cnt + count.s1 ]);\n", a.nam.c_str(), a.nam.c_str());
       else
          for(int c = 0; c < a.NmbItm; c++)
-            s += Fmt("   %sTab[ cnt + count.s1 ][%d] = %s[%d];\n", a.nam.c_str(), c, a.nam.c_str(), c);
+            s += Fmt("   __builtin_nontemporal_store(%s[%d], &%sTab[ cnt + count.s1 ][%d]);\n", a.nam.c_str(), c,
+                     a.nam.c_str(), c);
    }
    s += "}\n";
    return s;

@@ -60,8 +60,8 @@ __global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *_
    const v4i b = two ? ld_nt(&tet[t1]) : a;
    const v4f a0 = crd[a.x], a1 = crd[a.y], a2 = crd[a.z], a3 = crd[a.w];
    const v4f b0 = crd[b.x], b1 = crd[b.y], b2 = crd[b.z], b3 = crd[b.w];
-   q[t0] = meanratio3(a0, a1, a2, a3);
-   if(two) q[t1] = meanratio3(b0, b1, b2, b3);
+   __builtin_nontemporal_store(meanratio3(a0, a1, a2, a3), &q[t0]);
+   if(two) __builtin_nontemporal_store(meanratio3(b0, b1, b2, b3), &q[t1]);
 }
 
 __global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned i
       s *= 0.25f / (float)d;
       p = p + (s - p) * omega;
    }
-   xn[v] = p;
+   __builtin_nontemporal_store(p, &xn[v]);
 }
 
 static Shape g_shape[kNmbBuiltin];

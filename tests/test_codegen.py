@@ -41,10 +41,10 @@ def test_downlink_kernel_contract(mesh):
         # link argument first, then data in user order
         assert hdr.index("int4 *TetVerTab") < hdr.index("float4 *VerCrdTab") < hdr.index("float *qalTab")
         assert "int       TetIdx = cnt + count.s1;" in hdr
-        assert "TetVer = TetVerTab[ cnt + count.s1 ];" in hdr
+        assert "TetVer = __builtin_nontemporal_load(&TetVerTab[ cnt + count.s1 ]);" in hdr
         for i in range(4):
             assert "VerCrd[%d] = VerCrdTab[ TetVer.s%d ];" % (i, i) in hdr
-        assert "qalTab[ cnt + count.s1 ] = qal;" in hdr
+        assert "__builtin_nontemporal_store(qal, &qalTab[ cnt + count.s1 ]);" in hdr
         assert "float4 VerCrd[4];" in hdr
 
 
@@ -77,9 +77,9 @@ def test_uplink_kernel_contract_with_ref_and_params(mesh, staged):
         assert "Bary[23] = (BaryDeg >= 24) ? BaryTab[ gml_VerTetBal[23] ] : BaryNul;" in s
         assert "gml_lds" not in s
     for s in [s]:
-        assert "VerRef = VerRefTab[ cnt + count.s1 ];" in s
-        assert "VerCrdTab[ cnt + count.s1 ] = VerCrd;" in s
-        assert "VerRefTab[ cnt + count.s1 ] =" not in s
+        assert "VerRef = __builtin_nontemporal_load(&VerRefTab[ cnt + count.s1 ]);" in s
+        assert "__builtin_nontemporal_store(VerCrd, &VerCrdTab[ cnt + count.s1 ]);" in s
+        assert "__builtin_nontemporal_store(VerRef" not in s
         assert "float omega;" in s
     assert linkinfo == (len(crd), 32, 0, 0)
 
