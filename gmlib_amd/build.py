@@ -29,6 +29,7 @@ SOURCES = [
     "gml_batch.hip",
     "gml_laplacian.cpp",
     "gml_serve.cpp",
+    "gml_renumber.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

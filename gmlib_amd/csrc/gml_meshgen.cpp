@@ -10,6 +10,7 @@
 // Proc. 707 (2004) -- axes-to-transpose transform, then bit interleave.
 
 #include "../../include/gmlx.h"
+#include "gml_hilbert.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -17,6 +18,8 @@
 #include <vector>
 
 namespace {
+
+using gml::HilbertKey;
 
 uint64_t SplitMix(uint64_t x)
 {
@@ -31,31 +34,6 @@ double Jit(uint64_t seed, uint64_t v, int axis)
 {
    uint64_t h = SplitMix(seed ^ SplitMix(v * 3 + (uint64_t)axis + 0x51ed270b27a8a4d5ull));
    return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
-
-uint64_t HilbertKey(unsigned x, unsigned y, unsigned z, int bits)
-{
-   unsigned X[3] = {x, y, z};
-   unsigned M = 1u << (bits - 1);
-   for(unsigned Q = M; Q > 1; Q >>= 1)
-   {
-      unsigned P = Q - 1;
-      for(int i = 0; i < 3; i++)
-      {
-         if(X[i] & Q) X[0] ^= P;
-         else { unsigned t = (X[0] ^ X[i]) & P; X[0] ^= t; X[i] ^= t; }
-      }
-   }
-   for(int i = 1; i < 3; i++) X[i] ^= X[i - 1];
-   unsigned t = 0;
-   for(unsigned Q = M; Q > 1; Q >>= 1)
-      if(X[2] & Q) t ^= Q - 1;
-   for(int i = 0; i < 3; i++) X[i] ^= t;
-   uint64_t h = 0;
-   for(int b = bits - 1; b >= 0; b--)
-      for(int i = 0; i < 3; i++)
-         h = (h << 1) | ((X[i] >> b) & 1u);
-   return h;
 }
 
 template<class F> void ParFor(long long n, F f)

@@ -155,6 +155,16 @@ double GmlxGetBuiltinRunTime(size_t GmlIdx, int which, long long *launches);
 int    GmlxCubeMesh(int n, double jitter, unsigned long long seed, int order,
                     float *crd, int *ref, int *tet);
 
+/* Hilbert renumbering on the GPU (host arrays in, permuted in place):
+   vertices are sorted along a 3D Hilbert curve of their bounding box (21
+   bits per axis), elements (nnod vertex indices each, 0-based) along the
+   same curve by their barycentre, and the element tables renumbered.  crd
+   holds 4 floats per vertex; VerRef, EleRef, VerOld, EleOld may be NULL;
+   VerOld / EleOld receive, for each new index, the old one.  The sort is
+   stable (deterministic).  Returns 1, or 0 on invalid input or GPU failure. */
+int    GmlxHilbertRenumber(int NmbVer, float *crd, int *VerRef, int NmbEle, int nnod, int *ele, int *EleRef,
+                           int *VerOld, int *EleOld);
+
 /* Edge-graph 3x3 vector Laplacian of a tet mesh (host, no handle): block
    CSR with one block row per vertex, columns ascending, off-diagonal block
    (i, j) = -w (I + 0.25 s s^T) with w in [0.5, 2] and s a unit vector drawn
