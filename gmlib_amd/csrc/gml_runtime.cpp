@@ -662,10 +662,10 @@ int GmlGetDataBlock(size_t GmlIdx, int TypIdx, int BegIdx, int EndIdx,
       long long DatLen = ((float *)DatEnd - (float *)DatBeg) / n;
       long long RefLen = RefBeg ? (RefEnd - RefBeg) / n : 0;
       const float *c = (const float *)d->host;
-      int nc = DatLen >= 4 ? 4 : 3;
       for(long long i = BegIdx; i <= EndIdx; i++)
       {
-         for(int j = 0; j < nc; j++)
+         // x, y, z only: a wider user stride keeps its other slots (reference Q21 wrote 4)
+         for(int j = 0; j < 3; j++)
             uc[(i - BegIdx) * DatLen + j] = c[i * 4 + j];
          if(RefBeg) RefBeg[(i - BegIdx) * RefLen] = rt[i];
       }
