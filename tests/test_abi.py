@@ -59,13 +59,13 @@ def test_line_and_block_roundtrip_host_only():
         n, idx = C.c_int(), C.c_int()
         assert L.GmlGetMeshInfo(h, G.GmlTetrahedra, C.byref(n), C.byref(idx)) and (n.value, idx.value) == (2, ti)
         assert not L.GmlGetMeshInfo(h, G.GmlHexahedra, C.byref(n), C.byref(idx))
-        # block get: 4 floats per vertex line (w = 0)
-        crd = np.empty((3, 4), np.float32)
+        # block get with a 4-float user stride: x, y, z written, the fourth slot left as it was
+        crd = np.full((3, 4), 9.0, np.float32)
         ref = np.empty(3, np.int32)
         assert L.GmlGetDataBlock(h, G.GmlVertices, 0, 2, G.ptr(crd), C.c_void_p(crd.ctypes.data + 32),
                                  G.ptr(ref), C.c_void_p(ref.ctypes.data + 8))
         assert np.array_equal(crd[:, 0], [.5, 1.5, 2.5]) and np.array_equal(ref, [0, 7, 14])
-        assert np.all(crd[:, 3] == 0)
+        assert np.all(crd[:, 3] == 9.0)
 
 
 def test_block_set_with_stride_and_shared_ref():
