@@ -117,6 +117,8 @@ _SIGS = {
     "GmlxServeStart": (_S, [_I, _D, _I, _I]),
     "GmlxServeSubmit": (C.c_longlong, [_S, _I, _I, C.c_ulonglong, _D]),
     "GmlxServeWait": (_I, [_S, C.c_longlong, _DP, _DP]),
+    "GmlxServeAddMesh": (C.c_longlong, [_S, _I, _I, _P, _P, _P]),
+    "GmlxServeSubmitMesh": (C.c_longlong, [_S, _I, C.c_longlong, _D]),
     "GmlxServeInfo": (_I, [_S, _DP]),
     "GmlxServeStop": (None, [_S]),
     "GmlxSetStaging": (_I, [_I]),
@@ -324,7 +326,7 @@ class Gml:
         return out
 
     def vertices(self, nv):
-        crd = np.empty((nv, 4), np.float32)
+        crd = np.zeros((nv, 4), np.float32)   # GmlGetDataBlock writes x, y, z; w stays 0
         ref = np.empty(nv, np.int32)
         ok = self.L.GmlGetDataBlock(self.h, GmlVertices, 0, nv - 1, ptr(crd),
                                     C.c_void_p(crd.ctypes.data + (nv - 1) * 16),

@@ -19,6 +19,10 @@
 // coordinates first (and so does any job after one), so every result is
 // independent of what else ran: the same alone and in the mix.
 //
+// Meshes are either seeded cube meshes (cells per side, seed) or meshes the
+// caller registered with GmlxServeAddMesh (the server keeps their host copy
+// to load them again after an eviction).
+//
 // Resident = 0 gives the plain per-job path for comparison: every job loads
 // its mesh into a fresh handle, runs, and frees it, one job at a time.
 
@@ -30,10 +34,19 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <memory>
 #include <thread>
 
 namespace gml {
 
+// a mesh registered by the caller
+struct UserMesh {
+   int nver = 0, ntet = 0;
+   std::vector<float> crd;
+   std::vector<int> ref, tet;
+};
+
+// n > 0: cube mesh (n, seed); n == 0: user mesh number seed
 struct Resident {
    int n = 0;
    unsigned long long seed = 0;
@@ -67,6 +80,8 @@ struct Server {
    std::deque<ServeJob *> pending;
    std::map<long long, ServeJob *> jobs;
    std::vector<Resident *> cache;
+   std::map<unsigned long long, std::shared_ptr<UserMesh>> meshes;
+   unsigned long long next_mesh = 1;
    std::vector<std::thread> workers;
    bool stop = false;
    long long next_id = 1, clock = 0, loads = 0, evictions = 0, ndone = 0;
@@ -75,10 +90,20 @@ struct Server {
    double Now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
-// device bytes a job's mesh may need (upper estimate, before loading)
-static size_t Estimate(int n, int type)
+// vertex and tet counts of a job's mesh (lock held)
+static void MeshSize(const Server *s, int n, unsigned long long seed, double *V, double *T)
 {
-   const double V = std::pow(n + 1.0, 3), T = 6.0 * std::pow((double)n, 3);
+   if(n > 0) { *V = std::pow(n + 1.0, 3); *T = 6.0 * std::pow((double)n, 3); return; }
+   auto it = s->meshes.find(seed);
+   *V = it == s->meshes.end() ? 0. : it->second->nver;
+   *T = it == s->meshes.end() ? 0. : it->second->ntet;
+}
+
+// device bytes a job's mesh may need (upper estimate, before loading)
+static size_t Estimate(const Server *s, int n, unsigned long long seed, int type)
+{
+   double V, T;
+   MeshSize(s, n, seed, &V, &T);
    double b = 44 * T + 48 * V + (1 << 20);
    if(type == GmlxJobSmooth) b += 30 * T + 16 * V;
    if(type == GmlxJobCG) b += 900 * V;
@@ -94,15 +119,28 @@ static void Unload(Resident *r)
 }
 
 // load the mesh of r into a fresh handle; false on failure
-static bool Load(Server *s, Resident *r)
+static bool Load(Server *s, Resident *r, std::shared_ptr<UserMesh> um)
 {
    const int n = r->n;
-   const long long V = (long long)(n + 1) * (n + 1) * (n + 1), T = 6LL * n * n * n;
-   if(V >= (1LL << 31) || T >= (1LL << 31)) return false;
-   std::vector<float> crd((size_t)V * 4);
-   std::vector<int> ref((size_t)V), tref((size_t)T, 0);
-   r->tet.resize((size_t)T * 4);
-   if(!GmlxCubeMesh(n, 0.2, r->seed, 1, crd.data(), ref.data(), r->tet.data())) return false;
+   const long long V = n > 0 ? (long long)(n + 1) * (n + 1) * (n + 1) : (um ? um->nver : 0);
+   const long long T = n > 0 ? 6LL * n * n * n : (um ? um->ntet : 0);
+   if(V < 1 || V >= (1LL << 31) || T >= (1LL << 31)) return false;
+   std::vector<float> crd;
+   std::vector<int> ref, tref((size_t)T, 0);
+   if(n > 0)
+   {
+      crd.resize((size_t)V * 4);
+      ref.resize((size_t)V);
+      r->tet.resize((size_t)T * 4);
+      if(!GmlxCubeMesh(n, 0.2, r->seed, 1, crd.data(), ref.data(), r->tet.data())) return false;
+   }
+   else
+   {
+      crd = um->crd;
+      ref = um->ref;
+      r->tet = um->tet;
+   }
+   if(T == 0) return false;
    r->nver = (int)V;
    r->ntet = (int)T;
    r->h = GmlInit(s->dev);
@@ -246,7 +284,7 @@ static ServeJob *Admit(Server *s, Resident **out)
    for(auto it = s->pending.begin(); it != s->pending.end();)
    {
       ServeJob *j = *it;
-      const size_t est = Estimate(j->n, j->type);
+      const size_t est = Estimate(s, j->n, j->seed, j->type);
       if(est > s->budget)
       {
          it = s->pending.erase(it);
@@ -297,10 +335,16 @@ static void Worker(Server *s)
       s->running++;
       s->max_running = std::max(s->max_running, s->running);
       j->t_admit = s->Now();
+      std::shared_ptr<UserMesh> um;
+      if(r->n == 0)
+      {
+         auto it = s->meshes.find(r->seed);
+         if(it != s->meshes.end()) um = it->second;
+      }
       lk.unlock();
 
       int rc = 1;
-      if(!r->h && !Load(s, r)) rc = -5;
+      if(!r->h && !Load(s, r, um)) rc = -5;
       if(rc == 1) rc = RunJob(r, j);
 
       lk.lock();
@@ -346,10 +390,8 @@ size_t GmlxServeStart(int dev, double BudgetGB, int NmbSlot, int Resident)
    return (size_t)s;
 }
 
-long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long seed, double param)
+static long long Enqueue(Server *s, int type, int n, unsigned long long seed, double param)
 {
-   Server *s = reinterpret_cast<Server *>(SrvIdx);
-   if(!s || type < GmlxJobQuality || type > GmlxJobCG || n < 1 || n > 1000) return -1;
    ServeJob *j = new ServeJob();
    j->type = type;
    j->n = n;
@@ -362,6 +404,44 @@ long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long see
    s->pending.push_back(j);
    s->cv_work.notify_one();
    return j->id;
+}
+
+long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long seed, double param)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || type < GmlxJobQuality || type > GmlxJobCG || n < 1 || n > 1000) return -1;
+   return Enqueue(s, type, n, seed, param);
+}
+
+long long GmlxServeAddMesh(size_t SrvIdx, int NmbVer, int NmbTet, const float *crd, const int *ref, const int *tet)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || NmbVer < 1 || NmbTet < 1 || !crd || !tet) return -1;
+   for(long long i = 0; i < (long long)NmbTet * 4; i++)
+      if(tet[i] < 0 || tet[i] >= NmbVer) return -1;
+   auto m = std::make_shared<UserMesh>();
+   m->nver = NmbVer;
+   m->ntet = NmbTet;
+   m->crd.assign(crd, crd + (size_t)NmbVer * 4);
+   for(long long v = 0; v < NmbVer; v++) m->crd[v * 4 + 3] = 0.f;
+   m->ref.assign((size_t)NmbVer, 0);
+   if(ref) m->ref.assign(ref, ref + NmbVer);
+   m->tet.assign(tet, tet + (size_t)NmbTet * 4);
+   std::lock_guard<std::mutex> lk(s->mtx);
+   const unsigned long long key = s->next_mesh++;
+   s->meshes[key] = m;
+   return (long long)key;
+}
+
+long long GmlxServeSubmitMesh(size_t SrvIdx, int type, long long MeshKey, double param)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || type < GmlxJobQuality || type > GmlxJobCG || MeshKey < 1) return -1;
+   {
+      std::lock_guard<std::mutex> lk(s->mtx);
+      if(!s->meshes.count((unsigned long long)MeshKey)) return -1;
+   }
+   return Enqueue(s, type, 0, (unsigned long long)MeshKey, param);
 }
 
 int GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim)

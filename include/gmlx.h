@@ -176,8 +176,8 @@ long long GmlxEdgeLaplacian3(int NmbVer, int NmbTet, const int *tet, unsigned lo
                              int *rowptr, int *col, float *val, long long cap);
 
 /* ---- serving: a queue of mesh jobs run against device-resident meshes.
-   A job names a seeded cube mesh (GmlxCubeMesh(n, 0.2, seed, Hilbert)) and a
-   kind of work; up to NmbSlot jobs run at once, each on its mesh's own
+   A job names a seeded cube mesh (GmlxCubeMesh(n, 0.2, seed, Hilbert)) or a
+   mesh registered with GmlxServeAddMesh, and a kind of work; up to NmbSlot jobs run at once, each on its mesh's own
    stream; meshes stay resident under an HBM budget of BudgetGB (least
    recently used idle meshes are evicted when a new one needs room); a job
    is admitted as soon as a slot is free and its mesh is resident and idle or
@@ -188,6 +188,13 @@ size_t    GmlxServeStart(int dev, double BudgetGB, int NmbSlot, int Resident);
 /* param: smooth = sweep count (20 if <= 0), cg = relative tolerance (1e-8
    if <= 0), unused for quality.  Returns a job id > 0, or -1. */
 long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long seed, double param);
+/* register a caller's tet mesh with the server (copied: crd 4 floats per
+   vertex, ref per vertex or NULL for 0, tet 4 0-based vertex indices per
+   tet); returns a mesh key > 0, or -1.  The copy stays with the server so
+   the mesh can be loaded again after an eviction. */
+long long GmlxServeAddMesh(size_t SrvIdx, int NmbVer, int NmbTet, const float *crd, const int *ref, const int *tet);
+/* a job on a registered mesh (same types and param as GmlxServeSubmit) */
+long long GmlxServeSubmitMesh(size_t SrvIdx, int type, long long MeshKey, double param);
 /* block until the job is done.  res[4]: quality and smooth = (min, mean
    quality), cg = (FP64 relative residual, iterations); tim[4] = (submitted,
    admitted, done) in seconds since GmlxServeStart, and 1 if the job loaded

@@ -89,3 +89,28 @@ def test_serve_over_budget_job_is_refused():
     out = run(srv, [(G.GmlxJobQuality, 30, 1, 0.0), (G.GmlxJobQuality, 2, 1, 0.0)])
     L.GmlxServeStop(srv)
     assert out[0][0] == -7 and out[1][0] == 1
+
+
+def test_serve_registered_mesh_matches_cube_job():
+    """A registered copy of cube mesh (5, 77) serves the same quality and
+    smoothing results as the cube job that names it; CG solves converge."""
+    L = G.lib()
+    crd, ref, tet = G.cube_mesh(5, 0.2, 77)
+    srv = L.GmlxServeStart(0, 0.05, 2, 1)
+    key = L.GmlxServeAddMesh(srv, len(crd), len(tet), G.ptr(crd), G.ptr(ref), G.ptr(tet))
+    assert key > 0
+    bad = tet.copy()
+    bad[0, 0] = len(crd)
+    assert L.GmlxServeAddMesh(srv, len(crd), len(tet), G.ptr(crd), G.ptr(ref), G.ptr(bad)) == -1
+    assert L.GmlxServeSubmitMesh(srv, G.GmlxJobQuality, key + 100, 0.0) == -1
+    ids = [L.GmlxServeSubmitMesh(srv, G.GmlxJobQuality, key, 0.0), L.GmlxServeSubmitMesh(srv, G.GmlxJobSmooth, key, 10.0),
+           L.GmlxServeSubmitMesh(srv, G.GmlxJobCG, key, 1e-8),
+           L.GmlxServeSubmit(srv, G.GmlxJobQuality, 5, 77, 0.0), L.GmlxServeSubmit(srv, G.GmlxJobSmooth, 5, 77, 10.0)]
+    out = []
+    for i in ids:
+        res, tim = (C.c_double * 4)(), (C.c_double * 4)()
+        assert L.GmlxServeWait(srv, i, res, tim) == 1
+        out.append(tuple(res))
+    L.GmlxServeStop(srv)
+    assert out[0][:2] == out[3][:2] and out[1][:2] == out[4][:2]
+    assert out[2][0] <= 1e-8 and out[2][1] > 0
