@@ -28,19 +28,48 @@ namespace gml {
 // ---------------------------------------------------------------------------
 // GPU vertex-ball builder
 // ---------------------------------------------------------------------------
-// degree of every vertex; rank[e * nnod + j] = the slot the atomic gave
-// node j of element e in its vertex's row (any order: rows are sorted later)
-__global__ void k_ball_count(const int *__restrict__ ele, int stride, int nnod, long long nele,
-                             long long nver, int *__restrict__ deg, int *__restrict__ rank, int *__restrict__ bad)
+// degree of every vertex, and for each node j of element e a distinct slot
+// rank[e * nnod + j] in its vertex's row (in no particular order: rows are
+// sorted afterwards).  The block's element nodes are first aggregated in an
+// LDS hash table (local counts and local ranks by LDS atomics), then one
+// global atomic per distinct vertex of the block reserves its slots: with
+// locality-ordered meshes a block of 256 tets touches ~10x fewer vertices
+// than it has nodes.
+static const int kCntBlock = 256;
+static const int kCntHash = 4096;   // >= 2 x kCntBlock x 8 nodes
+
+__global__ void __launch_bounds__(kCntBlock) k_ball_count(const int *__restrict__ ele, int stride, int nnod,
+                                                          long long nele, long long nver, int *__restrict__ deg,
+                                                          int *__restrict__ rank, int *__restrict__ bad)
 {
-   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-   if(e >= nele) return;
+   __shared__ int hkey[kCntHash], hcnt[kCntHash];   // hcnt holds the base once reserved
+   for(int i = threadIdx.x; i < kCntHash; i += kCntBlock) { hkey[i] = -1; hcnt[i] = 0; }
+   __syncthreads();
+   const long long e = (long long)blockIdx.x * kCntBlock + threadIdx.x;
+   int slot[8], loc[8];
    for(int j = 0; j < nnod; j++)
    {
-      int v = ele[e * stride + j];
-      if(v < 0 || v >= nver) { atomicOr(bad, 1); rank[e * nnod + j] = -1; continue; }
-      rank[e * nnod + j] = atomicAdd(&deg[v], 1);
+      slot[j] = -1;
+      if(e >= nele) continue;
+      const int v = ele[e * stride + j];
+      if(v < 0 || v >= nver) { atomicOr(bad, 1); continue; }
+      unsigned int h = ((unsigned int)v * 2654435761u) & (kCntHash - 1);
+      for(;;)
+      {
+         const int k = atomicCAS(&hkey[h], -1, v);
+         if(k == -1 || k == v) break;
+         h = (h + 1) & (kCntHash - 1);
+      }
+      slot[j] = (int)h;
+      loc[j] = atomicAdd(&hcnt[h], 1);
    }
+   __syncthreads();
+   for(int i = threadIdx.x; i < kCntHash; i += kCntBlock)
+      if(hkey[i] >= 0) hcnt[i] = atomicAdd(&deg[hkey[i]], hcnt[i]);
+   __syncthreads();
+   if(e >= nele) return;
+   for(int j = 0; j < nnod; j++)
+      rank[e * nnod + j] = slot[j] >= 0 ? hcnt[slot[j]] + loc[j] : -1;
 }
 
 // width of each 64-row chunk = its largest degree, and the first row wider
@@ -164,6 +193,8 @@ int gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long lo
    int hs[3] = {0, 0, 0};
    const int B = 256;
    unsigned int nbe = (unsigned int)((nele + B - 1) / B), nbv = (unsigned int)((nver + 63 + B - 1) / B);
+   static_assert(kCntBlock == B, "k_ball_count takes one element per thread of a 256-thread block");
+   if(nnod < 1 || nnod > 8) return 0;
    *off_out = nullptr; *ent_out = nullptr;
 
    if(hipMalloc(&w, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
