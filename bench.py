@@ -43,7 +43,16 @@ saturated throughput (all jobs submitted at once) and the plain per-job path
 (every job loads its mesh into a fresh handle, one at a time), whose results
 must equal the served ones.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve]
+Fifth workload (--workload extract): GmlExtractEdges + GmlExtractFaces on the
+256^3-cell Kuhn cube (100,663,296 tets) with the tables resident on the
+device: every distinct edge and triangle once, in the reference's order
+(first appearance), as new mesh tables.  Counted as the element table read
+once per call (16 B per tet, twice) plus the tables written (edges 8 B nodes
++ 4 B ref, triangles 16 B + 4 B).  The CPU baseline is the oracle's
+restatement of the reference's hash (one thread, as the reference) on a
+smaller cube, in the same bytes per second.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -69,7 +78,7 @@ def body(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve"])
+    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract"])
     ap.add_argument("--qps", type=float, default=40.0, help="serve workload: arrival rate (jobs/s)")
     ap.add_argument("--jobs", type=int, default=400, help="serve workload: jobs in the timed stream")
     ap.add_argument("--pool", type=int, default=24, help="serve workload: meshes in the pool")
@@ -104,6 +113,8 @@ def main():
         run_batch(args)
     elif args.workload == "serve":
         run_serve(args)
+    elif args.workload == "extract":
+        run_extract(args)
     else:
         run_smoothing(args)
 
@@ -780,6 +791,108 @@ def run_batch(args):
         log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def extract_bytes(T, E, F):
+    """Algorithmic bytes of one edge + face extraction: the tet table read by
+    each call, the edge table (2 nodes + ref) and triangle table (int4 row
+    + ref) written."""
+    return 32 * T + 12 * E + 20 * F
+
+
+def run_extract(args):
+    import numpy as np
+    import gmlib_amd as G
+
+    n = args.n
+    crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
+    V, T = len(crd), len(tet)
+    g = G.Gml(0)
+    L, h = g.L, g.h
+    g.set_mesh(crd, ref, tet)
+    L.GmlxSync(h)
+
+    def drop(typ):
+        idx = C.c_int(0)
+        if L.GmlGetMeshInfo(h, typ, None, C.byref(idx)):
+            L.GmlFreeData(h, idx.value)
+
+    def step():
+        drop(G.GmlEdges)
+        drop(G.GmlTriangles)
+        L.GmlxSync(h)
+        t0 = time.perf_counter()
+        ne = L.GmlExtractEdges(h)
+        t1 = time.perf_counter()
+        nf = L.GmlExtractFaces(h)
+        t2 = time.perf_counter()
+        if ne <= 0 or nf <= 0:
+            raise RuntimeError("extraction failed (%d edges, %d faces)" % (ne, nf))
+        return ne, nf, t1 - t0, t2 - t1
+
+    for _ in range(args.warmup):
+        step()
+    te = tf = 0.0
+    for _ in range(args.steps):
+        ne, nf, a, b = step()
+        te += a
+        tf += b
+    E = 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+    F = (4 * T + 12 * n * n) // 2
+    if ne != E or nf != F:
+        raise RuntimeError("extraction counts %d, %d differ from the cube's %d edges, %d faces" % (ne, nf, E, F))
+    dt = (te + tf) / args.steps
+    alg = extract_bytes(T, E, F)
+    gbs = alg / dt / 1e9
+    log("extract: %d tets -> %d edges in %.2f ms, %d triangles in %.2f ms; %.1f GB/s algorithmic, %.2f Gtets/s"
+        % (T, E, te / args.steps * 1e3, F, tf / args.steps * 1e3, gbs, T / dt / 1e9))
+    out = {
+        "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded jittered Kuhn cube mesh)",
+        "config": {"workload": "GmlExtractEdges + GmlExtractFaces, %d^3-cell Kuhn cube (%d vertices, %d tets -> "
+                               "%d edges, %d triangles)" % (n, V, T, E, F),
+                   "vertices": V, "tets": T, "edges": E, "triangles": F, "bytes_per_step": alg,
+                   "edges_ms": round(te / args.steps * 1e3, 3), "faces_ms": round(tf / args.steps * 1e3, 3)},
+        "tets_per_s": round(T / dt, 1),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole extraction (key, radix sort, select, write kernels)",
+                     "bytes_per_launch": alg, "avg_us": round(dt * 1e6, 1)},
+        "cpu_baseline": None,
+    }
+    g.close()
+    if args.cpu_seconds > 0:
+        from tests import oracle_lib as O
+        m = 8
+        best = None
+        while True:
+            _, _, ct = G.cube_mesh(m, 0.2, 2024, True)
+            t0 = time.perf_counter()
+            e, _ = O.extract_items(2, [(G.GmlTetrahedra, ct)], tabsiz=len(ct))
+            f, _ = O.extract_items(3, [(G.GmlTetrahedra, ct)])
+            c = time.perf_counter() - t0
+            best = (m, len(ct), len(e), len(f), c)
+            if c > args.cpu_seconds / 4 or m >= n:
+                break
+            m = int(m * 1.6)
+        m, ct_n, ce, cf, c = best
+        out["cpu_baseline"] = {"value": round(extract_bytes(ct_n, ce, cf) / c / 1e9, 3), "unit": "GB/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "%d^3-cell cube (%d tets -> %d edges, %d triangles): the oracle's restatement "
+                                         "of the reference hash (GetItmNod / CalHshKey / AddHsh / GetHsh), one "
+                                         "thread, %.2f s" % (m, ct_n, ce, cf, c)}
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -10,7 +10,9 @@
 // extracted item are stored sorted ascending (its hash key order), and the
 // return value is the number of items (triangles + quads for faces).
 // Links built on a table that is replaced would go stale, so extraction is
-// refused while such a link exists.
+// refused while such a link exists.  Device handles extract on the GPU
+// (gml_extract_gpu.hip: radix sorts of packed node keys); host-only handles
+// use the ordered-map form below.
 
 #include "gml_internal.h"
 
@@ -107,6 +109,13 @@ int GmlExtractEdges(size_t GmlIdx)
       fprintf(stderr, "gml: GmlExtractEdges: links built on the edge table exist; extract before building links\n");
       return 0;
    }
+   if(!g->host_only)
+   {
+      const int n = GpuExtractItems(GmlIdx, g, GmlEdges, 2);
+      if(n < 0) return 0;
+      DbgPrint(g, "Hashed, setup and transfered %d edges to the GMlib.\n", n);
+      return n;
+   }
    Extracted x;
    const int *nod, *ref;
    int len;
@@ -138,6 +147,14 @@ int GmlExtractFaces(size_t GmlIdx)
    {
       fprintf(stderr, "gml: GmlExtractFaces: links built on the face tables exist; extract before building links\n");
       return 0;
+   }
+   if(!g->host_only)
+   {
+      const int nt = GpuExtractItems(GmlIdx, g, GmlTriangles, 3);
+      const int nq = nt < 0 ? -1 : GpuExtractItems(GmlIdx, g, GmlQuadrilaterals, 4);
+      if(nq < 0) return 0;
+      DbgPrint(g, "Hashed, setup and transfered %d triangles and %d quads to the GMlib.\n", nt, nq);
+      return nt + nq;
    }
    Extracted tri, qad;
    const int *nod, *ref;

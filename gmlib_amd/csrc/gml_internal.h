@@ -188,6 +188,10 @@ int  gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long l
                      long long nver, int *deg, unsigned int **off_out, int **ent_out,
                      long long *nslab_out, int *maxdeg_out, int *firsthigh_out, int baswid);
 
+// edge / face extraction on the device (gml_extract_gpu.hip): distinct items
+// of degree deg into the table of kind typ; count, or -1 on failure
+int  GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg);
+
 // reductions (gml_reduce.hip)
 int  ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed);
 size_t ReduceScratchBytes();

@@ -463,13 +463,16 @@ int GmlFreeData(size_t GmlIdx, int idx)
    Data *d = &g->dat[idx];
    if(d->kind == DkEle)
    {
-      // Dropping a mesh table also drops its ref table and the links built on it
+      // Dropping the current table of a mesh kind also drops its ref table
+      // (which the caller has no index for); links built on it are kept
       int t = d->MshTyp;
       if(g->TypIdx[t] == idx)
       {
          g->TypIdx[t] = 0;
          g->NmbEle[t] = 0;
          g->LnkMat[t][GmlVertices] = 0;
+         if(g->RefIdx[t] && g->RefIdx[t] != idx) FreeDataSlot(g, g->RefIdx[t]);
+         g->RefIdx[t] = 0;
       }
    }
    if(d->RedIdx) FreeDataSlot(g, d->RedIdx);

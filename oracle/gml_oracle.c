@@ -607,6 +607,108 @@ void orc_bsr_spmv_f32(int nrow, int b, const int *rowptr, const int *col,
    }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Topology: edge and face extraction                                        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Distinct items (edges: deg 2, triangles: 3, quads: 4) of a list of
+ * element sources, restating the hash of GmlExtractFaces
+ * (gmlib3.c:3880-4171) with its helpers: each item's nodes sorted ascending
+ * (GetItmNod, gmlib3.c:1567-1620), hash key sum(w_i * node_i) mod TabSiz
+ * with w = {3, 5, 7, 11} and TabSiz = the number of items hashed
+ * (CalHshKey, gmlib3.c:1627-1638), buckets chained by head insertion
+ * (AddHsh, gmlib3.c:1645-1674) and searched by node comparison along the
+ * chain (GetHsh, gmlib3.c:1681-1721), items visited source by source,
+ * element by element, local item by local item.
+ *
+ * The reference writes the output in a second pass: an existing item i is
+ * written when its first hashed appearance is i itself (gmlib3.c:4020-4035),
+ * an element item when its first appearance is (this kind, this element)
+ * (gmlib3.c:4089-4100).  With one appearance of an item per element, that is
+ * the list of first appearances in visiting order, which is what this
+ * function emits directly when an item is first added.  Source 0 may be the
+ * existing table (has_old): its items keep the ref of their first
+ * occurrence (oldref), all other items get ref 0.
+ *
+ * Sources: ne[s] elements of stride[s] ints at ele[s]; nloc[s] local items,
+ * local node indices loc[(s * 12 + j) * 4 + k].  tabsiz: the number of hash
+ * heads, 0 for the number of items hashed (the face tables' size); the edge
+ * table of the reference has one head per element (gmlib3.c:3730-3736).
+ * out_nodes (deg ints per item) and out_ref receive at most cap items;
+ * returns the item count.
+ *
+ * Edges: the reference's GmlExtractEdges (gmlib3.c:3719-3873) does not hash
+ * the existing edge table, which makes its handling of existing edges write
+ * past the new table; the face rule above (existing items hashed first) is
+ * applied to edges too.
+ */
+long long orc_extract_items(int deg, int nsrc, const long long *ne, const int *stride, const int *nloc,
+                            const int *loc, const int *const *ele, int has_old, const int *oldref,
+                            long long tabsiz, int *out_nodes, int *out_ref, long long cap)
+{
+   static const int wei[4] = {3, 5, 7, 11};
+   typedef struct { int nod[4], nxt, ref; } buc_t;
+   long long tab = 0, nbuc = 1, nout = 0, s, e;
+   int *head;
+   buc_t *buc;
+
+   for(s = 0; s < nsrc; s++)
+      tab += ne[s] * nloc[s];
+   if(!tab)
+      return 0;
+   if(tabsiz > 0)
+      tab = tabsiz;
+   head = calloc((size_t)tab, sizeof(int));
+   for(s = 0, nbuc = 0; s < nsrc; s++)
+      nbuc += ne[s] * nloc[s];
+   buc = malloc((size_t)(nbuc + 1) * sizeof(buc_t));
+   nbuc = 1;
+   if(!head || !buc)
+   {
+      free(head);
+      free(buc);
+      return -1;
+   }
+
+   for(s = 0; s < nsrc; s++)
+      for(e = 0; e < ne[s]; e++)
+      {
+         const int *en = ele[s] + e * stride[s];
+         int j;
+         for(j = 0; j < nloc[s]; j++)
+         {
+            int n[4] = {0, 0, 0, 0}, i, k, key = 0, b, hit = 0;
+            for(i = 0; i < deg; i++)
+               n[i] = en[ loc[(s * 12 + j) * 4 + i] ];
+            for(i = 0; i < deg; i++)
+               for(k = i + 1; k < deg; k++)
+                  if(n[i] > n[k]) { int t = n[i]; n[i] = n[k]; n[k] = t; }
+            for(i = 0; i < deg; i++)
+               key += wei[i] * n[i];
+            key = (int)(key % tab);
+            for(b = head[key]; b && !hit; b = buc[b].nxt)
+               hit = !memcmp(buc[b].nod, n, sizeof(n));
+            if(hit)
+               continue;
+            memcpy(buc[nbuc].nod, n, sizeof(n));
+            buc[nbuc].nxt = head[key];
+            buc[nbuc].ref = (has_old && s == 0 && oldref) ? oldref[e] : 0;
+            head[key] = (int)nbuc;
+            if(nout < cap)
+            {
+               memcpy(&out_nodes[nout * deg], n, (size_t)deg * sizeof(int));
+               out_ref[nout] = buc[nbuc].ref;
+            }
+            nbuc++;
+            nout++;
+         }
+      }
+   free(head);
+   free(buc);
+   return nout;
+}
+
 /* Thread count the OpenMP regions above use (for the CPU baseline record). */
 int orc_num_threads(void)
 {

@@ -49,6 +49,8 @@ def lib():
             "orc_ref_nodesmooth_sweep_f64": (None, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
             "orc_bsr_spmv_f64": (None, [_I, _I, _P, _P, _P, _P, _P]),
             "orc_bsr_spmv_f32": (None, [_I, _I, _P, _P, _P, _P, _P]),
+            "orc_extract_items": (C.c_longlong, [_I, _I, _P, _P, _P, _P, _P, _I, _P, C.c_longlong, _P, _P,
+                                                  C.c_longlong]),
             "orc_num_threads": (_I, []),
         }
         for n, (r, a) in sigs.items():
@@ -138,3 +140,57 @@ def bsr_spmv64(rowptr, col, val, x, b):
     y = np.empty(len(rowptr) - 1, dtype=np.float64).repeat(b)
     lib().orc_bsr_spmv_f64(len(rowptr) - 1, b, p(rowptr), p(col), p(val), p(np.ascontiguousarray(x, np.float64)), p(y))
     return y
+
+
+# Local items of each element kind (reference tables ItmEdgNod and
+# ItmFacNod, gmlib3.c:330-358), keyed by GmlTriangles..GmlHexahedra = 2..7.
+EDGE_LOC = {
+    2: [(0, 1), (1, 2), (2, 0)],
+    3: [(0, 1), (1, 2), (2, 3), (3, 0)],
+    4: [(0, 1), (1, 2), (2, 0), (3, 0), (3, 1), (3, 2)],
+    5: [(0, 1), (3, 2), (0, 3), (1, 2), (0, 4), (1, 4), (2, 4), (3, 4)],
+    6: [(0, 1), (1, 2), (2, 0), (3, 4), (4, 5), (5, 3), (0, 3), (1, 4), (2, 5)],
+    7: [(0, 1), (3, 2), (7, 6), (4, 5), (0, 3), (4, 7), (5, 6), (1, 2), (0, 4), (1, 5), (2, 6), (3, 7)],
+}
+FACE_LOC = {
+    2: [(0, 1, 2)],
+    3: [(0, 1, 2, 3)],
+    4: [(1, 2, 3), (2, 0, 3), (3, 0, 1), (0, 2, 1)],
+    5: [(0, 1, 4), (1, 2, 4), (2, 3, 4), (3, 0, 4), (3, 2, 1, 0)],
+    6: [(0, 2, 1), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4), (3, 5, 2, 0)],
+    7: [(0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2), (0, 3, 2, 1), (4, 5, 6, 7)],
+}
+
+
+def extract_items(deg, tables, old=None, oldref=None, tabsiz=0):
+    """Distinct items of degree deg (2 edges, 3 triangles, 4 quads) of the
+    element tables [(kind, int32 [n, nnod])...], visited in order after the
+    existing table old ([n, deg], refs oldref): (nodes [m, deg], refs [m])."""
+    srcs = []
+    if old is not None:
+        srcs.append((np.ascontiguousarray(old, np.int32), [tuple(range(deg))]))
+    for kind, t in tables:
+        loc = EDGE_LOC[kind] if deg == 2 else [f for f in FACE_LOC[kind] if len(f) == deg]
+        if loc and len(t):
+            srcs.append((np.ascontiguousarray(t, np.int32), loc))
+    if not srcs:
+        return np.zeros((0, deg), np.int32), np.zeros(0, np.int32)
+    ns = len(srcs)
+    ne = np.array([len(t) for t, _ in srcs], np.int64)
+    stride = np.array([t.shape[1] for t, _ in srcs], np.int32)
+    nloc = np.array([len(l) for _, l in srcs], np.int32)
+    loc = np.zeros((ns, 12, 4), np.int32)
+    for s, (_, l) in enumerate(srcs):
+        for j, f in enumerate(l):
+            loc[s, j, :len(f)] = f
+    ptrs = (C.c_void_p * ns)(*[t.ctypes.data for t, _ in srcs])
+    cap = int((ne * nloc).sum())
+    nodes = np.zeros((cap, deg), np.int32)
+    refs = np.zeros(cap, np.int32)
+    oref = np.ascontiguousarray(oldref if oldref is not None else np.zeros(len(old) if old is not None else 0),
+                                np.int32)
+    m = lib().orc_extract_items(deg, ns, p(ne), p(stride), p(nloc), p(loc), C.cast(ptrs, C.c_void_p),
+                                1 if old is not None else 0, p(oref), int(tabsiz), p(nodes), p(refs), cap)
+    if m < 0:
+        raise MemoryError("orc_extract_items")
+    return nodes[:m].copy(), refs[:m].copy()

@@ -1,0 +1,178 @@
+"""GmlExtractEdges / GmlExtractFaces on a device handle (radix sorts of
+packed node keys on the GPU) against the same calls on a host-only handle
+(the ordered-map form, itself checked against a brute-force restatement of
+sources/gmlib3.c:3719-4171 in test_extract.py) and against that restatement
+directly: bit-identical tables, refs and counts.  Covers the one-word key
+(b bits per node) and the two-word key (32 bits per node, forced by a vertex
+count above 2^21 for triangles and 2^16 for quads), mixed element kinds
+(tets, pyramids, prisms, hexes), an existing table with refs and duplicates,
+and the count closed forms of a large Kuhn cube."""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+import gmlib_amd as G
+from tests import oracle_lib as O
+from tests.test_extract import TET_EDGES, TET_FACES, first_appearance, get_block, set_block
+
+pytestmark = pytest.mark.gpu
+
+
+def set_vertices(g, nv):
+    crd = np.zeros((nv, 4), np.float32)
+    crd[:, 0] = np.arange(nv, dtype=np.float32)
+    ref = np.zeros(nv, np.int32)
+    assert g.L.GmlNewMeshData(g.h, G.GmlVertices, nv)
+    assert g.L.GmlSetDataBlock(g.h, G.GmlVertices, 0, nv - 1, G.ptr(crd), C.c_void_p(crd.ctypes.data + (nv - 1) * 16),
+                               G.ptr(ref), C.c_void_p(ref.ctypes.data + (nv - 1) * 4))
+
+
+def hex_grid(n):
+    """Hexes of an n^3 grid, GMlib node order (bottom 0-3, top 4-7)."""
+    idx = lambda i, j, k: (k * (n + 1) + j) * (n + 1) + i
+    h = []
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                b = [idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k), idx(i, j + 1, k)]
+                t = [idx(i, j, k + 1), idx(i + 1, j, k + 1), idx(i + 1, j + 1, k + 1), idx(i, j + 1, k + 1)]
+                h.append(b + t)
+    return np.array(h, np.int32), (n + 1) ** 3
+
+
+def mixed_mesh(n, spread=1):
+    """(nv, {kind: table}): an n^3 hex grid, odd cells split into two prisms,
+    every third even cell into a pyramid and a tet pair; node ids scaled by
+    spread (unused vertices in between) to reach the two-word key form."""
+    hexes, nv = hex_grid(n)
+    out = {G.GmlHexahedra: [], G.GmlPrisms: [], G.GmlPyramids: [], G.GmlTetrahedra: []}
+    for c, h in enumerate(hexes):
+        if c % 2:
+            out[G.GmlPrisms] += [[h[0], h[1], h[2], h[4], h[5], h[6]], [h[0], h[2], h[3], h[4], h[6], h[7]]]
+        elif c % 3 == 0:
+            out[G.GmlPyramids].append([h[0], h[1], h[2], h[3], h[4]])
+            out[G.GmlTetrahedra] += [[h[1], h[2], h[4], h[5]], [h[2], h[3], h[4], h[7]]]
+        else:
+            out[G.GmlHexahedra].append(list(h))
+    tabs = {k: np.array(v, np.int32) * spread for k, v in out.items() if v}
+    return nv * spread, tabs
+
+
+def build_mixed(nv, tabs, extra=None):
+    def build(g):
+        set_vertices(g, nv)
+        for typ, t in tabs.items():
+            set_block(g, typ, t, np.zeros(len(t), np.int32))
+        if extra:
+            for typ, (t, r) in extra.items():
+                set_block(g, typ, t, r)
+    return build
+
+
+def run(host_only, build, what):
+    """Extraction on one handle: count and {kind: (nodes, refs)}."""
+    with G.Gml(host_only=host_only) as g:
+        build(g)
+        res = {}
+        if what == "edges":
+            n = g.L.GmlExtractEdges(g.h)
+            res[G.GmlEdges] = get_block(g, G.GmlEdges, n, 2)
+        else:
+            n = g.L.GmlExtractFaces(g.h)
+            for typ, w in ((G.GmlTriangles, 3), (G.GmlQuadrilaterals, 4)):
+                m = count_of(g, typ)
+                if m:
+                    res[typ] = get_block(g, typ, m, w)
+        return n, res
+
+
+def count_of(g, typ):
+    """Lines of a mesh table (0 if absent)."""
+    n = C.c_int(0)
+    g.L.GmlGetMeshInfo(g.h, typ, C.byref(n), None)
+    return n.value
+
+
+def same(a, b):
+    assert a[0] == b[0]
+    assert sorted(a[1]) == sorted(b[1])
+    for k in a[1]:
+        assert np.array_equal(a[1][k][0], b[1][k][0]), k
+        assert np.array_equal(a[1][k][1], b[1][k][1]), k
+
+
+@pytest.mark.parametrize("n", [1, 4])
+def test_gpu_extract_tets_against_restatement(n):
+    crd, ref, tet = G.cube_mesh(n, 0.1, 2)
+    old = np.array([[tet[0, 1], tet[0, 0]], [0, len(crd) - 1], [tet[5, 2], tet[5, 3]], [tet[0, 0], tet[0, 1]]], np.int32)
+    old_ref = np.array([5, 6, 7, 8], np.int32)
+    surf = np.array([sorted(tet[2, list(TET_FACES[1])])[::-1], [0, 1, len(crd) - 1]], np.int32)
+    surf_ref = np.array([11, 12], np.int32)
+
+    def build_e(g):
+        g.set_mesh(crd, ref, tet)
+        set_block(g, G.GmlEdges, old, old_ref)
+
+    def build_f(g):
+        g.set_mesh(crd, ref, tet)
+        set_block(g, G.GmlTriangles, surf, surf_ref)
+
+    ne, e = run(False, build_e, "edges")
+    want, wref = first_appearance(zip(old, old_ref), tet, TET_EDGES)
+    assert ne == len(want)
+    assert np.array_equal(e[G.GmlEdges][0], want) and np.array_equal(e[G.GmlEdges][1], wref)
+    nf, f = run(False, build_f, "faces")
+    want, wref = first_appearance(zip(surf, surf_ref), tet, TET_FACES)
+    assert nf == len(want) and sorted(f) == [G.GmlTriangles]
+    assert np.array_equal(f[G.GmlTriangles][0], want) and np.array_equal(f[G.GmlTriangles][1], wref)
+
+
+@pytest.mark.parametrize("spread", [1, 40000])
+def test_gpu_extract_mixed_kinds_same_as_host(spread):
+    # spread 40000: 2.56M vertices (22-bit ids) -- two-word keys for triangles and quads
+    nv, tabs = mixed_mesh(3, spread)
+    quads = np.array([tabs[G.GmlHexahedra][0, [4, 5, 6, 7]][::-1], [0, spread, 2 * spread, 3 * spread]], np.int32)
+    extra = {G.GmlQuadrilaterals: (quads, np.array([21, 22], np.int32))}
+    build = build_mixed(nv, tabs, extra)
+    srcs = sorted(tabs.items())
+    qref = np.array([21, 22], np.int32)
+    want = {G.GmlEdges: O.extract_items(2, [(G.GmlQuadrilaterals, quads)] + srcs),
+            G.GmlTriangles: O.extract_items(3, srcs),
+            G.GmlQuadrilaterals: O.extract_items(4, srcs, quads, qref)}
+    for what in ("edges", "faces"):
+        dev, host = run(False, build, what), run(True, build, what)
+        same(dev, host)
+        assert dev[0] > 0
+        for typ, (nodes, refs) in dev[1].items():
+            assert np.array_equal(nodes, want[typ][0]) and np.array_equal(refs, want[typ][1]), typ
+
+
+def test_gpu_extract_large_cube_counts():
+    n = 64
+    crd, ref, tet = G.cube_mesh(n, 0.1, 3)
+    T = len(tet)
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        t0 = time.perf_counter()
+        ne = g.L.GmlExtractEdges(g.h)
+        t1 = time.perf_counter()
+        nf = g.L.GmlExtractFaces(g.h)
+        t2 = time.perf_counter()
+        e, _ = get_block(g, G.GmlEdges, ne, 2)
+        f, _ = get_block(g, G.GmlTriangles, nf, 3)
+    print("extraction of %d tets: %d edges in %.1f ms, %d faces in %.1f ms"
+          % (T, ne, 1e3 * (t1 - t0), nf, 1e3 * (t2 - t1)))
+    assert ne == 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+    assert nf == (4 * T + 12 * n * n) // 2
+    # sorted nodes, all distinct, every tet's items present
+    assert (e[:, 0] < e[:, 1]).all() and (f[:, 0] < f[:, 1]).all() and (f[:, 1] < f[:, 2]).all()
+    ke = e[:, 0].astype(np.int64) * len(crd) + e[:, 1]
+    assert len(np.unique(ke)) == ne
+    te = np.sort(tet[:, [0, 1]], axis=1)
+    kt = te[:, 0].astype(np.int64) * len(crd) + te[:, 1]
+    assert np.isin(kt, ke).all()
+    # first appearance: the first tet's edges lead, in local order
+    want, _ = first_appearance([], tet[:2], TET_EDGES)
+    assert np.array_equal(e[:len(want)], want)
