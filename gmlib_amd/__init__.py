@@ -90,6 +90,7 @@ _SIGS = {
     "GmlxGetStream": (_P, [_S]),
     "GmlxInvalidateHost": (_I, [_S, _I]),
     "GmlxBuildLink": (_I, [_S, _I, _I]),
+    "GmlxGetLinkData": (_I, [_S, _I, _I]),
     "GmlxGetUplinkTable": (_I, [_S, _I, _I, _P, _P, _I]),
     "GmlxCubeMesh": (_I, [_I, _D, C.c_ulonglong, _I, _P, _P, _P]),
     "GmlxSetVector": (_I, [_S, _I, _P]),

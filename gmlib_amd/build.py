@@ -30,7 +30,7 @@ SOURCES = [
     "gml_laplacian.cpp",
     "gml_serve.cpp",
     "gml_renumber.hip",
-    "gml_extract_gpu.hip",
+    "gml_items.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -11,7 +11,7 @@
 // return value is the number of items (triangles + quads for faces).
 // Links built on a table that is replaced would go stale, so extraction is
 // refused while such a link exists.  Device handles extract on the GPU
-// (gml_extract_gpu.hip: radix sorts of packed node keys); host-only handles
+// (gml_items.hip: radix sorts of packed node keys); host-only handles
 // use the ordered-map form below.
 
 #include "gml_internal.h"

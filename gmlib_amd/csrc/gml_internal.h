@@ -188,9 +188,20 @@ int  gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long l
                      long long nver, int *deg, unsigned int **off_out, int **ent_out,
                      long long *nslab_out, int *maxdeg_out, int *firsthigh_out, int baswid);
 
-// edge / face extraction on the device (gml_extract_gpu.hip): distinct items
-// of degree deg into the table of kind typ; count, or -1 on failure
+// SELL-64 layout of rows of the given device degrees (gml_topology.hip)
+int  gpu_sell_layout(hipStream_t s, const int *deg, long long n, int baswid, unsigned int **off_out,
+                     int **ent_out, long long *nslab_out, int *maxdeg_out, int *firsthigh_out);
+
+// topology from element items on the device (gml_items.hip): extraction of
+// the distinct items of degree deg into the table of kind typ (count, -1 on
+// failure); downlink table Src -> Dst (data index, 0 on failure); shell rows
+// of Src entities over Dst elements; neighbours of kind typ across items of
+// kind ngb into link data idx of rw ints per line
 int  GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg);
+int  GpuDownlink(GmlCtx *g, int Src, int Dst, const std::string &nam);
+int  GpuShell(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int **off, int **ent, long long *nslab,
+              int *maxdeg, int *firsthigh);
+int  GpuNeighbours(GmlCtx *g, int typ, int ngb, int idx, int rw);
 
 // reductions (gml_reduce.hip)
 int  ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed);

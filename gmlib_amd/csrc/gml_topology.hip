@@ -15,7 +15,9 @@
 // coalesced load.  Vertex balls are built on the GPU (degree count, whose
 // atomic returns also give each (element, node) its slot; chunk widths,
 // offset scan, atomic-free fill, per-row descending sort in registers); edge
-// and face shells and downlinks are built on the host.
+// and face shells, downlinks and neighbours of a device handle are built on
+// the GPU from sorted item keys (gml_items.hip), those of a host-only
+// handle by the host forms below.
 
 #include "gml_internal.h"
 
@@ -245,6 +247,49 @@ fail:
    return 0;
 }
 
+// SELL-64 layout of rows with the given degrees (device array of n ints):
+// chunk offsets (*off_out, n/64 + 1 entries) and a zeroed entry array
+// (*ent_out, nslab * 64 + 64 ints), both new device allocations owned by the
+// caller; the maximum degree and the first row wider than baswid (-1: none).
+int gpu_sell_layout(hipStream_t s, const int *deg, long long n, int baswid, unsigned int **off_out,
+                    int **ent_out, long long *nslab_out, int *maxdeg_out, int *firsthigh_out)
+{
+   const long long nchunk = (n + 63) / 64;
+   const int B = 256;
+   unsigned int *w = nullptr, *off = nullptr;
+   int *stats = nullptr, *ent = nullptr, hs[2] = {0, 0};
+   unsigned int nslab = 0;
+   const int init[4] = {0, (int)0x7fffffff, 0, 0};
+   *off_out = nullptr;
+   *ent_out = nullptr;
+   if(hipMalloc(&w, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
+   || hipMalloc(&off, (nchunk + 1) * sizeof(unsigned int)) != hipSuccess
+   || hipMalloc(&stats, 4 * sizeof(int)) != hipSuccess
+   || hipMemsetAsync(w, 0, (nchunk + 1) * sizeof(unsigned int), s) != hipSuccess
+   || hipMemcpyAsync(stats, init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess)
+      goto fail;
+   hipLaunchKernelGGL(k_ball_chunks, dim3((unsigned int)((n + 63 + B - 1) / B)), dim3(B), 0, s, deg, n, w, stats, baswid);
+   hipLaunchKernelGGL(k_ball_scan, dim3(1), dim3(1024), 0, s, w, nchunk, off, stats);
+   if(hipMemcpyAsync(&nslab, off + nchunk, sizeof(unsigned int), hipMemcpyDeviceToHost, s) != hipSuccess
+   || hipMemcpyAsync(hs, stats, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess
+   || hipStreamSynchronize(s) != hipSuccess
+   || hipMalloc(&ent, ((size_t)nslab * 64 + 64) * sizeof(int)) != hipSuccess
+   || hipMemsetAsync(ent, 0, ((size_t)nslab * 64 + 64) * sizeof(int), s) != hipSuccess)
+      goto fail;
+   (void)hipFree(w);
+   (void)hipFree(stats);
+   *off_out = off;
+   *ent_out = ent;
+   *nslab_out = nslab;
+   *maxdeg_out = hs[0];
+   *firsthigh_out = hs[1] == 0x7fffffff ? -1 : hs[1];
+   return 1;
+fail:
+   (void)hipGetLastError();
+   (void)hipFree(w); (void)hipFree(off); (void)hipFree(stats); (void)hipFree(ent);
+   return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
@@ -379,8 +424,21 @@ int BuildUplink(GmlCtx *g, int Src, int Dst)
       }
       return RegisterUplink(g, Src, Dst, degdev, off, ent, nslab, maxdeg, first);
    }
+   if(!g->host_only)
+   {
+      // edge / face shells: item keys sorted on the GPU
+      unsigned int *off = nullptr;
+      int *ent = nullptr, maxdeg = 0, first = -1;
+      long long nslab = 0;
+      if(!GpuShell(g, Src, Dst, degdev, &off, &ent, &nslab, &maxdeg, &first))
+      {
+         DevFree(g, degdev);
+         return 0;
+      }
+      return RegisterUplink(g, Src, Dst, degdev, off, ent, nslab, maxdeg, first);
+   }
 
-   // host builder: rows in descending element order
+   // host builder (host-only handles): rows in descending element order
    std::vector<int> dtab, stab;
    int dst = 0, sst = 0;
    if(!HostTable(g, Dst, &dtab, &dst) || (Src != GmlVertices && !HostTable(g, Src, &stab, &sst)))
@@ -469,6 +527,19 @@ int BuildUplink(GmlCtx *g, int Src, int Dst)
 int BuildDownlink(GmlCtx *g, int Src, int Dst)
 {
    if(g->LnkMat[Src][Dst]) return 1;
+   if(!g->TypIdx[Src] || !g->TypIdx[Dst])
+   {
+      fprintf(stderr, "gml: downlink %s -> %s needs both mesh tables\n", BalTypStr[Src], BalTypStr[Dst]);
+      return 0;
+   }
+   std::string nam = std::string(BalTypStr[Src]) + BalTypStr[Dst] + "Lnk";
+   if(!g->host_only)
+   {
+      const int idx = GpuDownlink(g, Src, Dst, nam);
+      if(!idx) return 0;
+      g->LnkMat[Src][Dst] = idx;
+      return 1;
+   }
    std::vector<int> stab, dtab;
    int sst = 0, dst = 0;
    if(!HostTable(g, Src, &stab, &sst) || !HostTable(g, Dst, &dtab, &dst))
@@ -493,7 +564,6 @@ int BuildDownlink(GmlCtx *g, int Src, int Dst)
    int w = LenMatBas[Src][Dst], cnt, vec, typ;
    GetCntVec(w, &cnt, &vec, &typ);
    int rw = cnt * vec, ni = NmbItems(Src, Dst);
-   std::string nam = std::string(BalTypStr[Src]) + BalTypStr[Dst] + "Lnk";
    int idx = NewDatIdx(g);
    if(!idx) return 0;
    Data *d = &g->dat[idx];
@@ -534,6 +604,13 @@ int GmlxBuildLink(size_t GmlIdx, int Src, int Dst)
    return 0;
 }
 
+int GmlxGetLinkData(size_t GmlIdx, int Src, int Dst)
+{
+   GML_CTX(g, GmlIdx);
+   if(Src < 0 || Src >= GmlMaxEleTyp || Dst <= GmlVertices || Dst >= GmlMaxEleTyp) return 0;
+   return g->LnkMat[Src][Dst];
+}
+
 int GmlxGetUplinkTable(size_t GmlIdx, int Src, int Dst, int *deg, int *ball, int width)
 {
    GML_CTX(g, GmlIdx);
@@ -554,11 +631,81 @@ int GmlxGetUplinkTable(size_t GmlIdx, int Src, int Dst, int *deg, int *ball, int
    return 1;
 }
 
+// Neighbours of the elements of kind typ (GmlSetNeighbours,
+// sources/gmlib3.c:4178-4267): a link data "ngb" with, for each element and
+// each of its items of kind NgbTyp[typ] (edges: vertices; triangles and
+// quads: edges; tets: triangles; hexes: quads), the element sharing that
+// item, 0 if none.  When more than one other element shares it, the lowest
+// index is kept: the last match of the reference's chain walk.  The
+// reference hands the table to GmlSetDataLine as a pointer where ints are
+// read (so its entries are not the neighbours) and sizes it with the number
+// of faces, too narrow for triangles and quads; here each line holds one
+// entry per item (2, 3, 4, 4, 6 for edges, triangles, quads, tets, hexes,
+// padded to the link vector width).  Pyramids and prisms are refused as in
+// the reference.  Returns the link data index, 0 on failure.
 int GmlSetNeighbours(size_t GmlIdx, int typ)
 {
-   (void)GmlIdx; (void)typ;
-   fprintf(stderr, "gml: GmlSetNeighbours is not provided by this engine (neighbour links are outside its scope)\n");
-   return 0;
+   static const int NgbTyp[GmlMaxEleTyp] = {-1, GmlVertices, GmlEdges, GmlEdges, GmlTriangles, GmlTriangles,
+                                            GmlTriangles, GmlQuadrilaterals};
+   GML_CTX(g, GmlIdx);
+   if(typ == GmlPyramids || typ == GmlPrisms)
+   {
+      puts("Neighbours calculation between prisms or pyramids is not yet implemented.");
+      return 0;
+   }
+   if(typ <= GmlVertices || typ >= GmlMaxEleTyp || !g->TypIdx[typ]) return 0;
+   if(g->LnkMat[typ][typ])
+   {
+      fprintf(stderr, "gml: GmlSetNeighbours: the %s neighbours are already built\n", BalTypStr[typ]);
+      return 0;
+   }
+   const int ngb = NgbTyp[typ], w = NmbTpoLnk[typ][ngb];
+   const int idx = GmlNewLinkData(GmlIdx, typ, typ, w, (char *)"ngb");
+   if(!idx) return 0;
+   Data *d = &g->dat[idx];
+   const int rw = d->NmbItm * d->ItmLen;
+   if(!g->host_only)
+   {
+      if(!GpuNeighbours(g, typ, ngb, idx, rw))
+      {
+         g->LnkMat[typ][typ] = 0;
+         FreeDataSlot(g, idx);
+         return 0;
+      }
+      return idx;
+   }
+   // host-only handle: (sorted nodes, element, local item) sorted, runs of equal nodes
+   std::vector<int> tab;
+   int st = 0;
+   if(!HostTable(g, typ, &tab, &st)) return 0;
+   const long long ne = g->NmbEle[typ];
+   const int ni = NmbItems(typ, ngb), nn = ngb == GmlVertices ? 1 : ngb == GmlEdges ? 2 : FaceNodes(ngb);
+   struct Ent { Key k; int e, j; };
+   std::vector<Ent> all;
+   all.reserve((size_t)ne * ni);
+   for(long long e = 0; e < ne; e++)
+      for(int j = 0; j < ni; j++)
+      {
+         Key k;
+         if(ItemKey(typ, &tab[(size_t)e * st], ngb, j, &k) != nn) continue;
+         all.push_back({k, (int)e, j});
+      }
+   std::sort(all.begin(), all.end(), [](const Ent &a, const Ent &b) {
+      return a.k != b.k ? a.k < b.k : (a.e != b.e ? a.e < b.e : a.j < b.j); });
+   int *t = (int *)d->dev;
+   for(size_t a = 0, b; a < all.size(); a = b)
+   {
+      for(b = a; b < all.size() && all[b].k == all[a].k; b++) {}
+      for(size_t i = a; i < b; i++)
+      {
+         int r = 0;
+         for(size_t k = a; k < b; k++)
+            if(all[k].e != all[i].e) { r = all[k].e; break; }
+         if(all[i].j < rw) t[(size_t)all[i].e * rw + all[i].j] = r;
+      }
+   }
+   d->host_valid = false;
+   return idx;
 }
 
 } // extern "C"

@@ -66,9 +66,15 @@ int    GmlxInvalidateHost(size_t GmlIdx, int DatIdx);
 /* --- topology ------------------------------------------------------------ */
 
 /* Build (if needed) the SrcTyp -> DstTyp link that GmlCompileKernel would
-   build, and return 1.  Uplinks (lower to higher dimension) are built on the
-   GPU when SrcTyp is GmlVertices. */
+   build, and return 1.  On a device handle every kind is built on the GPU:
+   vertex balls, edge and face shells, downlinks (element -> its edges or
+   faces). */
 int    GmlxBuildLink(size_t GmlIdx, int SrcTyp, int DstTyp);
+
+/* Data index of the SrcTyp -> DstTyp link table (a downlink, or the
+   neighbours of GmlSetNeighbours when SrcTyp == DstTyp), 0 if none; its
+   lines are read with GmlGetDataLine. */
+int    GmlxGetLinkData(size_t GmlIdx, int SrcTyp, int DstTyp);
 
 /* Copy an uplink out in the reference's row-major form: deg[NmbSrc] and
    ball[NmbSrc * width] with entries (element << 4) | local index in

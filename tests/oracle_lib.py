@@ -194,3 +194,99 @@ def extract_items(deg, tables, old=None, oldref=None, tabsiz=0):
     if m < 0:
         raise MemoryError("orc_extract_items")
     return nodes[:m].copy(), refs[:m].copy()
+
+
+# ---------------------------------------------------------------------------
+# The reference's hash links, restated in pure Python for small meshes:
+# GetItmNod (gmlib3.c:1567-1620), CalHshKey (1627-1638), AddHsh (1645-1674),
+# GetHsh (1681-1721), used by NewBallData (743-1168) for shells and
+# downlinks and by GmlSetNeighbours (4178-4267).  The row orders come out of
+# the chain walk itself (head insertion, walk from the head).
+# ---------------------------------------------------------------------------
+
+# NmbTpoLnk[element kind][item kind] (gmlib3.c:316-325) for the kinds used here
+TPO = {(1, 0): 2, (2, 1): 3, (3, 1): 4, (4, 0): 4, (4, 1): 6, (4, 2): 4, (5, 1): 8, (7, 0): 8, (7, 1): 12,
+       (7, 3): 6, (1, 1): 2, (2, 2): 3, (3, 3): 4, (4, 4): 4, (7, 7): 6}
+NGB_TYP = {1: 0, 2: 1, 3: 1, 4: 2, 7: 3}           # NgbTyp (gmlib3.c:327)
+ITEM_NODES = {0: 1, 1: 2, 2: 3, 3: 4}
+
+
+def item_nodes(ele, ele_kind, itm_kind, j):
+    """GetItmNod: the sorted nodes of local item j of an element."""
+    if itm_kind == 0:
+        return (int(ele[j]),)
+    if itm_kind == 1:
+        loc = EDGE_LOC[ele_kind][j] if ele_kind >= 2 else (0, 1)
+        return tuple(sorted(int(ele[k]) for k in loc))
+    f = FACE_LOC[ele_kind][j]
+    return tuple(sorted(int(ele[k]) for k in f[:ITEM_NODES[itm_kind]]))
+
+
+class RefHash:
+    W = (3, 5, 7, 11)
+
+    def __init__(self, tabsiz, datlen):
+        self.tabsiz, self.datlen = max(1, tabsiz), datlen
+        self.head = {}
+        self.buc = [None]
+
+    def key(self, nodes):
+        if self.datlen == 1:
+            return nodes[0]
+        return sum(w * n for w, n in zip(self.W, nodes)) % self.tabsiz
+
+    def add(self, nodes, e, j):
+        k = self.key(nodes)
+        self.buc.append((tuple(nodes), e, j, self.head.get(k, 0)))
+        self.head[k] = len(self.buc) - 1
+
+    def get(self, nodes):
+        """codes (e << 4) | j of the matches, in chain order"""
+        out, b = [], self.head.get(self.key(nodes), 0)
+        while b:
+            nod, e, j, nxt = self.buc[b]
+            if nod == tuple(nodes):
+                out.append((e << 4) | j)
+            b = nxt
+        return out
+
+
+def ref_shells(src, src_kind, dst, dst_kind):
+    """Uplink rows (edge or face -> elements) in the chain order."""
+    h = RefHash(len(src), ITEM_NODES[src_kind])
+    for i, e in enumerate(dst):
+        for j in range(TPO[(dst_kind, src_kind)]):
+            h.add(item_nodes(e, dst_kind, src_kind, j), i, j)
+    return [h.get(item_nodes(s, src_kind, src_kind, 0)) for s in src]
+
+
+def ref_downlink(src, src_kind, dst, dst_kind, width):
+    """Downlink table: the first chain match's entity index, 0 if none."""
+    h = RefHash(len(dst), ITEM_NODES[dst_kind])
+    for i, d in enumerate(dst):
+        h.add(item_nodes(d, dst_kind, dst_kind, 0), i, 0)
+    out = np.zeros((len(src), width), np.int32)
+    for i, e in enumerate(src):
+        for j in range(TPO[(src_kind, dst_kind)]):
+            m = h.get(item_nodes(e, src_kind, dst_kind, j))
+            if m:
+                out[i, j] = m[0] >> 4
+    return out
+
+
+def ref_neighbours(tab, kind, width):
+    """GmlSetNeighbours' TetNgb rows: per item, the last chain match of
+    another element, 0 if none."""
+    ng = NGB_TYP[kind]
+    ni = TPO[(kind, ng)]
+    h = RefHash(len(tab) * TPO[(kind, kind)] // 2, ITEM_NODES[ng])
+    for i, e in enumerate(tab):
+        for j in range(ni):
+            h.add(item_nodes(e, kind, ng, j), i, j)
+    out = np.zeros((len(tab), width), np.int32)
+    for i, e in enumerate(tab):
+        for j in range(ni):
+            for c in h.get(item_nodes(e, kind, ng, j)):
+                if c >> 4 != i:
+                    out[i, j] = c >> 4
+    return out
