@@ -123,7 +123,7 @@ _SIGS = {
     "GmlxServeInfo": (_I, [_S, _DP]),
     "GmlxServeStop": (None, [_S]),
     "GmlxSetStaging": (_I, [_I]),
-    "GmlxHilbertRenumber": (_I, [_I, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "GmlxHilbertRenumber": (_I, [_I, _P, _P, _I, _I, _P, _P, _P, _P, _I]),
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)

@@ -7,7 +7,11 @@
 //
 // Keys: 21 bits per axis (63-bit Hilbert index, gml_hilbert.h); sorting is
 // a stable radix sort of (key, index) pairs, so equal keys keep the input
-// order and the result is deterministic.
+// order and the result is deterministic.  The optional degree class sets the
+// vertex key's top bit for vertices in more than BasDeg elements, which puts
+// them after all the others (the "-gmlib" preprocessing the reference's
+// documentation asks for: low-degree vertices first, high-degree ones at the
+// end, Documentation/GMlib3_en.tex:401).
 
 #include "gml_internal.h"
 #include "gml_device.h"
@@ -30,13 +34,24 @@ __device__ __forceinline__ unsigned Quant(float x, float lo, float sc)
    return (unsigned)q;
 }
 
-__global__ void k_vertex_keys(long long nv, const v4f *__restrict__ crd, v4f lo, v4f sc,
-                              unsigned long long *__restrict__ key, int *__restrict__ idx)
+// element degree of every vertex (deg zeroed before)
+__global__ void k_vertex_degree(long long ne, int nnod, const int *__restrict__ ele, int *__restrict__ deg)
+{
+   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(i < ne * nnod) atomicAdd(&deg[ele[i]], 1);
+}
+
+// Hilbert key of each vertex; with deg, bit 63 set for degrees above basdeg
+// (the key uses 63 bits, so the high class sorts after the low one)
+__global__ void k_vertex_keys(long long nv, const v4f *__restrict__ crd, v4f lo, v4f sc, const int *__restrict__ deg,
+                              int basdeg, unsigned long long *__restrict__ key, int *__restrict__ idx)
 {
    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(v >= nv) return;
    v4f p = crd[v];
-   key[v] = HilbertKey(Quant(p.x, lo.x, sc.x), Quant(p.y, lo.y, sc.y), Quant(p.z, lo.z, sc.z), kHilBits);
+   unsigned long long k = HilbertKey(Quant(p.x, lo.x, sc.x), Quant(p.y, lo.y, sc.y), Quant(p.z, lo.z, sc.z), kHilBits);
+   if(deg && deg[v] > basdeg) k |= 1ull << 63;
+   key[v] = k;
    idx[v] = (int)v;
 }
 
@@ -87,9 +102,9 @@ using namespace gml;
 extern "C" {
 
 int GmlxHilbertRenumber(int NmbVer, float *crd, int *VerRef, int NmbEle, int nnod, int *ele, int *EleRef,
-                        int *VerOld, int *EleOld)
+                        int *VerOld, int *EleOld, int BasDeg)
 {
-   if(NmbVer < 1 || NmbEle < 0 || nnod < 1 || nnod > 8 || !crd || (NmbEle && !ele)) return 0;
+   if(NmbVer < 1 || NmbEle < 0 || nnod < 1 || nnod > 8 || !crd || (NmbEle && !ele) || BasDeg < 0) return 0;
    const long long nv = NmbVer, ne = NmbEle;
    for(long long i = 0; i < ne * nnod; i++)
       if(ele[i] < 0 || ele[i] >= NmbVer) return 0;
@@ -123,7 +138,7 @@ int GmlxHilbertRenumber(int NmbVer, float *crd, int *VerRef, int NmbEle, int nno
                       && hipMalloc(&eold, ne * 4) == hipSuccess))
           && (!ne || !EleRef || (hipMalloc(&der, ne * 4) == hipSuccess && hipMalloc(&der2, ne * 4) == hipSuccess));
    if(ok)
-      ok = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key2, idx, idx2, (int)nv, 0, 3 * kHilBits, s) == hipSuccess
+      ok = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key2, idx, idx2, (int)nv, 0, 64, s) == hipSuccess
         && (!ne || hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, key, key2, idx, idx2, (int)ne, 0, 3 * kHilBits, s)
                    == hipSuccess)
         && hipMalloc(&tmp, std::max(tb, tb2) + 16) == hipSuccess;
@@ -138,9 +153,17 @@ int GmlxHilbertRenumber(int NmbVer, float *crd, int *VerRef, int NmbEle, int nno
    if(ok)
    {
       // vertices: keys, sort, new -> old and old -> new, permuted coordinates
-      hipLaunchKernelGGL(k_vertex_keys, blocks(nv), dim3(B), 0, s, nv, dc, vlo, vsc, key, idx);
+      // (with a degree class: the degrees counted in vinv, rewritten below)
+      if(BasDeg > 0)
+      {
+         ok = hipMemsetAsync(vinv, 0, nv * 4, s) == hipSuccess;
+         if(ne) hipLaunchKernelGGL(k_vertex_degree, blocks(ne * nnod), dim3(B), 0, s, ne, nnod, de, vinv);
+      }
+      hipLaunchKernelGGL(k_vertex_keys, blocks(nv), dim3(B), 0, s, nv, dc, vlo, vsc, BasDeg > 0 ? vinv : nullptr,
+                         BasDeg, key, idx);
       size_t t = tb;
-      ok = hipcub::DeviceRadixSort::SortPairs(tmp, t, key, key2, idx, vold, (int)nv, 0, 3 * kHilBits, s) == hipSuccess;
+      ok = ok && hipcub::DeviceRadixSort::SortPairs(tmp, t, key, key2, idx, vold, (int)nv, 0,
+                                                    BasDeg > 0 ? 64 : 3 * kHilBits, s) == hipSuccess;
       hipLaunchKernelGGL(k_invert, blocks(nv), dim3(B), 0, s, nv, vold, vinv);
       hipLaunchKernelGGL(k_gather_vertices, blocks(nv), dim3(B), 0, s, nv, vold, dc, dvr, dc2, dvr2);
       // elements: barycentre keys (original coordinates), sort, renumbered tables

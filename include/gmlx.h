@@ -166,10 +166,15 @@ int    GmlxCubeMesh(int n, double jitter, unsigned long long seed, int order,
    bits per axis), elements (nnod vertex indices each, 0-based) along the
    same curve by their barycentre, and the element tables renumbered.  crd
    holds 4 floats per vertex; VerRef, EleRef, VerOld, EleOld may be NULL;
-   VerOld / EleOld receive, for each new index, the old one.  The sort is
-   stable (deterministic).  Returns 1, or 0 on invalid input or GPU failure. */
+   VerOld / EleOld receive, for each new index, the old one.  BasDeg > 0
+   also sorts the vertices by degree class: those in more than BasDeg
+   elements go after all the others, each class in Hilbert order (the
+   preprocessing the reference expects so that its base / high ball rows
+   are two ranges, Documentation/GMlib3_en.tex:401); 0 keeps the Hilbert
+   order only.  The sort is stable (deterministic).  Returns 1, or 0 on
+   invalid input or GPU failure. */
 int    GmlxHilbertRenumber(int NmbVer, float *crd, int *VerRef, int NmbEle, int nnod, int *ele, int *EleRef,
-                           int *VerOld, int *EleOld);
+                           int *VerOld, int *EleOld, int BasDeg);
 
 /* Edge-graph 3x3 vector Laplacian of a tet mesh (host, no handle): block
    CSR with one block row per vertex, columns ascending, off-diagonal block

@@ -74,7 +74,7 @@ def test_hilbert_renumber_shuffled_cube():
     c, r, t, tr = c0.copy(), r0.copy(), t0.copy(), tr0.copy()
     vold = np.empty(V, np.int32)
     eold = np.empty(T, np.int32)
-    assert G.lib().GmlxHilbertRenumber(V, G.ptr(c), G.ptr(r), T, 4, G.ptr(t), G.ptr(tr), G.ptr(vold), G.ptr(eold)) == 1
+    assert G.lib().GmlxHilbertRenumber(V, G.ptr(c), G.ptr(r), T, 4, G.ptr(t), G.ptr(tr), G.ptr(vold), G.ptr(eold), 0) == 1
     # same mesh, exactly permuted and renumbered
     assert np.array_equal(np.sort(vold), np.arange(V)) and np.array_equal(np.sort(eold), np.arange(T))
     vinv = np.empty(V, np.int64)
@@ -108,10 +108,33 @@ def test_hilbert_renumber_is_deterministic_and_checks_input():
     outs = []
     for _ in range(2):
         c, t = crd.copy(), tet.copy()
-        assert G.lib().GmlxHilbertRenumber(len(c), G.ptr(c), None, len(t), 4, G.ptr(t), None, None, None) == 1
+        assert G.lib().GmlxHilbertRenumber(len(c), G.ptr(c), None, len(t), 4, G.ptr(t), None, None, None, 0) == 1
         outs.append((c, t))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     bad = tet.copy()
     bad[0, 0] = len(crd)
     c = crd.copy()
-    assert G.lib().GmlxHilbertRenumber(len(c), G.ptr(c), None, len(bad), 4, G.ptr(bad), None, None, None) == 0
+    assert G.lib().GmlxHilbertRenumber(len(c), G.ptr(c), None, len(bad), 4, G.ptr(bad), None, None, None, 0) == 0
+
+
+def test_hilbert_renumber_degree_classes():
+    # Kuhn cube: interior vertices are in 24 tets, boundary ones in fewer; with
+    # BasDeg 20 every vertex above 20 goes after the others, each class in
+    # Hilbert order, and the mesh is the same
+    crd, ref, tet = G.cube_mesh(7, 0.2, 5, hilbert=False)
+    V, T = len(crd), len(tet)
+    deg0 = np.bincount(tet.ravel(), minlength=V)
+    c, t = crd.copy(), tet.copy()
+    vold = np.empty(V, np.int32)
+    assert G.lib().GmlxHilbertRenumber(V, G.ptr(c), None, T, 4, G.ptr(t), None, G.ptr(vold), None, 20) == 1
+    deg = np.bincount(t.ravel(), minlength=V)
+    assert np.array_equal(deg, deg0[vold]) and np.array_equal(c, crd[vold])
+    high = deg > 20
+    nlow = int((~high).sum())
+    assert nlow > 0 and high[nlow:].all() and not high[:nlow].any()
+    lo = crd[:, :3].min(axis=0).astype(np.float32)
+    hi = crd[:, :3].max(axis=0).astype(np.float32)
+    sc = (np.float32((1 << BITS) - 1) / (hi - lo)).astype(np.float32)
+    kv = keys(c, lo, sc)
+    assert all(a <= b for a, b in zip(kv[:nlow], kv[1:nlow])) and all(a <= b for a, b in zip(kv[nlow:], kv[nlow + 1:]))
+    assert sorted(O.tet_quality64(t, c)) == sorted(O.tet_quality64(tet, crd))
