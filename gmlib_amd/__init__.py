@@ -127,7 +127,8 @@ _SIGS = {
 }
 
 # variadic entry points: no argtypes (callers pass ctypes values)
-_VARIADIC = {"GmlSetDataLine": _I, "GmlGetDataLine": _I, "GmlCompileKernel": _I}
+_VARIADIC = {"GmlSetDataLine": _I, "GmlGetDataLine": _I, "GmlCompileKernel": _I, "GmlImportMesh": _I,
+             "GmlExportSolution": _I}
 
 _lib = None
 

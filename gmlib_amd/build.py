@@ -20,6 +20,7 @@ SOURCES = [
     "gml_runtime.cpp",
     "gml_codegen.cpp",
     "gml_extract.cpp",
+    "gml_meshb.cpp",
     "gml_meshgen.cpp",
     "gml_topology.hip",
     "gml_reduce.hip",

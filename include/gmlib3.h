@@ -93,6 +93,12 @@ int      GmlAddVec3           (size_t, int, int, int, int);            /* gmlib3
 int      GmlScaleVec          (size_t, int, double *);                 /* gmlib3.c:3495 */
 int      GmlNormVec           (size_t, int, int, double *);            /* gmlib3.c:3534 */
 
+/* Mesh and solution files (sources/gmlib3.h:120-123, there under
+   WITH_LIBMESHB): 0-terminated lists of libMeshb keyword codes (import) or
+   data indices (export); the codes are in gml_gmf.h or libMeshb's header. */
+int      GmlImportMesh        (size_t, char *, ...);                   /* gmlib3.c:4482 */
+int      GmlExportSolution    (size_t, char *, ...);                   /* gmlib3.c:4600 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,3 +176,27 @@ def test_gpu_extract_large_cube_counts():
     # first appearance: the first tet's edges lead, in local order
     want, _ = first_appearance([], tet[:2], TET_EDGES)
     assert np.array_equal(e[:len(want)], want)
+
+
+def test_gpu_extract_on_reference_sample_mesh(tmp_path):
+    # the reference's 2D sample mesh (a NACA 0012 airfoil triangulation, its
+    # 465 boundary edges with refs): GmlImportMesh, then the edge table of the
+    # whole mesh, the boundary edges first with their refs
+    import lzma
+    import os
+    data = lzma.open(os.path.join(os.path.dirname(__file__), "golden", "naca012_2D_triangles.meshb.xz")).read()
+    path = str(tmp_path / "naca.meshb")
+    open(path, "wb").write(data)
+    res = []
+    for host_only in (False, True):
+        with G.Gml(host_only=host_only) as g:
+            assert g.L.GmlImportMesh(C.c_size_t(g.h), C.c_char_p(path.encode()), C.c_int(4), C.c_int(5),
+                                     C.c_int(6), C.c_int(0)) == 3
+            old = get_block(g, G.GmlEdges, 465, 2)
+            ne = g.L.GmlExtractEdges(g.h)
+            res.append((ne, get_block(g, G.GmlEdges, ne, 2), old))
+    (ne, (e, r), (old, oref)), (ne2, (e2, r2), _) = res
+    assert ne == ne2 and np.array_equal(e, e2) and np.array_equal(r, r2)
+    # Euler for a plane triangulation with one hole: V - E + T = 0
+    assert ne == 36624 + 72783
+    assert np.array_equal(e[:465], np.sort(old, axis=1)) and np.array_equal(r[:465], oref) and (r[465:] == 0).all()
