@@ -52,7 +52,16 @@ once per call (16 B per tet, twice) plus the tables written (edges 8 B nodes
 restatement of the reference's hash (one thread, as the reference) on a
 smaller cube, in the same bytes per second.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract]
+Sixth workload (--workload heat): the reference's HeatCellCentered
+application (gmlib_amd/heat.py) on a 128^3-cell Kuhn cube (12.6M tets,
+25.3M faces): a step is one explicit iteration -- face values, cell
+gradients, face gradients, flux balance, update (five generated kernels over
+triangle -> tet shells and tet -> triangle downlinks) and the L2 residual,
+counted as 120 B per tet + 72 B per face + 32 B per vertex.  The CPU
+baseline is the numpy FP64 restatement of the same iteration on a smaller
+cube.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract|heat]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -78,7 +87,7 @@ def body(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract"])
+    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract", "heat"])
     ap.add_argument("--qps", type=float, default=40.0, help="serve workload: arrival rate (jobs/s)")
     ap.add_argument("--jobs", type=int, default=400, help="serve workload: jobs in the timed stream")
     ap.add_argument("--pool", type=int, default=24, help="serve workload: meshes in the pool")
@@ -115,6 +124,8 @@ def main():
         run_serve(args)
     elif args.workload == "extract":
         run_extract(args)
+    elif args.workload == "heat":
+        run_heat(args)
     else:
         run_smoothing(args)
 
@@ -891,6 +902,92 @@ def run_extract(args):
                                "sample": "%d^3-cell cube (%d tets -> %d edges, %d triangles): the oracle's restatement "
                                          "of the reference hash (GetItmNod / CalHshKey / AddHsh / GetHsh), one "
                                          "thread, %.2f s" % (m, ct_n, ce, cf, c)}
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+def heat_bytes(T, F, V):
+    """Algorithmic bytes of one heat iteration, each datum once: face values
+    (shell slots + degree + ref 16F, cell values 4T, out 4F), cell gradients
+    (connectivity 16T, coordinates 16V, downlink 16T, face values 4F, out
+    16T), face gradients (shell + ref 16F, cell gradients 16T, out 16F), flux
+    balance (16T + 16V + 16T, face gradients 16F, out 4T), update (12T), L2
+    residual (4T)."""
+    return 120 * T + 72 * F + 32 * V
+
+
+def run_heat(args):
+    import numpy as np
+    import gmlib_amd as G
+    from gmlib_amd import heat as H
+
+    n = args.n if args.n != 256 else 128
+    t0 = time.time()
+    crd, vref, tet, tri, side = H.heat_cube(n, 0.2, 11)
+    s = H.HeatSolver(crd, vref, tet, tri, side)
+    T, F, V = len(tet), s.nface, len(crd)
+    log("heat: %d^3 cube, %d tets, %d faces, set-up (mesh, faces, links, compile, dt) %.1fs"
+        % (n, T, F, time.time() - t0))
+    s.iterate(max(args.warmup, 1))
+    kern = {"heat_face_value": s.k_face, "heat_cell_gradient": s.k_grad, "heat_face_gradient": s.k_fgrad,
+            "heat_balance": s.k_bal, "heat_step": s.k_step}
+    k0 = {k: s.g.kernel_time(v) for k, v in kern.items()}
+    s.g.L.GmlxSync(s.g.h)
+    t0 = time.perf_counter()
+    res = s.iterate(args.steps)
+    s.g.L.GmlxSync(s.g.h)
+    dt = (time.perf_counter() - t0) / args.steps
+    kt = {k: (s.g.kernel_time(v) - k0[k]) / args.steps for k, v in kern.items()}
+    kb = {"heat_face_value": 20 * F + 4 * T, "heat_cell_gradient": 48 * T + 16 * V + 4 * F,
+          "heat_face_gradient": 32 * F + 16 * T, "heat_balance": 36 * T + 16 * V + 16 * F, "heat_step": 12 * T}
+    dom = max(kt, key=kt.get)
+    alg = heat_bytes(T, F, V)
+    log("heat: %.3f ms per iteration (%.1f GB/s of the byte model); kernels (us): %s; residual %.3e -> %.3e"
+        % (dt * 1e3, alg / dt / 1e9, {k: round(v * 1e6, 1) for k, v in kt.items()}, res[0], res[-1]))
+    kgbs = kb[dom] / kt[dom] / 1e9
+    out = {
+        "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
+        "value": round(alg / dt / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered Kuhn cube mesh, boundary triangles refs 1-6 by side)",
+        "config": {"workload": "cell-centered heat equation (HeatCellCentered), %d^3-cell Kuhn cube (%d tets, "
+                               "%d faces, %d vertices), one explicit iteration + L2 residual per step" % (n, T, F, V),
+                   "tets": T, "faces": F, "vertices": V, "bytes_per_step": alg,
+                   "kernel_us": {k: round(v * 1e6, 2) for k, v in kt.items()}},
+        "iterations_per_s": round(1 / dt, 1),
+        "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "bytes_per_launch": kb[dom], "avg_us": round(kt[dom] * 1e6, 2)},
+        "cpu_baseline": None,
+    }
+    s.close()
+    if args.cpu_seconds > 0:
+        from tests import oracle_lib as O
+        m = min(n, 48)
+        c, vr, tt, tr, sd = H.heat_cube(m, 0.2, 11)
+        full, fref = O.extract_items(3, [(G.GmlTetrahedra, tt)], tr, sd)
+        O.heat_iterations(c, tt, full, fref, H.DEFAULT_BC, 1)
+        k = 1
+        while True:
+            t0 = time.perf_counter()
+            O.heat_iterations(c, tt, full, fref, H.DEFAULT_BC, k)
+            ct = time.perf_counter() - t0
+            if ct > args.cpu_seconds / 3 or k >= 64:
+                break
+            k *= 2
+        per = ct / k
+        out["cpu_baseline"] = {"value": round(heat_bytes(len(tt), len(full), len(c)) / per / 1e9, 3), "unit": "GB/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "%d iterations of the numpy FP64 restatement (links and geometry set up "
+                                         "once per call) on a %d^3 cube (%d tets), %.2f s" % (k, m, len(tt), ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 

@@ -290,3 +290,84 @@ def ref_neighbours(tab, kind, width):
                 if c >> 4 != i:
                     out[i, j] = c >> 4
     return out
+
+
+# ---------------------------------------------------------------------------
+# Cell-centered heat equation (the reference's examples/HeatCellCentered
+# scheme: sol_ext.cl, grd_tet.cl, grd_ext.cl, flx_bal.cl, tim_int.cl, dt.cl),
+# restated in numpy FP64.  Links from the mesh: the two tets of each
+# triangle in descending index (the shell order; the second is absent on the
+# boundary and reads as 0) and the triangle of each tet face, faces in the
+# order opposite vertex 0, 1, 2, 3.
+# ---------------------------------------------------------------------------
+
+HEAT_FACES = ((1, 2, 3), (2, 0, 3), (3, 0, 1), (0, 2, 1))
+CAL_TET_VOL = 0.166666          # the toolkit's CalTetVol factor (toolkit.cl)
+
+
+def heat_links(nver, tet, tri):
+    """(shell [F, 2] with -1 for none, down [T, 4])"""
+    tet = np.asarray(tet, np.int64)
+    tri = np.asarray(tri, np.int64)
+    V = nver
+
+    def key(a):
+        a = np.sort(a, axis=1)
+        return (a[:, 0] * V + a[:, 1]) * V + a[:, 2]
+
+    ktri = key(tri)
+    order = np.argsort(ktri, kind="stable")
+    T = len(tet)
+    down = np.zeros((T, 4), np.int64)
+    shell = -np.ones((len(tri), 2), np.int64)
+    for j, f in enumerate(HEAT_FACES):
+        k = key(tet[:, list(f)])
+        pos = np.searchsorted(ktri[order], k)
+        assert (ktri[order][pos] == k).all()
+        down[:, j] = order[pos]
+    # tets of each triangle, descending
+    t = np.repeat(np.arange(T), 4)
+    f = down.ravel()
+    o = np.lexsort((-t, f))
+    f, t = f[o], t[o]
+    first = np.r_[True, f[1:] != f[:-1]]
+    shell[f[first], 0] = t[first]
+    second = ~first
+    shell[f[second], 1] = t[second]
+    return shell, down
+
+
+def heat_geometry(crd, tet):
+    """area vectors (x2) of the four faces of each tet [T, 4, 3] and the volume as CalTetVol [T]"""
+    c = np.asarray(crd, np.float64)[:, :3]
+    tet = np.asarray(tet, np.int64)
+    P = [c[tet[:, i]] for i in range(4)]
+    nrm = np.stack([np.cross(P[b] - P[a], P[cc] - P[a]) for a, b, cc in HEAT_FACES], 1)
+    vol = CAL_TET_VOL * np.einsum("ij,ij->i", np.cross(P[1] - P[0], P[2] - P[0]), P[3] - P[0])
+    return nrm, vol
+
+
+def heat_iterations(crd, tet, tri, triref, bc, niter, sol=None, dt=None):
+    """niter explicit steps from sol (zeros by default): (sol, [residual per step], dt)"""
+    shell, down = heat_links(len(crd), tet, tri)
+    nrm, vol = heat_geometry(crd, tet)
+    if dt is None:
+        dt = float((0.3 * vol ** (2. / 3.)).min())
+    u = np.zeros(len(tet)) if sol is None else np.asarray(sol, np.float64).copy()
+    has2 = shell[:, 1] >= 0
+    s1 = np.where(has2, shell[:, 1], 0)
+    res = []
+    for _ in range(niter):
+        uf = 0.5 * (u[shell[:, 0]] + np.where(has2, u[s1], 0.))
+        for kind, ref, val in bc:
+            m = triref == ref
+            uf[m] = val if kind == 0 else u[shell[m, 0]]
+        g = 0.5 * np.einsum("tj,tjk->tk", uf[down], nrm) / vol[:, None]
+        gf = 0.5 * (g[shell[:, 0]] + np.where(has2[:, None], g[s1], 0.))
+        for kind, ref, val in bc:
+            m = triref == ref
+            gf[m] = g[shell[m, 0]] if kind == 0 else val
+        rhs = 0.5 * np.einsum("tjk,tjk->t", gf[down], nrm) / vol
+        u = u + dt * rhs
+        res.append(float((rhs * rhs).sum()))
+    return u, res, dt
