@@ -239,3 +239,40 @@ def test_reference_ball_quality_bodies_compile():
         deg, ball = g.uplink_table(G.GmlVertices, G.GmlTriangles, 4, 8)
         assert list(deg) == [1, 2, 2, 1]
         assert list(ball[1, :2]) == [(1 << 4) | 0, (0 << 4) | 1] and list(ball[2, :2]) == [(1 << 4) | 2, (0 << 4) | 2]
+
+
+@needs_ref
+def test_reference_heat_bodies_compile():
+    # HeatCellCentered: triangle kernels over tri -> tet shells with TriRef,
+    # tet kernels over the tet -> tri downlink, the parameter block of enums
+    from gmlib_amd import heat as H
+    crd, vref, tet, tri, side = H.heat_cube(2, 0.2, 1)
+    with G.Gml(host_only=True) as g:
+        L, h = g.L, g.h
+        vi, ti = g.set_mesh(crd, vref, tet)
+        g.new_parameters(76, ref_body("HeatCellCentered/param.cl"))
+        nt = len(tri)
+        assert L.GmlNewMeshData(h, G.GmlTriangles, nt)
+        assert L.GmlSetDataBlock(h, G.GmlTriangles, 0, nt - 1, G.ptr(tri), C.c_void_p(tri.ctypes.data + (nt - 1) * 12),
+                                 G.ptr(side), C.c_void_p(side.ctypes.data + (nt - 1) * 4))
+        assert L.GmlExtractFaces(h) > nt
+        idx = C.c_int(0)
+        L.GmlGetMeshInfo(h, G.GmlTriangles, None, C.byref(idx))
+        tr = idx.value
+        T, F = G.GmlTetrahedra, G.GmlTriangles
+        sol = g.solution(T, 1, G.GmlFlt, "SolTet")
+        grd = g.solution(T, 1, G.GmlFlt4, "GrdTet")
+        sx = g.solution(F, 1, G.GmlFlt, "SolExt")
+        gx = g.solution(F, 1, G.GmlFlt4, "GrdExt")
+        rhs = g.solution(T, 1, G.GmlFlt, "Rhs")
+        dt = g.solution(T, 1, G.GmlFlt, "dt")
+        R, W, RF = G.GmlReadMode, G.GmlWriteMode, G.GmlRefFlag
+        cases = [("ini_tet", T, [(vi, R, 0), (sol, W, 0)]),
+                 ("sol_ext", F, [(tr, R | RF, 0), (sol, R, 0), (sx, W, 0)]),
+                 ("grd_tet", T, [(vi, R, 0), (sx, R, 0), (grd, W, 0)]),
+                 ("grd_ext", F, [(tr, R | RF, 0), (grd, R, 0), (gx, W, 0)]),
+                 ("flx_bal", T, [(vi, R, 0), (gx, R, 0), (rhs, W, 0)]),
+                 ("tim_int", T, [(rhs, R, 0), (sol, R | W, 0)]),
+                 ("dt", T, [(vi, R, 0), (dt, W, 0)])]
+        for name, typ, args in cases:
+            assert g.compile(ref_body("HeatCellCentered/%s.cl" % name), name, typ, args) > 0, name
