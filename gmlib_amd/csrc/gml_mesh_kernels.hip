@@ -191,15 +191,19 @@ __global__ void __launch_bounds__(256) k_sweep_fused(long long nv, const unsigne
 }
 
 // Operator-form Jacobi sweep over one tile: stage the tile's 1-ring
-// coordinates in LDS, then x_v += omega (sum_u c_vu x_u / (4 deg v) - x_v)
-// with the 1-ring read as (tile-local index << 8 | c_vu) slots.
-// The vertex's own data (degree word, ref, coordinates, first 16 ring
-// slots) are loaded before the tile staging so that both latencies overlap.
-__global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned int *__restrict__ vofs,
-                                                   const int *__restrict__ vlist, const unsigned int *__restrict__ meta,
-                                                   const unsigned int *__restrict__ noff,
-                                                   const unsigned short *__restrict__ nbr, const int *__restrict__ ref,
-                                                   const v4f *__restrict__ xo, v4f *__restrict__ xn, float omega, int remap)
+// coordinates in LDS (its own vertices at local 0..TV-1 straight from the
+// coordinate array, the listed halo from TV on), then
+// x_v += omega (sum_u c_vu x_u / (4 deg v) - x_v) with the 1-ring read as
+// (tile-local index << 4 | c_vu) slots.  The vertex's own data (degree
+// word, ref, coordinates, first 16 ring slots) are loaded before the halo
+// staging so that both latencies overlap.  One vertex per thread, TV =
+// work-group size (64..1024).
+__global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned int *__restrict__ vofs,
+                                                    const int *__restrict__ vlist, const unsigned int *__restrict__ meta,
+                                                    const unsigned int *__restrict__ noff,
+                                                    const unsigned short *__restrict__ nbr, const int *__restrict__ ref,
+                                                    const v4f *__restrict__ xo, v4f *__restrict__ xn, float omega,
+                                                    int remap)
 {
    extern __shared__ v4f lds[];
    const unsigned int tile = xcd_block(remap);
@@ -216,21 +220,23 @@ __global__ void __launch_bounds__(256) k_sweep_op(long long nv, const unsigned i
    v4f p = act ? xo[v] : (v4f){0.f, 0.f, 0.f, 0.f};
    const int rf = act ? ref[v] : 1;
 
-   const unsigned int v0 = vofs[tile], nvt = vofs[tile + 1] - v0;
+   const unsigned int h0 = vofs[tile], nh = vofs[tile + 1] - h0;
    v4f *X = lds;
-   for(unsigned int i = tid; i < nvt; i += 4 * TV)
+   X[tid] = p;
+   v4f *H = lds + TV;
+   for(unsigned int i = tid; i < nh; i += 4 * TV)
    {
       int vi[4];
 #pragma unroll
       for(int u = 0; u < 4; u++)
-         vi[u] = i + u * TV < nvt ? __builtin_nontemporal_load(&vlist[v0 + i + u * TV]) : -1;
+         vi[u] = i + u * TV < nh ? __builtin_nontemporal_load(&vlist[h0 + i + u * TV]) : -1;
       v4f x[4];
 #pragma unroll
       for(int u = 0; u < 4; u++)
          x[u] = vi[u] >= 0 ? xo[vi[u]] : (v4f){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for(int u = 0; u < 4; u++)
-         if(vi[u] >= 0) X[i + u * TV] = x[u];
+         if(vi[u] >= 0) H[i + u * TV] = x[u];
    }
    __syncthreads();
    if(!act) return;
@@ -358,6 +364,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    if(g_smooth_mode == 1)
    {
       const Shape sf = g_shape[GmlxBuiltinSweep];
+      if(sf.grp > 256) return -4;              // the exact fused kernel takes tiles of up to 256
       if(g->plan && g->plan->TV != sf.grp) { FreeSweepPlan(g, g->plan); g->plan = nullptr; }
       if(!g->plan && !(g->plan = NewSweepPlan(g, sf.grp, 0))) return -4;
       if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
@@ -424,7 +431,7 @@ int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
 int GmlxSetBuiltinShape(int which, int GrpSiz, int Remap)
 {
    if(which < 0 || which > GmlxBuiltinCgUpdate || GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
-   if(which >= GmlxBuiltinSweep && GrpSiz > 256) return 0;
+   if(which > GmlxBuiltinSweep && GrpSiz > 256) return 0;
    g_shape[which].grp = GrpSiz;
    g_shape[which].remap = Remap ? 1 : 0;
    return 1;

@@ -12,13 +12,16 @@
 // LDS, and sums each vertex's ball from LDS in the ball's own order -- the
 // same additions, in the same order, as the scatter + gather kernels.
 //
-// The operator form (op = 1) keeps the tile vertex lists and replaces the
-// tet lists by each vertex's 1-ring: the mean ball barycentre of v is
-// sum_u c_vu x_u / (4 deg v), c_vu = number of ball tets of v that contain
-// u (c_vv = deg v), so a sweep applies a fixed sparse operator.  The ring
-// of v without v itself is stored as 16-bit entries (tile-local index << 4
-// | c_vu), ascending local index (a count above 15 continues in the next
-// entry); one 32-bit word per vertex holds (deg v << 16 | ring length).
+// The operator form (op = 1) replaces the tet lists by each vertex's 1-ring:
+// the mean ball barycentre of v is sum_u c_vu x_u / (4 deg v), c_vu =
+// number of ball tets of v that contain u (c_vv = deg v), so a sweep applies
+// a fixed sparse operator.  Its tile-local numbering puts the tile's own
+// vertices first (local index v - v0, staged with coalesced loads) and the
+// halo (the other 1-ring vertices, ascending) from TV on; only the halo is
+// listed.  The ring of v without v itself is stored as 16-bit entries
+// (tile-local index << 4 | c_vu), ascending local index (a count above 15
+// continues in the next entry); one 32-bit word per vertex holds
+// (deg v << 16 | ring length).
 
 #include "gml_internal.h"
 
@@ -75,6 +78,16 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
             if(ts.size() > 65535 || vs.size() > 65535) { fits = false; continue; }
             if(op)
             {
+               // tile-local numbering: the tile's own vertices first (local
+               // v - v0), then its halo (the other 1-ring vertices) from TV
+               // on, ascending; the plan stores the halo list only
+               std::vector<int> halo;
+               for(int x : vs)
+                  if(x < v0 || x >= v1) halo.push_back(x);
+               auto loc = [&](int x) -> unsigned int {
+                  if(x >= v0 && x < v1) return (unsigned int)(x - v0);
+                  return (unsigned int)(TV + (std::lower_bound(halo.begin(), halo.end(), x) - halo.begin()));
+               };
                // 1-ring counts of every tile vertex, ascending tile-local index
                std::vector<unsigned int> cnt;
                for(long long v = v0; v < v1; v++)
@@ -83,13 +96,12 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
                   for(int k = 0; k < deg[v]; k++)
                   {
                      int e = ent[((size_t)off[v >> 6] + k) * 64 + (v & 63)] >> 4;
-                     for(int j = 0; j < 4; j++)
-                        cnt.push_back((unsigned int)(std::lower_bound(vs.begin(), vs.end(), tet[(size_t)e * 4 + j]) - vs.begin()));
+                     for(int j = 0; j < 4; j++) cnt.push_back(loc(tet[(size_t)e * 4 + j]));
                   }
                   std::sort(cnt.begin(), cnt.end());
                   auto &R = ring[v];
                   R.clear();
-                  const unsigned int self = (unsigned int)(std::lower_bound(vs.begin(), vs.end(), (int)v) - vs.begin());
+                  const unsigned int self = (unsigned int)(v - v0);
                   for(size_t i = 0; i < cnt.size();)
                   {
                      size_t j = i;
@@ -102,7 +114,7 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
                   }
                   if(R.size() > 65535 || deg[v] > 65535) fits = false;
                }
-               tv[t] = vs;
+               tv[t] = halo;
                continue;
             }
             auto &L = tl[t];
@@ -136,6 +148,8 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
    std::vector<int> vlist(vofs[ntile] + 1);
    if(op)
    {
+      // staged per tile: TV own slots + the halo
+      p->max_nv += TV;
       for(long long t = 0; t < ntile; t++)
          std::copy(tv[t].begin(), tv[t].end(), vlist.begin() + vofs[t]);
       std::vector<unsigned int> noff((size_t)nchunk + 1, 0);
@@ -171,7 +185,7 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
          return nullptr;
       }
       p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + meta.size() * 4 + nbr.size() * 2;
-      p->sum_nv = vofs[ntile];
+      p->sum_nv = vofs[ntile] + nv;
       g->MemSiz += p->bytes;
       DbgPrint(g, "Operator sweep plan: %lld tiles of %d vertices, %u tile vertices, %lld ring slabs, %zu MB\n",
                ntile, TV, vofs[ntile], p->nslab, p->bytes >> 20);

@@ -142,8 +142,9 @@ int    GmlxSetSmoothMode(int mode);
    tile, plan bytes, total 1-ring slots (operator form), mode} */
 int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
 /* launch shape of a built-in kernel for every handle of the process: GrpSiz a
-   multiple of 64, up to 1024 (quality, ball-min, scatter, gather) or 256
-   (sweep, SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
+   multiple of 64, up to 1024 (quality, ball-min, scatter, gather, sweep: the
+   tile of the operator-form sweep; the exact fused sweep takes up to 256) or
+   256 (SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
 int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);
 /* summed device time (s) and launch count of a built-in kernel on a handle
    (GmlxBuiltinSpMV: the block SpMV of GmlMultMatVec; GmlxBuiltinCgSpMV and

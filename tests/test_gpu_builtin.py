@@ -19,11 +19,13 @@ def body(name):
 
 @pytest.mark.parametrize("n,grp,remap,fused", [(20, 256, 1, 1), (13, 64, 0, 1), (11, 128, 1, 1),
                                                 (20, 256, 1, 0), (5, 1024, 1, 0),
-                                                (20, 256, 1, 2), (13, 64, 0, 2)])
+                                                (20, 256, 1, 2), (13, 64, 0, 2), (20, 512, 1, 2),
+                                                (11, 1024, 0, 2)])
 def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused):
     for w in range(4):
         assert G.lib().GmlxSetBuiltinShape(w, grp, remap)
-    assert G.lib().GmlxSetBuiltinShape(G.GmlxBuiltinSweep, min(grp, 256), remap)
+    tile = grp if fused == 2 else min(grp, 256)        # the operator form takes tiles up to 1024
+    assert G.lib().GmlxSetBuiltinShape(G.GmlxBuiltinSweep, tile, remap)
     assert G.lib().GmlxSetSmoothMode(fused)
     crd, ref, tet = G.cube_mesh(n, 0.2, 3)
     omega = 0.2
@@ -57,7 +59,7 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
                 assert g.L.GmlxGetBuiltinRunTime(g.h, which, C.byref(nl)) > 0 and nl.value == 5
                 if fused:
                     info = (C.c_longlong * 9)()
-                    assert g.L.GmlxGetSweepPlanInfo(g.h, info) and info[0] == min(grp, 256)
+                    assert g.L.GmlxGetSweepPlanInfo(g.h, info) and info[0] == tile
                     print("plan: tiles %d, tile tets per tet %.3f, tile vertices per vertex %.3f"
                           % (info[1], info[3] / len(tet), info[2] / len(crd)))
             c, r = g.vertices(len(crd))
