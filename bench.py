@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--nmesh", type=int, default=512, help="batch workload: number of meshes")
     ap.add_argument("--plain", type=int, default=1, help="batch workload: also time the plain per-mesh path (0: skip)")
     ap.add_argument("--shape", default="", help="batch workload: TV/Pad to use instead of the autotuner (e.g. 256/4)")
+    ap.add_argument("--staging", type=int, default=-1,
+                    help="LDS staging of uplink gathers in generated kernels: 1 on, 0 off, -1 the library default")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -112,6 +114,9 @@ def main():
                          "written), to calibrate HBM byte counters")
     args = ap.parse_args()
 
+    if args.staging >= 0:
+        import gmlib_amd as G
+        G.lib().GmlxSetStaging(args.staging)
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is single-device (multi-GPU sharding is a design "
               "note only); run with --gpus 1", file=sys.stderr)

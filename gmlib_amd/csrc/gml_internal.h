@@ -127,6 +127,8 @@ struct SweepPlan {
    size_t bytes = 0;
 };
 
+struct ScratchBlock { void *p; size_t bytes; bool busy; };
+
 struct GmlCtx {
    int          dev = 0;
    hipStream_t  stream = nullptr;
@@ -162,6 +164,8 @@ struct GmlCtx {
    // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
    // generated and compiled but never launched
    bool         host_only = false;
+   // cached temporaries of the item kernels (gml_items.hip)
+   std::vector<ScratchBlock> iscratch;
 };
 
 // --- helpers ---------------------------------------------------------------
@@ -202,6 +206,9 @@ int  GpuDownlink(GmlCtx *g, int Src, int Dst, const std::string &nam);
 int  GpuShell(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int **off, int **ent, long long *nslab,
               int *maxdeg, int *firsthigh);
 int  GpuNeighbours(GmlCtx *g, int typ, int ngb, int idx, int rw);
+bool ItemAlloc(GmlCtx *g, void **p, size_t bytes);
+void ItemFree(GmlCtx *g, void *p);
+void FreeItemPool(GmlCtx *g, bool idle_only = false);
 
 // reductions (gml_reduce.hip)
 int  ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed);

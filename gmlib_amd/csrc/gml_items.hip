@@ -15,7 +15,9 @@
 // A stable LSD radix sort of the item indices by key puts equal items next
 // to each other, each run in visiting order (ascending element), and every
 // query becomes a run lookup:
-//   extraction  the runs' heads, selected and sorted by index;
+//   extraction  the runs' smallest positions, sorted (a block-local pass in
+//               LDS first keeps one copy per work-group of each item, with
+//               its smallest position, which shrinks the sort 2-4x);
 //   downlink    the last item of the run (the highest entity index, the
 //               head of the reference's chain);
 //   shell       the run reversed (descending element, the chain order);
@@ -26,6 +28,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <vector>
 
@@ -116,6 +119,35 @@ __device__ __forceinline__ void sort_nodes(int *a, int k)
       for(int j = i; j > 0 && a[j - 1] > a[j]; j--) { const int t = a[j]; a[j] = a[j - 1]; a[j - 1] = t; }
 }
 
+// key (lo, hi) of item j of element e; false for a node outside [0, nv)
+__device__ __forceinline__ bool item_key(const int *__restrict__ ele, long long e, int len, const LocalItems &li,
+                                         int j, const KeyForm &kf, long long nv, unsigned long long *lo,
+                                         unsigned long long *hi)
+{
+   int a[4] = {0, 0, 0, 0};
+   bool out = false;
+   for(int k = 0; k < kf.deg; k++)
+   {
+      a[k] = ele[e * len + li.nod[j][k]];
+      out |= a[k] < 0 || a[k] >= nv;
+   }
+   sort_nodes(a, kf.deg);
+   if(!kf.two)
+   {
+      unsigned long long k = 0;
+      for(int i = 0; i < kf.deg; i++) k = (k << kf.b) | (unsigned)a[i];
+      *lo = k;
+      *hi = 0;
+   }
+   else
+   {
+      const unsigned long long a0 = (unsigned)a[0], a1 = (unsigned)a[1], a2 = (unsigned)a[2], a3 = (unsigned)a[3];
+      if(kf.deg == 3) { *hi = a0; *lo = (a1 << 32) | a2; }
+      else            { *hi = (a0 << 32) | a1; *lo = (a2 << 32) | a3; }
+   }
+   return !out;
+}
+
 // key of item j of element e (index base + e * li.n + j); a node outside
 // [0, nv) raises *bad
 __global__ void k_item_keys(long long ne, int len, const int *__restrict__ ele, LocalItems li, KeyForm kf,
@@ -126,27 +158,10 @@ __global__ void k_item_keys(long long ne, int len, const int *__restrict__ ele, 
    if(t >= ne * li.n) return;
    const long long e = t / li.n;
    const int j = (int)(t - e * li.n);
-   int a[4] = {0, 0, 0, 0};
-   bool out = false;
-   for(int k = 0; k < kf.deg; k++)
-   {
-      a[k] = ele[e * len + li.nod[j][k]];
-      out |= a[k] < 0 || a[k] >= nv;
-   }
-   if(out) *bad = 1;
-   sort_nodes(a, kf.deg);
-   if(!kf.two)
-   {
-      unsigned long long k = 0;
-      for(int i = 0; i < kf.deg; i++) k = (k << kf.b) | (unsigned)a[i];
-      klo[t] = k;
-   }
-   else
-   {
-      const unsigned long long a0 = (unsigned)a[0], a1 = (unsigned)a[1], a2 = (unsigned)a[2], a3 = (unsigned)a[3];
-      if(kf.deg == 3) { khi[t] = a0; klo[t] = (a1 << 32) | a2; }
-      else            { khi[t] = (a0 << 32) | a1; klo[t] = (a2 << 32) | a3; }
-   }
+   unsigned long long lo, hi;
+   if(!item_key(ele, e, len, li, j, kf, nv, &lo, &hi)) *bad = 1;
+   klo[t] = lo;
+   if(kf.two) khi[t] = hi;
 }
 
 __global__ void k_iota(long long n, int *__restrict__ p)
@@ -189,59 +204,117 @@ __device__ long long bound(long long n, const unsigned long long *__restrict__ s
 // Items sorted by key: klo/khi by item index, slo/shi in sorted order, ord
 // the item index at each sorted position (khi, shi null in the one-word form)
 struct ItemSort {
+   GmlCtx *g;
    long long n = 0;
    KeyForm kf;
    unsigned long long *klo = nullptr, *khi = nullptr, *slo = nullptr, *shi = nullptr;
    int *ord = nullptr, *idx = nullptr;
    void *tmp = nullptr;
    size_t tbytes = 0;
+   explicit ItemSort(GmlCtx *c) : g(c) {}
    ~ItemSort()
    {
-      for(void *p : {(void *)klo, (void *)khi, (void *)slo, (void *)shi, (void *)ord, (void *)idx, tmp})
-         if(p) (void)hipFree(p);
+      for(void *p : {(void *)klo, (void *)khi, (void *)slo, (void *)shi, (void *)ord, (void *)idx, tmp}) ItemFree(g, p);
    }
 };
+
+template <typename T> bool DAlloc(GmlCtx *g, T **p, size_t bytes) { return ItemAlloc(g, (void **)p, bytes); }
 
 const unsigned kB = 256;
 dim3 blocks(long long n) { return dim3((unsigned)((n + kB - 1) / kB)); }
 bool ok_(hipError_t e) { return e == hipSuccess; }
 
+} // namespace
+
+// Temporaries of the item kernels (keys, sort buffers: several GB for a
+// 100M-tet mesh) are cached per handle: a freed block is kept and handed
+// out again to a later request it fits (best fit), so repeated extractions
+// and link builds map their memory once instead of at every call (mapping
+// and unmapping multi-GB buffers costs more than the kernels).  All users
+// run on the handle's stream, so a block freed by one kernel sequence and
+// reused by the next is ordered by the stream.  GmlStop releases the cache;
+// a failed allocation first releases the idle blocks and retries.
+bool ItemAlloc(GmlCtx *g, void **p, size_t bytes)
+{
+   bytes = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+   ScratchBlock *best = nullptr;
+   for(ScratchBlock &b : g->iscratch)
+      if(!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+   if(best)
+   {
+      best->busy = true;
+      *p = best->p;
+      return true;
+   }
+   void *q = nullptr;
+   if(hipMalloc(&q, bytes) != hipSuccess)
+   {
+      (void)hipGetLastError();
+      FreeItemPool(g, true);
+      if(hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); *p = nullptr; return false; }
+   }
+   g->iscratch.push_back({q, bytes, true});
+   *p = q;
+   return true;
+}
+
+void ItemFree(GmlCtx *g, void *p)
+{
+   if(!p) return;
+   for(ScratchBlock &b : g->iscratch)
+      if(b.p == p) { b.busy = false; return; }
+}
+
+// release the cached blocks (idle_only: those not handed out)
+void FreeItemPool(GmlCtx *g, bool idle_only)
+{
+   if(g->iscratch.empty()) return;
+   (void)hipStreamSynchronize(g->stream);
+   std::vector<ScratchBlock> keep;
+   for(ScratchBlock &b : g->iscratch)
+      if(idle_only && b.busy) keep.push_back(b);
+      else (void)hipFree(b.p);
+   g->iscratch.swap(keep);
+}
+
+namespace {
+
 // Keys of every item of the sources (in order) and their stable sort.
 // bad: device flag raised by a node out of range.  The temporary storage
 // also fits a select and a 32-bit key sort over n items.
-bool SortItems(hipStream_t s, const std::vector<Source> &src, const KeyForm &kf, long long nv, int *bad,
-               ItemSort *it)
+// Arrays of an item sort of n items; the temporary storage also fits a
+// select, a 32-bit key sort and a 32-bit pair sort over n items.
+bool AllocItems(GmlCtx *g, long long n, const KeyForm &kf, ItemSort *it)
 {
-   long long n = 0;
-   for(const Source &sr : src) n += sr.ne * sr.li.n;
+   hipStream_t s = g->stream;
    it->n = n;
    it->kf = kf;
    if(!n) return true;
-   size_t t1 = 0, t2 = 0, t3 = 0;
+   size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
    int *cnt = nullptr;
    unsigned char *flags = nullptr;
-   bool ok = ok_(hipMalloc(&it->klo, n * 8)) && ok_(hipMalloc(&it->slo, n * 8))
-          && (!kf.two || (ok_(hipMalloc(&it->khi, n * 8)) && ok_(hipMalloc(&it->shi, n * 8))))
-          && ok_(hipMalloc(&it->ord, n * 4)) && ok_(hipMalloc(&it->idx, n * 4))
+   bool ok = DAlloc(g, &it->klo, n * 8) && DAlloc(g, &it->slo, n * 8)
+          && (!kf.two || (DAlloc(g, &it->khi, n * 8) && DAlloc(g, &it->shi, n * 8)))
+          && DAlloc(g, &it->ord, n * 4) && DAlloc(g, &it->idx, n * 4)
           && ok_(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, it->klo, it->slo, it->idx, it->ord, (int)n, 0, 64, s))
           && ok_(hipcub::DeviceSelect::Flagged(nullptr, t2, it->ord, flags, it->idx, cnt, (int64_t)n, s))
-          && ok_(hipcub::DeviceRadixSort::SortKeys(nullptr, t3, it->ord, it->idx, (int)n, 0, 32, s));
+          && ok_(hipcub::DeviceRadixSort::SortKeys(nullptr, t3, it->ord, it->idx, (int)n, 0, 32, s))
+          && ok_(hipcub::DeviceRadixSort::SortPairs(nullptr, t4, it->ord, it->idx, it->ord, it->idx, (int)n, 0, 32, s));
    if(ok)
    {
-      it->tbytes = std::max(t1, std::max(t2, t3)) + 256;
-      ok = ok_(hipMalloc(&it->tmp, it->tbytes));
+      it->tbytes = std::max(std::max(t1, t2), std::max(t3, t4)) + 256;
+      ok = DAlloc(g, &it->tmp, it->tbytes);
    }
-   long long at = 0;
-   for(const Source &sr : src)
-   {
-      if(!ok) break;
-      const long long m = sr.ne * sr.li.n;
-      if(m)
-         hipLaunchKernelGGL(k_item_keys, blocks(m), dim3(kB), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv,
-                            it->klo + at, kf.two ? it->khi + at : nullptr, bad);
-      at += m;
-   }
-   if(ok)
+   return ok;
+}
+
+// stable sort of the it->n keys already in klo / khi
+bool SortPrepared(hipStream_t s, ItemSort *it)
+{
+   const long long n = it->n;
+   const KeyForm &kf = it->kf;
+   if(!n) return true;
+   bool ok = true;
    {
       hipLaunchKernelGGL(k_iota, blocks(n), dim3(kB), 0, s, n, it->idx);
       size_t t = it->tbytes;
@@ -262,22 +335,46 @@ bool SortItems(hipStream_t s, const std::vector<Source> &src, const KeyForm &kf,
    return ok && ok_(hipGetLastError());
 }
 
+// Keys of every item of the sources (in order) and their stable sort.
+// bad: device flag raised by a node out of range.
+bool SortItems(GmlCtx *g, const std::vector<Source> &src, const KeyForm &kf, long long nv, int *bad,
+               ItemSort *it)
+{
+   hipStream_t s = g->stream;
+   long long n = 0;
+   for(const Source &sr : src) n += sr.ne * sr.li.n;
+   if(!AllocItems(g, n, kf, it)) return false;
+   long long at = 0;
+   for(const Source &sr : src)
+   {
+      const long long m = sr.ne * sr.li.n;
+      if(m)
+         hipLaunchKernelGGL(k_item_keys, blocks(m), dim3(kB), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv,
+                            it->klo + at, kf.two ? it->khi + at : nullptr, bad);
+      at += m;
+   }
+   return SortPrepared(s, it);
+}
+
 // queries: keys of the items of the sources, unsorted
 struct QueryKeys {
+   GmlCtx *g;
    long long n = 0;
    unsigned long long *lo = nullptr, *hi = nullptr;
+   explicit QueryKeys(GmlCtx *c) : g(c) {}
    ~QueryKeys()
    {
-      if(lo) (void)hipFree(lo);
-      if(hi) (void)hipFree(hi);
+      ItemFree(g, lo);
+      ItemFree(g, hi);
    }
 };
 
-bool KeysOf(hipStream_t s, const Source &sr, const KeyForm &kf, long long nv, int *bad, QueryKeys *q)
+bool KeysOf(GmlCtx *g, const Source &sr, const KeyForm &kf, long long nv, int *bad, QueryKeys *q)
 {
+   hipStream_t s = g->stream;
    q->n = sr.ne * sr.li.n;
    if(!q->n) return true;
-   if(!ok_(hipMalloc(&q->lo, q->n * 8)) || (kf.two && !ok_(hipMalloc(&q->hi, q->n * 8)))) return false;
+   if(!DAlloc(g, &q->lo, q->n * 8) || (kf.two && !DAlloc(g, &q->hi, q->n * 8))) return false;
    hipLaunchKernelGGL(k_item_keys, blocks(q->n), dim3(kB), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, q->lo, q->hi,
                       bad);
    return ok_(hipGetLastError());
@@ -287,25 +384,117 @@ bool KeysOf(hipStream_t s, const Source &sr, const KeyForm &kf, long long nv, in
 // extraction
 // ---------------------------------------------------------------------------
 
-// heads of the runs of equal keys in sorted order
-__global__ void k_item_heads(long long n, const unsigned long long *__restrict__ slo,
-                             const unsigned long long *__restrict__ shi, unsigned char *__restrict__ head)
+// Block-local first appearances: a work-group takes kDedupThreads elements
+// of one source (one per thread), inserts their items' keys into an LDS
+// hash table that keeps each key's smallest position, and appends its
+// distinct items (key, position) to the output, reserved with one global
+// atomic.  Items of an element are inserted one local index at a time, in
+// rounds separated by barriers: an item claims an empty slot (compare-and-
+// swap on the slot's owner word) or, after the barrier, compares its key
+// with the slot's (written before the barrier by the slot's owner) and
+// either keeps the smaller position there or probes the next slot.  At most
+// HS x 3/4 items (HS = 2048 for up to 6 items per element, 4096 for up to
+// 12) go into HS slots, so a slot is always free and every round ends.
+constexpr int kDedupThreads = 256;
+
+__device__ __forceinline__ unsigned int dedup_slot(unsigned long long lo, unsigned long long hi)
+{
+   const unsigned long long h = (lo ^ (hi * 0xc2b2ae3d27d4eb4full)) * 0x9e3779b97f4a7c15ull;
+   return (unsigned int)(h >> 40);
+}
+
+template <bool TWO, int HS>
+__global__ void __launch_bounds__(kDedupThreads)
+k_item_dedup(long long ne, int len, const int *__restrict__ ele, LocalItems li, KeyForm kf, long long nv,
+             long long pos0, unsigned long long *__restrict__ olo, unsigned long long *__restrict__ ohi,
+             int *__restrict__ opos, unsigned long long *__restrict__ ocnt, int *__restrict__ bad)
+{
+   __shared__ unsigned long long hlo[HS];
+   __shared__ unsigned long long hhi[TWO ? HS : 1];
+   __shared__ int hpos[HS], hown[HS];
+   __shared__ unsigned int nuniq, nout;
+   __shared__ unsigned long long base;
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads) hown[i] = 0;
+   if(threadIdx.x == 0) nuniq = nout = 0;
+   __syncthreads();
+   const long long e = (long long)blockIdx.x * kDedupThreads + threadIdx.x;
+   const bool act = e < ne;
+   for(int j = 0; j < li.n; j++)
+   {
+      unsigned long long lo = 0, hi = 0;
+      if(act && !item_key(ele, e, len, li, j, kf, nv, &lo, &hi)) *bad = 1;
+      const int pos = act ? (int)(pos0 + e * li.n + j) : 0;
+      unsigned int sl = dedup_slot(lo, hi) & (HS - 1);
+      bool pend = act;
+      while(__syncthreads_or(pend))
+      {
+         bool check = false;
+         if(pend)
+         {
+            if(atomicCAS(&hown[sl], 0, 1) == 0)
+            {
+               hlo[sl] = lo;
+               if(TWO) hhi[sl] = hi;
+               hpos[sl] = pos;
+               atomicAdd(&nuniq, 1u);
+               pend = false;
+            }
+            else
+               check = true;
+         }
+         __syncthreads();
+         if(check)
+         {
+            if(hlo[sl] == lo && (!TWO || hhi[sl] == hi))
+            {
+               atomicMin(&hpos[sl], pos);
+               pend = false;
+            }
+            else
+               sl = (sl + 1) & (HS - 1);
+         }
+      }
+   }
+   __syncthreads();
+   if(threadIdx.x == 0) base = atomicAdd(ocnt, (unsigned long long)nuniq);
+   __syncthreads();
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads)
+      if(hown[i])
+      {
+         const unsigned long long k = base + atomicAdd(&nout, 1u);
+         olo[k] = hlo[i];
+         if(TWO) ohi[k] = hhi[i];
+         opos[k] = hpos[i];
+      }
+}
+
+// for each run of equal keys in sorted order (after the block-local pass a
+// key occurs at most once per work-group): its head gets flag 1 and the
+// run's smallest position
+__global__ void k_run_first(long long n, const unsigned long long *__restrict__ slo,
+                            const unsigned long long *__restrict__ shi, const int *__restrict__ ord,
+                            const int *__restrict__ pos, unsigned char *__restrict__ head, int *__restrict__ first)
 {
    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(i >= n) return;
-   head[i] = i == 0 || slo[i] != slo[i - 1] || (shi && shi[i] != shi[i - 1]);
+   const bool h = i == 0 || slo[i] != slo[i - 1] || (shi && shi[i] != shi[i - 1]);
+   head[i] = h;
+   if(!h) return;
+   int m = pos[ord[i]];
+   for(long long k = i + 1; k < n && slo[k] == slo[i] && (!shi || shi[k] == shi[i]); k++) m = min(m, pos[ord[k]]);
+   first[i] = m;
 }
 
-// row k of the new table: the nodes of item u[k] decoded from its key,
-// its ref (the existing table's ref for an existing item, else 0)
-__global__ void k_item_write(long long m, const int *__restrict__ u, KeyForm kf,
+// row k of the new table: the nodes of item u[k] decoded from its key, its
+// ref (the existing table's ref for a first appearance in it, else 0)
+__global__ void k_item_write(long long m, const int *__restrict__ u, const int *__restrict__ upos, KeyForm kf,
                              const unsigned long long *__restrict__ klo, const unsigned long long *__restrict__ khi,
                              long long nold, const int *__restrict__ oref, int w, int *__restrict__ tab,
                              int *__restrict__ ref)
 {
    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(k >= m) return;
-   const int i = u[k];
+   const int i = u[k], p = upos[k];
    int a[4] = {0, 0, 0, 0};
    if(!kf.two)
    {
@@ -320,7 +509,7 @@ __global__ void k_item_write(long long m, const int *__restrict__ u, KeyForm kf,
       else            { a[0] = (int)(hi >> 32); a[1] = (int)(unsigned)hi; a[2] = (int)(lo >> 32); a[3] = (int)(unsigned)lo; }
    }
    for(int j = 0; j < w; j++) tab[k * w + j] = j < kf.deg ? a[j] : 0;
-   ref[k] = (i < nold && oref) ? oref[i] : 0;
+   ref[k] = (p < nold && oref) ? oref[p] : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -438,41 +627,72 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    hipStream_t s = g->stream;
    const long long nv = g->NmbEle[GmlVertices];
    const KeyForm kf = MakeKeyForm(deg, nv);
-   ItemSort it;
-   int *cnt = nullptr, *uni = nullptr, *uni2 = nullptr;
+   auto clock = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+   double tim[5] = {clock(), 0, 0, 0, 0};
+   ItemSort it(g);
+   unsigned long long *cnt = nullptr, hc[2] = {0, 0};
+   int *pos = nullptr, *first = nullptr, *hsel = nullptr, *usel = nullptr, *p2 = nullptr, *u2 = nullptr, *nsel = nullptr;
    unsigned char *head = nullptr;
-   int rc = -1, hcnt[2] = {0, 0};
-   bool ok = ok_(hipMalloc(&cnt, 8)) && ok_(hipMemsetAsync(cnt, 0, 8, s))
-          && SortItems(s, src, kf, nv, cnt + 1, &it)
-          && ok_(hipMalloc(&uni, total * 4)) && ok_(hipMalloc(&uni2, total * 4)) && ok_(hipMalloc(&head, total));
-   // first appearances, in visiting order
-   if(ok)
+   int rc = -1, hbad = 0, m = 0;
+   // 1. block-local first appearances of every source's items
+   bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
+          && AllocItems(g, total, kf, &it) && DAlloc(g, &pos, total * 4);
+   tim[1] = clock();
+   long long at = 0;
+   for(const Source &sr : src)
    {
-      hipLaunchKernelGGL(k_item_heads, blocks(total), dim3(kB), 0, s, total, it.slo, it.shi, head);
-      size_t t = it.tbytes;
-      ok = ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, it.ord, head, uni, cnt, (int64_t)total, s))
-        && ok_(hipMemcpyAsync(hcnt, cnt, 8, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+      if(!ok) break;
+      const dim3 grid((unsigned)((sr.ne + kDedupThreads - 1) / kDedupThreads));
+      if(!sr.ne) continue;
+      const bool small = sr.li.n <= 6;
+      auto kern = kf.two ? (small ? k_item_dedup<true, 2048> : k_item_dedup<true, 4096>)
+                         : (small ? k_item_dedup<false, 2048> : k_item_dedup<false, 4096>);
+      hipLaunchKernelGGL(kern, grid, dim3(kDedupThreads), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo,
+                         kf.two ? it.khi : nullptr, pos, cnt, (int *)(cnt + 1));
+      at += sr.ne * sr.li.n;
    }
-   if(ok && hcnt[1])
+   ok = ok && ok_(hipGetLastError()) && ok_(hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, s))
+     && ok_(hipStreamSynchronize(s));
+   hbad = (int)(hc[1] & 0xffffffffu);
+   tim[2] = clock();
+   if(ok && hbad)
    {
       fprintf(stderr, "gml: extraction: an element references a vertex outside 0..%lld\n", nv - 1);
       ok = false;
    }
-   const long long m = hcnt[0];
+   // 2. sort the block-distinct items by key; each run's smallest position is
+   // the item's first appearance; those sorted by position are the table
+   const long long n2 = (long long)hc[0];
+   if(ok)
+   {
+      it.n = n2;
+      ok = SortPrepared(s, &it) && DAlloc(g, &head, n2) && DAlloc(g, &first, n2 * 4) && DAlloc(g, &hsel, n2 * 4)
+        && DAlloc(g, &usel, n2 * 4) && DAlloc(g, &p2, n2 * 4) && DAlloc(g, &u2, n2 * 4);
+   }
+   if(ok)
+   {
+      hipLaunchKernelGGL(k_run_first, blocks(n2), dim3(kB), 0, s, n2, it.slo, it.shi, it.ord, pos, head, first);
+      size_t t = it.tbytes;
+      ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, first, head, hsel, nsel, (int64_t)n2, s));
+      t = it.tbytes;
+      ok = ok && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, it.ord, head, usel, nsel, (int64_t)n2, s))
+        && ok_(hipMemcpyAsync(&m, nsel, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+   }
+   tim[3] = clock();
    if(ok)
    {
       size_t t = it.tbytes;
-      ok = ok_(hipcub::DeviceRadixSort::SortKeys(it.tmp, t, uni, uni2, (int)m, 0, bits_for(total), s));
+      ok = ok_(hipcub::DeviceRadixSort::SortPairs(it.tmp, t, hsel, p2, usel, u2, m, 0, bits_for(total), s));
    }
-   // the new table (created before the old one is dropped: the old refs are read)
+   // 3. the new table (created before the old one is dropped: the old refs are read)
    if(ok)
    {
-      const int nidx = GmlNewMeshData(GmlIdx, typ, (int)m);
+      const int nidx = GmlNewMeshData(GmlIdx, typ, m);
       if(!nidx) ok = false;
       else
       {
          const Data *d = &g->dat[nidx];
-         hipLaunchKernelGGL(k_item_write, blocks(m), dim3(kB), 0, s, m, uni2, kf, it.klo, it.khi, nold,
+         hipLaunchKernelGGL(k_item_write, blocks(m), dim3(kB), 0, s, (long long)m, u2, p2, kf, it.klo, it.khi, nold,
                             oref ? (const int *)g->dat[oref].dev : nullptr, d->ItmLen, (int *)d->dev,
                             (int *)g->dat[g->RefIdx[typ]].dev);
          ok = ok_(hipGetLastError()) && ok_(hipStreamSynchronize(s));
@@ -481,14 +701,19 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
             GmlFreeData(GmlIdx, old);
             if(oref && g->dat[oref].use) FreeDataSlot(g, oref);
          }
-         if(ok) rc = (int)m;
+         if(ok) rc = m;
       }
    }
    if(!ok) (void)hipGetLastError();
    (void)hipStreamSynchronize(s);
-   for(void *p : {(void *)uni, (void *)uni2, (void *)head, (void *)cnt})
-      if(p) (void)hipFree(p);
-   if(rc < 0 && !hcnt[1]) fprintf(stderr, "gml: GPU extraction of %lld items failed\n", total);
+   for(void *p : {(void *)cnt, (void *)nsel, (void *)pos, (void *)first, (void *)hsel, (void *)usel, (void *)p2,
+                  (void *)u2, (void *)head})
+      ItemFree(g, p);
+   if(rc < 0 && !hbad) fprintf(stderr, "gml: GPU extraction of %lld items failed\n", total);
+   tim[4] = clock();
+   DbgPrint(g, "Extraction of %lld items (%lld after the block pass, %d distinct): allocation %.1f ms, "
+            "block pass %.1f ms, sorts and selects %.1f ms, table and frees %.1f ms\n", total, n2, m,
+            1e3 * (tim[1] - tim[0]), 1e3 * (tim[2] - tim[1]), 1e3 * (tim[3] - tim[2]), 1e3 * (tim[4] - tim[3]));
    return rc;
 }
 
@@ -518,18 +743,18 @@ int GpuDownlink(GmlCtx *g, int Src, int Dst, const std::string &nam)
    if(!AllocData(g, d)) return 0;
    hipStream_t s = g->stream;
    int *bad = nullptr, hbad = 0;
-   bool ok = ok_(hipMalloc(&bad, 4)) && ok_(hipMemsetAsync(bad, 0, 4, s));
+   bool ok = DAlloc(g, &bad, 4) && ok_(hipMemsetAsync(bad, 0, 4, s));
    {
-      ItemSort it;
-      QueryKeys q;
-      ok = ok && SortItems(s, {dsrc}, kf, nv, bad, &it) && KeysOf(s, ssrc, kf, nv, bad, &q);
+      ItemSort it(g);
+      QueryKeys q(g);
+      ok = ok && SortItems(g, {dsrc}, kf, nv, bad, &it) && KeysOf(g, ssrc, kf, nv, bad, &q);
       if(ok && q.n && it.n)
          hipLaunchKernelGGL(k_down_lookup, blocks(q.n), dim3(kB), 0, s, q.n, ssrc.li.n, ssrc.li, q.lo, q.hi, it.n,
                             it.slo, it.shi, it.ord, rw, (int *)d->dev);
       ok = ok && ok_(hipGetLastError()) && ok_(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s))
         && ok_(hipStreamSynchronize(s));
    }
-   if(bad) (void)hipFree(bad);
+   ItemFree(g, bad);
    if(!ok || hbad)
    {
       (void)hipGetLastError();
@@ -559,12 +784,12 @@ int GpuShell(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int **off, int *
    int *bad = nullptr, *beg = nullptr, hbad = 0;
    *off = nullptr;
    *ent = nullptr;
-   bool ok = ok_(hipMalloc(&bad, 4)) && ok_(hipMemsetAsync(bad, 0, 4, s)) && ok_(hipMalloc(&beg, ssrc.ne * 4 + 4))
+   bool ok = DAlloc(g, &bad, 4) && ok_(hipMemsetAsync(bad, 0, 4, s)) && DAlloc(g, &beg, ssrc.ne * 4 + 4)
           && ok_(hipMemsetAsync(degdev, 0, ssrc.ne * 4 + 4, s));
    {
-      ItemSort it;
-      QueryKeys q;
-      ok = ok && SortItems(s, {dsrc}, kf, nv, bad, &it) && KeysOf(s, ssrc, kf, nv, bad, &q);
+      ItemSort it(g);
+      QueryKeys q(g);
+      ok = ok && SortItems(g, {dsrc}, kf, nv, bad, &it) && KeysOf(g, ssrc, kf, nv, bad, &q);
       if(ok && q.n && it.n)
          hipLaunchKernelGGL(k_shell_runs, blocks(q.n), dim3(kB), 0, s, q.n, q.lo, q.hi, it.n, it.slo, it.shi, beg,
                             degdev);
@@ -576,8 +801,8 @@ int GpuShell(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int **off, int *
                             *off, *ent);
       ok = ok && ok_(hipGetLastError()) && ok_(hipStreamSynchronize(s));
    }
-   if(bad) (void)hipFree(bad);
-   if(beg) (void)hipFree(beg);
+   ItemFree(g, bad);
+   ItemFree(g, beg);
    if(!ok)
    {
       (void)hipGetLastError();
@@ -605,17 +830,17 @@ int GpuNeighbours(GmlCtx *g, int typ, int ngb, int idx, int rw)
    if(src.ne * src.li.n >= INT_MAX) return 0;
    hipStream_t s = g->stream;
    int *bad = nullptr, hbad = 0;
-   bool ok = ok_(hipMalloc(&bad, 4)) && ok_(hipMemsetAsync(bad, 0, 4, s));
+   bool ok = DAlloc(g, &bad, 4) && ok_(hipMemsetAsync(bad, 0, 4, s));
    {
-      ItemSort it;
-      ok = ok && SortItems(s, {src}, kf, nv, bad, &it);
+      ItemSort it(g);
+      ok = ok && SortItems(g, {src}, kf, nv, bad, &it);
       if(ok && it.n)
          hipLaunchKernelGGL(k_neighbours, blocks(it.n), dim3(kB), 0, s, it.n, it.slo, it.shi, it.ord, src.li.n, src.li,
                             rw, (int *)g->dat[idx].dev);
       ok = ok && ok_(hipGetLastError()) && ok_(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s))
         && ok_(hipStreamSynchronize(s));
    }
-   if(bad) (void)hipFree(bad);
+   ItemFree(g, bad);
    if(!ok || hbad)
    {
       (void)hipGetLastError();

@@ -321,6 +321,7 @@ void GmlStop(size_t GmlIdx)
    (void)hipFree(g->red_scratch);
    (void)hipHostFree(g->red_host);
    (void)hipFree(g->dummy_par);
+   FreeItemPool(g);
    (void)hipStreamDestroy(g->stream);
    delete g;
 }
