@@ -950,6 +950,10 @@ def run_heat(args):
     log("heat: %.3f ms per iteration (%.1f GB/s of the byte model); kernels (us): %s; residual %.3e -> %.3e"
         % (dt * 1e3, alg / dt / 1e9, {k: round(v * 1e6, 1) for k, v in kt.items()}, res[0], res[-1]))
     kgbs = kb[dom] / kt[dom] / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_heat128.json")
+    if n == 128 and os.path.exists(tf):
+        traffic = json.load(open(tf)).get(dom)
     out = {
         "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
         "value": round(alg / dt / 1e9, 2),
@@ -969,7 +973,7 @@ def run_heat(args):
                    "kernel_us": {k: round(v * 1e6, 2) for k, v in kt.items()}},
         "iterations_per_s": round(1 / dt, 1),
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "bytes_per_launch": kb[dom], "avg_us": round(kt[dom] * 1e6, 2)},
         "cpu_baseline": None,
     }
