@@ -107,6 +107,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: run oracle sweeps for about this long (0: skip)")
     ap.add_argument("--sweep-tile", type=int, default=256, help="vertices per tile of the fused sweep")
+    ap.add_argument("--quality-shape", default="", help="quality kernel launch: group/tets per thread (e.g. 256/2)")
     ap.add_argument("--paths", default="generated,builtin")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="launch a plain copy-like kernel of known size this many times first (smoothing: "
@@ -203,11 +204,23 @@ def run_smoothing(args):
                 for w in (G.GmlxBuiltinSweep, G.GmlxBuiltinQuality)]
 
     L.GmlxSetBuiltinShape(G.GmlxBuiltinSweep, args.sweep_tile, 1)
+    if args.quality_shape:
+        qg, qp = (int(x) for x in args.quality_shape.split("/"))
+        if not (L.GmlxSetBuiltinShape(G.GmlxBuiltinQuality, qg, 1) and L.GmlxSetBuiltinItems(G.GmlxBuiltinQuality, qp)):
+            raise SystemExit("invalid --quality-shape %s" % args.quality_shape)
     paths = {}
     for path, step in (("generated", step_generated), ("builtin", step_builtin)):
         if path not in args.paths.split(","):
             continue
         for _ in range(args.warmup):
+            qmin = step()
+        if path == "generated":
+            # the launch-shape tuner of a generated kernel tries 11 shapes on
+            # its first launches: finish it for the quality kernel too, which a
+            # step launches once, so that the timed steps are steady-state
+            while L.GmlxGetKernelRows(h, kq) == 0:
+                if L.GmlLaunchKernel(h, kq) != 1:
+                    raise RuntimeError("launch failed")
             qmin = step()
         sg = ktimes(path)
         L.GmlxSync(h)

@@ -83,13 +83,16 @@ _SIGS = {
     "GmlxInitHostOnly": (_S, []),
     "GmlxCompileKernelv": (_I, [_S, _CP, _CP, _I, _I, _IP]),
     "GmlxGetKernelSource": (_CP, [_S, _I]),
+    "GmlxGetKernelPairSource": (_CP, [_S, _I]),
+    "GmlxGetKernelRows": (_I, [_S, _I]),
+    "GmlxSetKernelShape": (_I, [_S, _I, _I, _I, _I]),
     "GmlxGetKernelShape": (_I, [_S, _I, _IP, _IP]),
-    "GmlxSetKernelShape": (_I, [_S, _I, _I, _I]),
     "GmlxGetDevicePointer": (_P, [_S, _I, C.POINTER(C.c_size_t)]),
     "GmlxSync": (_I, [_S]),
     "GmlxGetStream": (_P, [_S]),
     "GmlxInvalidateHost": (_I, [_S, _I]),
     "GmlxBuildLink": (_I, [_S, _I, _I]),
+    "GmlxSetBuiltinItems": (_I, [_I, _I]),
     "GmlxGetLinkData": (_I, [_S, _I, _I]),
     "GmlxGetUplinkTable": (_I, [_S, _I, _I, _P, _P, _I]),
     "GmlxCubeMesh": (_I, [_I, _D, C.c_ulonglong, _I, _P, _P, _P]),

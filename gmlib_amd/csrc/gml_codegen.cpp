@@ -257,10 +257,64 @@ static bool BodyUses(const char *body, const std::string &id)
    return false;
 }
 
+// A kernel may also be emitted in paired form (pair = true): each thread
+// takes two rows, cnt and cnt + blockDim.x, with the reads of both rows
+// issued before either body runs, so that twice as many loads are in flight
+// per thread.  The row's locals become the members of a local struct; the
+// reads, the body and the writes are lambdas that bind the members under
+// their usual names, so the emitted reads, the user's body and the writes
+// are the same text as in the one-row form.  A body that returns early
+// skips its row's writes, as in the one-row form.  In the last work-group
+// a thread without a second row runs the body twice on its first row and
+// writes once, so bodies that write the parameter block or use atomics keep
+// the one-row form.
+// true if the body assigns a field of the parameter block (GmlPar->x = ...,
+// GmlPar->x[i] += ..., ++GmlPar->x, ...): such a body must run once per row
+static bool BodyWritesParams(const char *body)
+{
+   for(const char *p = strstr(body, "GmlPar->"); p; p = strstr(p + 1, "GmlPar->"))
+   {
+      const char *b = p;
+      while(b > body && (b[-1] == ' ' || b[-1] == '\t')) b--;
+      if(b - body >= 2 && ((b[-1] == '+' && b[-2] == '+') || (b[-1] == '-' && b[-2] == '-'))) return true;
+      const char *q = p + 8;
+      while(IsIdent(*q)) q++;
+      while(*q == '[' || *q == ' ' || *q == '\t' || *q == '.')
+      {
+         if(*q == '[') { int depth = 1; q++; while(*q && depth) { depth += (*q == '[') - (*q == ']'); q++; } }
+         else if(*q == '.') { q++; while(IsIdent(*q)) q++; }
+         else q++;
+      }
+      if((q[0] == '+' && q[1] == '+') || (q[0] == '-' && q[1] == '-')) return true;
+      if(q[0] == '=' && q[1] != '=') return true;
+      if(strchr("+-*/%&|^", q[0]) && q[0] && q[1] == '=') return true;
+      if((q[0] == '<' && q[1] == '<' && q[2] == '=') || (q[0] == '>' && q[1] == '>' && q[2] == '=')) return true;
+   }
+   return false;
+}
+
+static bool PairEligible(const std::vector<Arg> &args, const char *KrnSrc, bool lazy, bool stage)
+{
+   if(lazy || stage || BodyWritesParams(KrnSrc) || strstr(KrnSrc, "atomic")) return false;
+   for(const Arg &a : args)
+      if(a.kind == AkBall || a.CntIdx >= 0 || (a.FlgTab & GmlManual)) return false;
+   for(const char *id : {"barrier", "get_global_id", "get_local_id", "get_group_id", "get_local_size",
+                         "get_global_size", "get_num_groups", "__syncthreads", "gml_gid"})
+      if(BodyUses(KrnSrc, id)) return false;
+   return true;
+}
+
 static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam, int MshTyp,
-                              const std::vector<Arg> &args, const char *ParSrc, bool lazy, bool stage)
+                              const std::vector<Arg> &args, const char *ParSrc, bool lazy, bool stage,
+                              bool pair = false)
 {
    std::string s;
+   struct Local { std::string type, name, dims; };
+   std::vector<Local> locals;
+   auto local = [&](const std::string &t, const std::string &n, const std::string &d) {
+      locals.push_back({t, n, d});
+      return Fmt("   %s %s%s;\n", t.c_str(), n.c_str(), d.c_str());
+   };
    s.reserve(16384);
    std::vector<int> staged;
    const int sball = stage ? StagedArgs(args, &staged) : -1;
@@ -310,7 +364,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    s += "\n   GmlParSct *GmlPar,\n   const int2 count,\n   const int gml_remap)\n{\n";
 
    // --- locals (gmlib3.c:2670-2732)
-   s += "// KERNEL VARIABLES DEFINITION\n";
+   std::string decl, defs;
    for(size_t i = 0; i < args.size(); i++)
    {
       const Arg &a = args[i];
@@ -321,39 +375,46 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       bool lz = lazy && C && a.NmbItm > 1;
       if(a.kind == AkBall)
       {
-         if((int)i == sball) s += Fmt("   unsigned short gml_l_%s[%d];\n", a.nam.c_str(), a.NmbItm * a.ItmLen);
-         s += Fmt("   int gml_%s[%d];\n", a.nam.c_str(), a.NmbItm * a.ItmLen);  // shifted entries, one per slot
-         if(a.NmbItm > 1) s += Fmt("   int16 %s[%d];\n", a.nam.c_str(), a.NmbItm);
-         else             s += Fmt("   %s %s;\n", OclTypStr[a.ItmTyp], a.nam.c_str());
+         if((int)i == sball) decl += local("unsigned short", "gml_l_" + a.nam, Fmt("[%d]", a.NmbItm * a.ItmLen));
+         decl += local("int", "gml_" + a.nam, Fmt("[%d]", a.NmbItm * a.ItmLen));  // shifted entries, one per slot
+         if(a.NmbItm > 1) decl += local("int16", a.nam, Fmt("[%d]", a.NmbItm));
+         else             decl += local(OclTypStr[a.ItmTyp], a.nam, "");
       }
       else if(lz)
          ;  // declared after the reads as an accessor
       else if(L && L->MaxDeg > 1)
       {
-         if(a.NmbItm > 1) s += Fmt("   %s %s[%d][%d];\n", T, a.nam.c_str(), L->MaxDeg, a.NmbItm);
-         else             s += Fmt("   %s %s[%d];\n", T, a.nam.c_str(), L->MaxDeg);
+         if(a.NmbItm > 1) decl += local(T, a.nam, Fmt("[%d][%d]", L->MaxDeg, a.NmbItm));
+         else             decl += local(T, a.nam, Fmt("[%d]", L->MaxDeg));
       }
       else
       {
-         if(a.NmbItm > 1) s += Fmt("   %s %s[%d];\n", T, a.nam.c_str(), a.NmbItm);
-         else             s += Fmt("   %s %s;\n", T, a.nam.c_str());
+         if(a.NmbItm > 1) decl += local(T, a.nam, Fmt("[%d]", a.NmbItm));
+         else             decl += local(T, a.nam, "");
       }
       if(C)
       {
-         s += Fmt("   %s %sDeg;\n", OclTypStr[C->ItmTyp], a.nam.c_str());
-         s += Fmt("   %s %sNul;\n", T, a.nam.c_str());
-         s += Fmt("   #define   %sDegMax %d\n", a.nam.c_str(), L->MaxDeg);
+         decl += local(OclTypStr[C->ItmTyp], a.nam + "Deg", "");
+         decl += local(T, a.nam + "Nul", "");
+         const std::string d = Fmt("   #define   %sDegMax %d\n", a.nam.c_str(), L->MaxDeg);
+         decl += d;
+         defs += d;
       }
       else if(L && L->LnkDir == 0)
-         s += Fmt("   %s %sNul;\n", T, a.nam.c_str());
+         decl += local(T, a.nam + "Nul", "");
       if(a.FlgTab & GmlVoyeurs)
-         s += Fmt("   char      %s[%d];\n", a.VoyNam.c_str(), a.NmbItm * a.ItmLen);
+         decl += local("char     ", a.VoyNam, Fmt("[%d]", a.NmbItm * a.ItmLen));
    }
 
    // --- counters (gmlib3.c:2739-2748)
-   s += "// KERNEL COUNTERS\n";
-   s += "   int       cnt = gml_gid(gml_remap);\n";
-   s += Fmt("   int       %sIdx = cnt + count.s1;\n\n", BalTypStr[MshTyp]);
+   const std::string idxnam = std::string(BalTypStr[MshTyp]) + "Idx";
+   if(!pair)
+   {
+      s += "// KERNEL VARIABLES DEFINITION\n" + decl;
+      s += "// KERNEL COUNTERS\n";
+      s += "   int       cnt = gml_gid(gml_remap);\n";
+      s += Fmt("   int       %s = cnt + count.s1;\n\n", idxnam.c_str());
+   }
    if(sball >= 0)
    {
       s += "// STAGED UPLINK DATA: the work-group's tile, coalesced loads into LDS\n"
@@ -375,9 +436,11 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
          s += Fmt("         gml_s_%s[gml_i] = %sTab[gml_x];\n", args[i].nam.c_str(), args[i].nam.c_str());
       s += "      }\n      __syncthreads();\n   }\n\n";
    }
-   s += "   if(cnt >= count.s0)\n      return;\n\n";
+   if(!pair) s += "   if(cnt >= count.s0)\n      return;\n\n";
 
-   // --- reads (gmlib3.c:2755-2885)
+   // --- reads (gmlib3.c:2755-2885), emitted into their own string
+   std::string body_s;
+   std::swap(s, body_s);
    s += "// KERNEL MEMORY READINGS\n";
    for(size_t i = 0; i < args.size(); i++)
    {
@@ -482,12 +545,16 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       s += "\n";
    }
 
+   std::string reads;
+   std::swap(s, reads);
+   s = body_s;
+
    // --- body
-   s += "// USER'S KERNEL CODE\n";
-   s += SinglePrecisionLiterals(KrnSrc);
-   s += "\n";
+   const std::string user = "// USER'S KERNEL CODE\n" + SinglePrecisionLiterals(KrnSrc) + "\n";
 
    // --- writes (gmlib3.c:2903-2938): only the loop entity's own line
+   std::string writes_s;
+   std::swap(s, writes_s);
    s += "\n// KERNEL MEMORY WRITINGS\n";
    for(auto &a : args)
    {
@@ -499,7 +566,44 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
             s += Fmt("   __builtin_nontemporal_store(%s[%d], &%sTab[ cnt + count.s1 ][%d]);\n", a.nam.c_str(), c,
                      a.nam.c_str(), c);
    }
-   s += "}\n";
+   std::string writes;
+   std::swap(s, writes);
+   s = writes_s;
+   if(!pair)
+   {
+      s += reads + user + writes + "}\n";
+      return s;
+   }
+
+   // paired form: the row struct, the read / body / write lambdas, two rows
+   std::string bind = "      int &cnt = gml_x.cnt;\n      int &" + idxnam + " = gml_x." + idxnam + ";\n";
+   s += "// ROW VARIABLES (two rows per thread)\n" + defs + "   struct gml_row_t {\n";
+   for(const Local &l : locals)
+   {
+      s += Fmt("      %s %s%s;\n", l.type.c_str(), l.name.c_str(), l.dims.c_str());
+      if(l.dims.empty()) bind += Fmt("      %s &%s = gml_x.%s;\n", l.type.c_str(), l.name.c_str(), l.name.c_str());
+      else bind += Fmt("      %s (&%s)%s = gml_x.%s;\n", l.type.c_str(), l.name.c_str(), l.dims.c_str(), l.name.c_str());
+   }
+   s += "      int cnt, " + idxnam + ";\n      bool gml_done;\n   };\n";
+   s += "   auto gml_read = [&](gml_row_t &gml_x) {\n" + bind + reads + "   };\n";
+   s += "   auto gml_body = [&](gml_row_t &gml_x) {\n" + bind + "      gml_x.gml_done = false;\n" + user +
+        "      gml_x.gml_done = true;\n   };\n";
+   s += "   auto gml_write = [&](gml_row_t &gml_x) {\n" + bind + writes + "   };\n";
+   s += "// KERNEL COUNTERS\n"
+        "   const int gml_c0 = gml_gid2(gml_remap);\n"
+        "   if(gml_c0 >= count.s0)\n      return;\n"
+        "   const bool gml_has1 = gml_c0 + (int)blockDim.x < count.s0;\n"
+        "   gml_row_t gml_r0, gml_r1;\n"
+        "   gml_r0.cnt = gml_c0;\n"
+        "   gml_r1.cnt = gml_has1 ? gml_c0 + (int)blockDim.x : gml_c0;\n";
+   s += "   gml_r0." + idxnam + " = gml_r0.cnt + count.s1;\n   gml_r1." + idxnam + " = gml_r1.cnt + count.s1;\n";
+   // both bodies run (the second on a repeat of the first row at the end of
+   // the range): a conditional second body would let the compiler sink the
+   // second row's loads below the first body
+   s += "   gml_read(gml_r0);\n   gml_read(gml_r1);\n"
+        "   gml_body(gml_r0);\n   gml_body(gml_r1);\n"
+        "   if(gml_r0.gml_done) gml_write(gml_r0);\n"
+        "   if(gml_has1 && gml_r1.gml_done) gml_write(gml_r1);\n}\n";
    return s;
 }
 
@@ -546,8 +650,14 @@ static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std:
 // ---------------------------------------------------------------------------
 // Launch
 // ---------------------------------------------------------------------------
-static const int kShapes[][2] = {{256, 1}, {128, 1}, {64, 1}, {256, 0}, {128, 0}, {64, 0}, {256, 1}};
-static const int kNmbShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+// launch shapes the tuner tries: {group size, XCD remap, two rows per
+// thread}; the paired shapes only for kernels with a paired form.  The first
+// shape is launched again last, once code and caches are warm.
+static const int kShapes[][3] = {{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {256, 0, 0}, {128, 0, 0}, {64, 0, 0},
+                                 {256, 1, 1}, {128, 1, 1}, {256, 0, 1}, {128, 0, 1}};
+static const int kNmbOne = 6, kNmbAll = 10;
+static int NmbShapes(const Kernel *k) { return (k->fn2 || !k->src2.empty() ? kNmbAll : kNmbOne) + 1; }
+static const int *ShapeAt(const Kernel *k, int i) { return kShapes[i < NmbShapes(k) - 1 ? i : 0]; }
 
 void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
 {
@@ -598,11 +708,13 @@ int RunKernel(GmlCtx *g, Kernel *k)
    if(n <= 0) return 1;
    int count[2] = {(int)k->NmbLin[0], (int)k->NmbLin[1]};
 
-   int grp = k->grp, remap = k->remap, tuning = 0;
+   int grp = k->grp, remap = k->remap, pair = k->pair, tuning = 0;
    if(!k->tuned)
    {
-      grp = kShapes[k->tune_step][0];
-      remap = kShapes[k->tune_step][1];
+      const int *sh = ShapeAt(k, k->tune_step);
+      grp = sh[0];
+      remap = sh[1];
+      pair = sh[2] && k->fn2;
       tuning = 1;
    }
    // staged uplink data: the tile plan of this work-group size; the tuner
@@ -664,8 +776,9 @@ int RunKernel(GmlCtx *g, Kernel *k)
    bool tun = tuning && NewRec(&tr);
    if(have) (void)hipEventRecord(rec.beg, g->stream);
    if(tun) (void)hipEventRecord(tr.beg, g->stream);
-   unsigned int nb = (unsigned int)((n + grp - 1) / grp);
-   hipError_t e = hipModuleLaunchKernel(k->fn, nb, 1, 1, grp, 1, 1, lds, g->stream, kp.data(), nullptr);
+   const long long span = pair ? 2LL * grp : grp;
+   unsigned int nb = (unsigned int)((n + span - 1) / span);
+   hipError_t e = hipModuleLaunchKernel(pair ? k->fn2 : k->fn, nb, 1, 1, grp, 1, 1, lds, g->stream, kp.data(), nullptr);
    if(e != hipSuccess)
    {
       (void)hipGetLastError();
@@ -686,8 +799,9 @@ int RunKernel(GmlCtx *g, Kernel *k)
    {
       if(tun) k->tune_ev.push_back(tr);
       k->tune_step++;
-      if(k->tune_step == kNmbShapes)
+      if(k->tune_step == NmbShapes(k))
       {
+         const int kNmbShapes = NmbShapes(k);
          // every shape launched once: keep the fastest.  The first shape is
          // launched again last, once code and caches are warm, and only that
          // second timing of it counts.
@@ -703,8 +817,10 @@ int RunKernel(GmlCtx *g, Kernel *k)
          (void)hipGetLastError();
          for(auto &t : k->tune_ev) { (void)hipEventDestroy(t.beg); (void)hipEventDestroy(t.end); }
          k->tune_ev.clear();
-         k->grp = k->stage_src >= 0 ? k->stage_grp : kShapes[bi][0];
-         k->remap = kShapes[bi][1];
+         const int *sh = ShapeAt(k, bi);
+         k->grp = k->stage_src >= 0 ? k->stage_grp : sh[0];
+         k->remap = sh[1];
+         k->pair = sh[2] && k->fn2;
          k->tuned = 1;
       }
    }
@@ -911,6 +1027,30 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
    DbgPrint(g, "Generated source for kernel=%s, index=%2d\n%s\n", PrcNam, KrnIdx, g->DbgFlg ? src.c_str() : "");
 
    Kernel *k = &g->krn[KrnIdx];
+   // the paired form (two rows per thread) of a kernel without uplinks; the
+   // launch tuner chooses between the two forms
+   if(HghSrc < 0 && k->stage_src < 0 && PairEligible(args, KrnSrc, false, false)
+   && k->src.find("gml_upn<") == std::string::npos)
+   {
+      std::string psrc = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false, false, true), code, plog;
+      if(RtcCompile(psrc, PrcNam, g->cflags, &code, &plog))
+      {
+         k->src2 = psrc;
+         if(!g->host_only
+         && (hipModuleLoadData(&k->mod2, code.data()) != hipSuccess
+             || hipModuleGetFunction(&k->fn2, k->mod2, PrcNam) != hipSuccess))
+         {
+            (void)hipGetLastError();
+            if(k->mod2) (void)hipModuleUnload(k->mod2);
+            k->mod2 = nullptr;
+            k->fn2 = nullptr;
+            k->src2.clear();
+         }
+      }
+      else
+         DbgPrint(g, "Kernel %s: the paired form does not compile, the one-row form is kept\n%s\n", PrcNam,
+                  plog.c_str());
+   }
    for(auto &a : args)
    {
       k->args.push_back(a.DatIdx);
@@ -1040,6 +1180,45 @@ const char *GmlxGetKernelSource(size_t GmlIdx, int idx)
    return g->krn[idx].src.c_str();
 }
 
+int GmlxGetKernelRows(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return -1;
+   const Kernel *k = &g->krn[idx];
+   if(k->src2.empty()) return 1;
+   return k->tuned ? (k->pair ? 2 : 1) : 0;
+}
+
+int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
+   Kernel *k = &g->krn[idx];
+   if(GrpSiz < 64 || GrpSiz > 256 || (GrpSiz & 63) || (Rows != 1 && Rows != 2)) return 0;
+   if(Rows == 2 && !k->fn2 && (g->host_only ? k->src2.empty() : true)) return 0;
+   if(k->stage_src >= 0 && GrpSiz != 64 && GrpSiz != 128 && GrpSiz != 256) return 0;
+   for(int i : {idx, k->HghIdx})
+   {
+      if(!i) continue;
+      Kernel *t = &g->krn[i];
+      for(auto &e : t->tune_ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+      t->tune_ev.clear();
+      t->grp = GrpSiz;
+      if(t->stage_src >= 0) t->stage_grp = GrpSiz;
+      t->remap = Remap ? 1 : 0;
+      t->pair = Rows == 2 && (t->fn2 || !t->src2.empty());
+      t->tuned = 1;
+   }
+   return 1;
+}
+
+const char *GmlxGetKernelPairSource(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use || g->krn[idx].src2.empty()) return nullptr;
+   return g->krn[idx].src2.c_str();
+}
+
 int GmlxGetKernelShape(size_t GmlIdx, int idx, int *GrpSiz, int *Remap)
 {
    GML_CTX(g, GmlIdx);
@@ -1050,21 +1229,5 @@ int GmlxGetKernelShape(size_t GmlIdx, int idx, int *GrpSiz, int *Remap)
    return 1;
 }
 
-int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap)
-{
-   GML_CTX(g, GmlIdx);
-   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
-   if(GrpSiz < 64 || GrpSiz > 256 || (GrpSiz & 63)) return 0;
-   for(int i : {idx, g->krn[idx].HghIdx})
-   {
-      if(!i) continue;
-      Kernel *k = &g->krn[i];
-      k->grp = GrpSiz;
-      k->remap = Remap ? 1 : 0;
-      k->tuned = 1;
-      k->tune_ev.clear();
-   }
-   return 1;
-}
 
 } // extern "C"

@@ -90,8 +90,12 @@ struct Kernel {
    long long     NmbLin[2] = {0, 0}; // {count, offset}
    int           HghIdx = 0;
    bool          has_par = true;
-   // launch shape autotuning (group size x XCD remap)
-   int           grp = 256, remap = 1, tuned = 0, tune_step = 0;
+   // launch shape autotuning (group size x XCD remap x rows per thread)
+   int           grp = 256, remap = 1, tuned = 0, tune_step = 0, pair = 0;
+   // paired form (two rows per thread), when the kernel has one
+   hipModule_t   mod2 = nullptr;
+   hipFunction_t fn2 = nullptr;
+   std::string   src2;
    // LDS-staged uplink data (stage_src >= 0): the uplink, the byte size of
    // each staged item, the work-group size its plan was built for
    int           stage_src = -1, stage_dst = -1, stage_grp = 0;
@@ -222,7 +226,7 @@ void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
 // launch shape of a built-in kernel (work-group size, XCD block remap),
 // process-wide, set by GmlxSetBuiltinShape
 const int kNmbBuiltin = 16;
-struct Shape { int grp = 256, remap = 1; };
+struct Shape { int grp = 256, remap = 1, per = 2; };   // per: items per thread where a kernel takes several
 Shape BuiltinShape(int which);
 
 // Block index of this work-group in launch order, or, with remap, remapped

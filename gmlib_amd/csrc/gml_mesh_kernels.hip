@@ -48,20 +48,30 @@ void FreeBuiltinStats(GmlCtx *g)
       }
 }
 
-// two tets per thread (a work-group covers 2 x its size, each half
-// coalesced): eight coordinate gathers in flight per thread
+// TPT tets per thread (a work-group covers TPT x its size, each part
+// coalesced): 4 TPT coordinate gathers in flight per thread; a missing tet
+// of the last group repeats the thread's first one and is not stored
+template <int TPT>
 __global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *__restrict__ tet,
                                                        const v4f *__restrict__ crd, float *__restrict__ q, int remap)
 {
-   const long long t0 = (long long)xcd_block(remap) * 2 * blockDim.x + threadIdx.x, t1 = t0 + blockDim.x;
+   const long long t0 = (long long)xcd_block(remap) * TPT * blockDim.x + threadIdx.x;
    if(t0 >= nt) return;
-   const bool two = t1 < nt;
-   const v4i a = ld_nt(&tet[t0]);
-   const v4i b = two ? ld_nt(&tet[t1]) : a;
-   const v4f a0 = crd[a.x], a1 = crd[a.y], a2 = crd[a.z], a3 = crd[a.w];
-   const v4f b0 = crd[b.x], b1 = crd[b.y], b2 = crd[b.z], b3 = crd[b.w];
-   __builtin_nontemporal_store(meanratio3(a0, a1, a2, a3), &q[t0]);
-   if(two) __builtin_nontemporal_store(meanratio3(b0, b1, b2, b3), &q[t1]);
+   v4i e[TPT];
+   e[0] = ld_nt(&tet[t0]);
+#pragma unroll
+   for(int u = 1; u < TPT; u++)
+      e[u] = t0 + u * blockDim.x < nt ? ld_nt(&tet[t0 + u * blockDim.x]) : e[0];
+   v4f c[TPT][4];
+#pragma unroll
+   for(int u = 0; u < TPT; u++)
+   {
+      c[u][0] = crd[e[u].x]; c[u][1] = crd[e[u].y]; c[u][2] = crd[e[u].z]; c[u][3] = crd[e[u].w];
+   }
+#pragma unroll
+   for(int u = 0; u < TPT; u++)
+      if(t0 + u * blockDim.x < nt)
+         __builtin_nontemporal_store(meanratio3(c[u][0], c[u][1], c[u][2], c[u][3]), &q[t0 + u * blockDim.x]);
 }
 
 __global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
@@ -292,9 +302,12 @@ int GmlxTetQuality(size_t GmlIdx, int QalIdx)
    float *q = (float *)g->dat[QalIdx].dev;
    g->dat[QalIdx].host_valid = false;
    if(!nt) return 1;
+   const int per = sh.per;
+   auto kern = per == 1 ? k_tet_quality<1> : per == 4 ? k_tet_quality<4> : k_tet_quality<2>;
+   const long long span = (long long)(per == 1 || per == 4 ? per : 2) * sh.grp;
    return TimedLaunch(g, GmlxBuiltinQuality, [&]() {
-      hipLaunchKernelGGL(k_tet_quality, dim3((unsigned int)((nt + 2 * sh.grp - 1) / (2 * sh.grp))), dim3(sh.grp), 0, g->stream,
-                         nt, tet, crd, q, sh.remap);
+      hipLaunchKernelGGL(kern, dim3((unsigned int)((nt + span - 1) / span)), dim3(sh.grp), 0, g->stream, nt, tet, crd,
+                         q, sh.remap);
    });
 }
 
@@ -425,6 +438,13 @@ int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
    info[0] = p->TV; info[1] = p->ntile; info[2] = p->sum_nv; info[3] = p->sum_nt;
    info[4] = p->max_nv; info[5] = p->max_nt; info[6] = (long long)p->bytes;
    info[7] = p->sum_ring; info[8] = g_smooth_mode;
+   return 1;
+}
+
+int GmlxSetBuiltinItems(int which, int PerThread)
+{
+   if(which != GmlxBuiltinQuality || (PerThread != 1 && PerThread != 2 && PerThread != 4)) return 0;
+   g_shape[which].per = PerThread;
    return 1;
 }
 

@@ -303,6 +303,7 @@ void GmlStop(size_t GmlIdx)
       for(auto &e : k->evts) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
       for(auto &e : k->tune_ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
       if(k->mod) (void)hipModuleUnload(k->mod);
+      if(k->mod2) (void)hipModuleUnload(k->mod2);
    }
    for(int r = 0; r < GmlMaxRed; r++)
       for(auto &e : g->red_evts[r]) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }

@@ -29,23 +29,33 @@ size_t GmlxInitHostOnly(void);
 int    GmlxCompileKernelv(size_t GmlIdx, const char *KrnSrc, const char *PrcNam,
                           int MshTyp, int NmbTyp, const int *triplets);
 
-/* Generated HIP source of a compiled kernel (NUL-terminated, owned by the
-   library, valid until GmlStop); NULL if the index is invalid. */
 /* LDS staging of uplink data in generated kernels (default on): read-only,
    single-item data reached through a ball (e.g. tet data read by a vertex
    kernel) are loaded once per work-group tile into LDS with coalesced loads
    and read from there; results are identical either way.  Applies to
    kernels compiled afterwards; returns the previous setting. */
 int    GmlxSetStaging(int on);
+
+/* Generated HIP source of a compiled kernel (NUL-terminated, owned by the
+   library, valid until GmlStop); NULL if the index is invalid.  The paired
+   form (two rows per thread, the reads of both issued before either body;
+   built for kernels without uplinks) has its own source, NULL if none. */
 const char *GmlxGetKernelSource(size_t GmlIdx, int KrnIdx);
+const char *GmlxGetKernelPairSource(size_t GmlIdx, int KrnIdx);
 
 /* Group size and XCD remap flag the launch-shape tuner settled on for a
    kernel (0 in both while it is still tuning). */
 int    GmlxGetKernelShape(size_t GmlIdx, int KrnIdx, int *GrpSiz, int *Remap);
 
-/* Pin the launch shape of a kernel (disables its tuner); GrpSiz is 64, 128
-   or 256 (generated kernels are compiled for at most 256 work-items). */
-int    GmlxSetKernelShape(size_t GmlIdx, int KrnIdx, int GrpSiz, int Remap);
+/* Rows per thread of a kernel's launches: 1 or 2 (its paired form), 0 while
+   the tuner is still choosing; -1 for an invalid index. */
+int    GmlxGetKernelRows(size_t GmlIdx, int KrnIdx);
+
+/* Pin the launch shape of a kernel (disables its tuner): group size 64, 128
+   or 256 (generated kernels are compiled for at most 256 work-items), XCD
+   remap, rows per thread (2 needs the paired form).  Returns 1, or 0 if the
+   shape is not available for the kernel. */
+int    GmlxSetKernelShape(size_t GmlIdx, int KrnIdx, int GrpSiz, int Remap, int Rows);
 
 /* --- device residency ---------------------------------------------------- */
 
@@ -146,6 +156,9 @@ int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
    tile of the operator-form sweep; the exact fused sweep takes up to 256) or
    256 (SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
 int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);
+/* items per thread of a built-in kernel that takes several: the quality
+   kernel (GmlxBuiltinQuality) takes 1, 2 (default) or 4 tets per thread */
+int    GmlxSetBuiltinItems(int which, int PerThread);
 /* summed device time (s) and launch count of a built-in kernel on a handle
    (GmlxBuiltinSpMV: the block SpMV of GmlMultMatVec; GmlxBuiltinCgSpMV and
    GmlxBuiltinCgUpdate: the two kernels of a GmlxSolveCG iteration) */

@@ -276,3 +276,34 @@ def test_reference_heat_bodies_compile():
                  ("dt", T, [(vi, R, 0), (dt, W, 0)])]
         for name, typ, args in cases:
             assert g.compile(ref_body("HeatCellCentered/%s.cl" % name), name, typ, args) > 0, name
+
+
+def test_paired_form_emitted_only_where_it_is_safe(mesh):
+    # element loops over downlinks get a paired form (two rows per thread);
+    # uplink kernels, bodies writing the parameter block and bodies using
+    # work-item builtins keep the one-row form only
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        g.new_parameters(16, body("smooth_params"))
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+        kq = g.compile(body("tet_meanratio"), "q", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        kg = g.compile(body("smooth_gather"), "gt", G.GmlVertices,
+                       [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+        kw = g.compile("qal = 1.f; GmlPar->omega += qal;", "w", G.GmlTetrahedra, [(q, G.GmlWriteMode, 0)])
+        ki = g.compile("qal = (float)get_global_id(0);", "i", G.GmlTetrahedra, [(q, G.GmlWriteMode, 0)])
+        kr = g.compile("if(TetIdx > 3) return; qal = 2.f;", "r", G.GmlTetrahedra, [(q, G.GmlWriteMode, 0)])
+        src = g.L.GmlxGetKernelPairSource(g.h, kq).decode()
+        assert g.L.GmlxGetKernelPairSource(g.h, kg) is None
+        assert g.L.GmlxGetKernelPairSource(g.h, kw) is None
+        assert g.L.GmlxGetKernelPairSource(g.h, ki) is None
+        rsrc = g.L.GmlxGetKernelPairSource(g.h, kr).decode()
+        assert g.L.GmlxGetKernelRows(g.h, kq) == 0 and g.L.GmlxGetKernelRows(g.h, kg) == 1
+        assert g.L.GmlxSetKernelShape(g.h, kq, 128, 1, 2) == 1 and g.L.GmlxGetKernelRows(g.h, kq) == 2
+        assert g.L.GmlxSetKernelShape(g.h, kg, 128, 1, 2) == 0
+    # both rows read before either body; a row whose body returns early is not written
+    assert src.index("gml_read(gml_r1);") < src.index("gml_body(gml_r0);")
+    assert "if(gml_has1 && gml_r1.gml_done) gml_write(gml_r1);" in src
+    assert "VerCrd[0] = VerCrdTab[ TetVer.s0 ];" in src
+    assert "gml_x.gml_done = true;" in rsrc

@@ -200,3 +200,29 @@ def test_gpu_extract_on_reference_sample_mesh(tmp_path):
     # Euler for a plane triangulation with one hole: V - E + T = 0
     assert ne == 36624 + 72783
     assert np.array_equal(e[:465], np.sort(old, axis=1)) and np.array_equal(r[:465], oref) and (r[465:] == 0).all()
+
+
+def test_gpu_extract_small_and_empty_meshes():
+    # one tet: its 6 edges and 4 faces in local order; vertices only: nothing
+    crd = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    tet = np.array([[0, 1, 2, 3]], np.int32)
+    for host_only in (False, True):
+        with G.Gml(host_only=host_only) as g:
+            vi = g.L.GmlNewMeshData(g.h, G.GmlVertices, 4)
+            ti = g.L.GmlNewMeshData(g.h, G.GmlTetrahedra, 1)
+            assert vi and ti
+            for i in range(4):
+                assert G.set_line(g.h, vi, i, *[C.c_double(float(x)) for x in crd[i, :3]], C.c_int(0))
+            assert G.set_line(g.h, ti, 0, *[C.c_int(int(x)) for x in tet[0]], C.c_int(0))
+            ne = g.L.GmlExtractEdges(g.h)
+            nf = g.L.GmlExtractFaces(g.h)
+            e = get_block(g, G.GmlEdges, ne, 2)[0]
+            f = get_block(g, G.GmlTriangles, nf, 3)[0]
+        want_e, _ = first_appearance([], tet, TET_EDGES)
+        want_f, _ = first_appearance([], tet, TET_FACES)
+        assert ne == 6 and nf == 4 and np.array_equal(e, want_e) and np.array_equal(f, want_f), host_only
+        with G.Gml(host_only=host_only) as g:
+            assert g.L.GmlNewMeshData(g.h, G.GmlVertices, 4)
+            assert g.L.GmlExtractEdges(g.h) == 0 and g.L.GmlExtractFaces(g.h) == 0
+            assert not g.L.GmlGetMeshInfo(g.h, G.GmlEdges, None, None)
+            assert not g.L.GmlGetMeshInfo(g.h, G.GmlTriangles, None, None)
