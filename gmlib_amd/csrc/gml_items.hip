@@ -10,8 +10,8 @@
 // entity of an edge / face table) are laid out in the reference's visiting
 // order -- source by source, element by element, local item by local item --
 // so that an item's index is its visiting rank, and each gets a key: its
-// sorted nodes packed into one 64-bit word (b bits per node, b from the
-// vertex count) or, when deg * b > 64, into two words (32 bits per node).
+// sorted nodes packed b bits each (b from the vertex count) into one 64-bit
+// word, or, when deg * b > 64, into a 64-bit low word and a high word.
 // A stable LSD radix sort of the item indices by key puts equal items next
 // to each other, each run in visiting order (ascending element), and every
 // query becomes a run lookup:
@@ -52,9 +52,9 @@ struct Source {
 };
 
 struct KeyForm {
-   int deg, b;        // nodes per item, bits per node in the one-word form
-   bool two;          // two-word form: 32 bits per node, hi = leading nodes
-   int lo_bits, hi_bits;
+   int deg, b;        // nodes per item, bits per node
+   bool two;          // two-word form: the deg * b-bit key split into a low
+   int lo_bits, hi_bits;   // 64-bit word and a high word of deg * b - 64 bits
 };
 
 int bits_for(long long n)   // bits to hold 0 .. n-1 (at least 1)
@@ -70,8 +70,8 @@ KeyForm MakeKeyForm(int deg, long long nv)
    kf.deg = deg;
    kf.b = bits_for(std::max(2ll, nv));
    kf.two = deg * kf.b > 64;
-   kf.lo_bits = kf.two ? 32 + kf.b : deg * kf.b;
-   kf.hi_bits = kf.two ? (deg == 3 ? kf.b : 32 + kf.b) : 0;
+   kf.lo_bits = kf.two ? 64 : deg * kf.b;
+   kf.hi_bits = kf.two ? deg * kf.b - 64 : 0;
    return kf;
 }
 
@@ -132,19 +132,16 @@ __device__ __forceinline__ bool item_key(const int *__restrict__ ele, long long 
       out |= a[k] < 0 || a[k] >= nv;
    }
    sort_nodes(a, kf.deg);
-   if(!kf.two)
+   // the sorted nodes packed b bits each, most significant first, as a
+   // (hi, lo) pair of 64-bit words (b < 64)
+   unsigned long long l = 0, h = 0;
+   for(int i = 0; i < kf.deg; i++)
    {
-      unsigned long long k = 0;
-      for(int i = 0; i < kf.deg; i++) k = (k << kf.b) | (unsigned)a[i];
-      *lo = k;
-      *hi = 0;
+      h = (h << kf.b) | (l >> (64 - kf.b));
+      l = (l << kf.b) | (unsigned)a[i];
    }
-   else
-   {
-      const unsigned long long a0 = (unsigned)a[0], a1 = (unsigned)a[1], a2 = (unsigned)a[2], a3 = (unsigned)a[3];
-      if(kf.deg == 3) { *hi = a0; *lo = (a1 << 32) | a2; }
-      else            { *hi = (a0 << 32) | a1; *lo = (a2 << 32) | a3; }
-   }
+   *lo = l;
+   *hi = kf.two ? h : 0;
    return !out;
 }
 
@@ -496,17 +493,13 @@ __global__ void k_item_write(long long m, const int *__restrict__ u, const int *
    if(k >= m) return;
    const int i = u[k], p = upos[k];
    int a[4] = {0, 0, 0, 0};
-   if(!kf.two)
+   unsigned long long lo = klo[i], hi = kf.two ? khi[i] : 0;
+   const unsigned long long mask = (1ull << kf.b) - 1;
+   for(int j = kf.deg - 1; j >= 0; j--)
    {
-      unsigned long long key = klo[i];
-      const unsigned long long mask = (1ull << kf.b) - 1;
-      for(int j = kf.deg - 1; j >= 0; j--) { a[j] = (int)(key & mask); key >>= kf.b; }
-   }
-   else
-   {
-      const unsigned long long lo = klo[i], hi = khi[i];
-      if(kf.deg == 3) { a[0] = (int)hi; a[1] = (int)(lo >> 32); a[2] = (int)(unsigned)lo; }
-      else            { a[0] = (int)(hi >> 32); a[1] = (int)(unsigned)hi; a[2] = (int)(lo >> 32); a[3] = (int)(unsigned)lo; }
+      a[j] = (int)(lo & mask);
+      lo = (lo >> kf.b) | (hi << (64 - kf.b));
+      hi >>= kf.b;
    }
    for(int j = 0; j < w; j++) tab[k * w + j] = j < kf.deg ? a[j] : 0;
    ref[k] = (p < nold && oref) ? oref[p] : 0;

@@ -3,8 +3,9 @@ packed node keys on the GPU) against the same calls on a host-only handle
 (the ordered-map form, itself checked against a brute-force restatement of
 sources/gmlib3.c:3719-4171 in test_extract.py) and against that restatement
 directly: bit-identical tables, refs and counts.  Covers the one-word key
-(b bits per node) and the two-word key (32 bits per node, forced by a vertex
-count above 2^21 for triangles and 2^16 for quads), mixed element kinds
+(b bits per node in one 64-bit word) and the two-word key (the same b-bit
+packing spilling into a high word, forced by a vertex count above 2^21 for
+triangles and 2^16 for quads), mixed element kinds
 (tets, pyramids, prisms, hexes), an existing table with refs and duplicates,
 and the count closed forms of a large Kuhn cube."""
 import ctypes as C
