@@ -340,6 +340,12 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
            "   T (*tab)[NI]; const int *idx; int deg; T nul;\n"
            "   __device__ __forceinline__ gml_uprow<T, NI> operator[](int k) const { return gml_uprow<T, NI>{tab, idx[k], k < deg, nul}; }\n"
            "};\n";
+   if(lazy && sball >= 0)
+      // staged single-item data: each read from LDS where the body uses it
+      s += "template<class T, int W> struct gml_stg {\n"
+           "   const T *s; const unsigned short *l; int deg; T nul;\n"
+           "   __device__ __forceinline__ T operator[](int k) const { return k < deg ? s[l[k]] : nul; }\n"
+           "};\n";
    if(g->UsrTlk)
    {
       s += "// USER'S TOOLKIT\n";
@@ -372,7 +378,8 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       const Arg *L = a.LnkIdx >= 0 ? &args[a.LnkIdx] : nullptr;
       const Arg *C = a.CntIdx >= 0 ? &args[a.CntIdx] : nullptr;
       const char *T = OclTypStr[a.ItmTyp];
-      bool lz = lazy && C && a.NmbItm > 1;
+      const bool sd = std::find(staged.begin(), staged.end(), (int)i) != staged.end();
+      bool lz = lazy && C && (a.NmbItm > 1 || sd);
       if(a.kind == AkBall)
       {
          if((int)i == sball) decl += local("unsigned short", "gml_l_" + a.nam, Fmt("[%d]", a.NmbItm * a.ItmLen));
@@ -415,8 +422,12 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       s += "   int       cnt = gml_gid(gml_remap);\n";
       s += Fmt("   int       %s = cnt + count.s1;\n\n", idxnam.c_str());
    }
+   // staged form: the tile's loads into LDS, emitted after the row's own
+   // global reads so that both are in flight together
+   std::string stage_s;
    if(sball >= 0)
    {
+      std::swap(s, stage_s);
       s += "// STAGED UPLINK DATA: the work-group's tile, coalesced loads into LDS\n"
            "   extern __shared__ __align__(16) char gml_lds[];\n"
            "   size_t gml_lo = 0;\n";
@@ -427,19 +438,42 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
          s += Fmt("   %s *gml_s_%s = (%s *)(gml_lds + gml_lo);\n", T, a.nam.c_str(), T);
          s += Fmt("   gml_lo += ((size_t)gml_stage_n * sizeof(%s) + 15) & ~(size_t)15;\n", T);
       }
-      s += "   {\n"
-           "      const unsigned int gml_t = (unsigned int)(cnt - (int)threadIdx.x) / blockDim.x;\n"
-           "      const unsigned int gml_t0 = gml_tofs[gml_t], gml_nt = gml_tofs[gml_t + 1] - gml_t0;\n"
-           "      for(unsigned int gml_i = threadIdx.x; gml_i < gml_nt; gml_i += blockDim.x)\n      {\n"
-           "         const int gml_x = gml_tlist[gml_t0 + gml_i];\n";
+      // U entities per thread and pass, all U loads in flight before the
+      // LDS stores; a slot past the end is clamped to the last entity, whose
+      // value it loads and stores again (the same bytes)
+      int bytes = 0;
+      for(int i : staged) bytes += OclTypSiz[args[i].ItmTyp];
+      const int U = bytes <= 16 ? 8 : (bytes <= 32 ? 4 : 2);
+      s += Fmt("   {\n"
+               "      const unsigned int gml_t = (unsigned int)(cnt - (int)threadIdx.x) / blockDim.x;\n"
+               "      const unsigned int gml_t0 = gml_tofs[gml_t], gml_nt = gml_tofs[gml_t + 1] - gml_t0;\n"
+               "      for(unsigned int gml_i = threadIdx.x; gml_i < gml_nt; gml_i += %d * blockDim.x)\n      {\n"
+               "         int gml_x[%d];\n"
+               "#pragma unroll\n"
+               "         for(int gml_u = 0; gml_u < %d; gml_u++)\n"
+               "            gml_x[gml_u] = gml_tlist[gml_t0 + min(gml_i + gml_u * blockDim.x, gml_nt - 1)];\n",
+               U, U, U);
       for(int i : staged)
-         s += Fmt("         gml_s_%s[gml_i] = %sTab[gml_x];\n", args[i].nam.c_str(), args[i].nam.c_str());
-      s += "      }\n      __syncthreads();\n   }\n\n";
+      {
+         const char *n = args[i].nam.c_str();
+         s += Fmt("         %s gml_v_%s[%d];\n#pragma unroll\n"
+                  "         for(int gml_u = 0; gml_u < %d; gml_u++)\n"
+                  "            gml_v_%s[gml_u] = %sTab[gml_x[gml_u]];\n",
+                  OclTypStr[args[i].ItmTyp], n, U, U, n, n);
+      }
+      s += Fmt("#pragma unroll\n"
+               "         for(int gml_u = 0; gml_u < %d; gml_u++)\n"
+               "         {\n"
+               "            const unsigned int gml_j = min(gml_i + gml_u * blockDim.x, gml_nt - 1);\n", U);
+      for(int i : staged)
+         s += Fmt("            gml_s_%s[gml_j] = gml_v_%s[gml_u];\n", args[i].nam.c_str(), args[i].nam.c_str());
+      s += "         }\n      }\n      __syncthreads();\n   }\n\n";
+      std::swap(s, stage_s);
    }
-   if(!pair) s += "   if(cnt >= count.s0)\n      return;\n\n";
+   if(!pair && sball < 0) s += "   if(cnt >= count.s0)\n      return;\n\n";
 
    // --- reads (gmlib3.c:2755-2885), emitted into their own string
-   std::string body_s;
+   std::string body_s, staged_reads;
    std::swap(s, body_s);
    s += "// KERNEL MEMORY READINGS\n";
    for(size_t i = 0; i < args.size(); i++)
@@ -526,9 +560,15 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
                lnk = L->nam + l1 + l2;
             }
             const bool sd = std::find(staged.begin(), staged.end(), (int)i) != staged.end();
-            if(C && sd)
-               s += Fmt("   %s%s = (%sDeg >= %d) ? gml_s_%s[ gml_l_%s[%d] ] : %sNul;\n",
-                        n, td2.c_str(), n, k + 1, n, L->nam.c_str(), k, n);
+            if(C && sd && lazy)
+            {
+               if(k == 0)
+                  staged_reads += Fmt("   gml_stg<%s, %d> %s{gml_s_%s, gml_l_%s, %sDeg, %sNul};\n", OclTypStr[a.ItmTyp],
+                                      siz, n, n, L->nam.c_str(), n, n);
+            }
+            else if(C && sd)
+               staged_reads += Fmt("   %s%s = (%sDeg >= %d) ? gml_s_%s[ gml_l_%s[%d] ] : %sNul;\n",
+                                   n, td2.c_str(), n, k + 1, n, L->nam.c_str(), k, n);
             else if(C)
                s += Fmt("   %s%s%s = (%sDeg >= %d) ? %sTab[ %s ]%s : %sNul;\n",
                         n, td2.c_str(), td1.c_str(), n, k + 1, n, lnk.c_str(), td1.c_str(), n);
@@ -569,6 +609,16 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    std::string writes;
    std::swap(s, writes);
    s = writes_s;
+   if(!pair && sball >= 0)
+   {
+      // the row's global reads (a thread past the end reads the last row
+      // and later returns: every thread reaches the barrier, and no branch
+      // makes the compiler wait for these loads before the staging's), the
+      // tile staging, then the reads of the staged data from LDS
+      s += "   const int gml_cnt = cnt;\n   cnt = min(cnt, count.s0 - 1);\n" + reads + "   cnt = gml_cnt;\n\n" + stage_s +
+           "   if(cnt >= count.s0)\n      return;\n\n// STAGED READINGS\n" + staged_reads + "\n" + user + writes + "}\n";
+      return s;
+   }
    if(!pair)
    {
       s += reads + user + writes + "}\n";
@@ -998,7 +1048,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
    int KrnIdx = 0;
    for(int st = sball >= 0 ? 1 : 0; st >= 0 && !KrnIdx; st--)
    {
-      if(any_lazy)
+      if(any_lazy || st)
       {
          src = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, true, st);
          KrnIdx = NewKernel(g, src, PrcNam, &log);
