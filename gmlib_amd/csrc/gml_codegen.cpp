@@ -241,6 +241,9 @@ static int StagedArgs(const std::vector<Arg> &args, std::vector<int> *staged)
       const Arg &a = args[i];
       if(a.kind != AkData || a.CntIdx < 0 || a.LnkIdx < 0 || args[a.LnkIdx].kind != AkBall) continue;
       if(!(a.FlgTab & GmlReadMode) || (a.FlgTab & (GmlWriteMode | GmlManual)) || a.NmbItm != 1) continue;
+      // a short ball (a face's two cells) gains nothing from the tile's
+      // shared loads: its rows are gathered directly
+      if(args[a.LnkIdx].MaxDeg <= 4) continue;
       if(ball < 0) ball = a.LnkIdx;
       if(a.LnkIdx == ball) staged->push_back((int)i);
    }
@@ -296,8 +299,15 @@ static bool BodyWritesParams(const char *body)
 static bool PairEligible(const std::vector<Arg> &args, const char *KrnSrc, bool lazy, bool stage)
 {
    if(lazy || stage || BodyWritesParams(KrnSrc) || strstr(KrnSrc, "atomic")) return false;
+   // uplinks only while a row's gathered values stay small (32 words)
+   int words = 0;
    for(const Arg &a : args)
-      if(a.kind == AkBall || a.CntIdx >= 0 || (a.FlgTab & GmlManual)) return false;
+   {
+      if(a.FlgTab & GmlManual) return false;
+      if(a.kind == AkBall) words += 2 * a.MaxDeg;
+      else if(a.CntIdx >= 0) words += args[a.LnkIdx].MaxDeg * a.NmbItm * ((OclTypSiz[a.ItmTyp] + 3) / 4);
+   }
+   if(words > 32) return false;
    for(const char *id : {"barrier", "get_global_id", "get_local_id", "get_group_id", "get_local_size",
                          "get_global_size", "get_num_groups", "__syncthreads", "gml_gid"})
       if(BodyUses(KrnSrc, id)) return false;

@@ -66,12 +66,14 @@ def test_uplink_kernel_contract_with_ref_and_params(mesh, staged):
     assert "#define   BaryDegMax 32" in s
     assert "int16 VerTetBal[2];" in s and "int VerTetDeg;" in s and "int BaryDeg;" in s
     if staged:
-        # tile loads into LDS before the bounds return, reads through tile-local indices;
-        # the body does not name the ball, so its entries are not read at all
+        # the row's own reads, then the tile's loads into LDS, the barrier and
+        # the bounds return; the body reads Bary through an LDS accessor by
+        # tile-local index; it does not name the ball, so its entries are not read
         h = s[s.index("// KERNEL HEADER"):]
-        assert h.index("gml_s_Bary[gml_i] = BaryTab[gml_x];") < h.index("__syncthreads();") \
-            < h.index("if(cnt >= count.s0)")
-        assert "Bary[23] = (BaryDeg >= 24) ? gml_s_Bary[ gml_l_VerTetBal[23] ] : BaryNul;" in s
+        assert h.index("VerRef = __builtin_nontemporal_load(&VerRefTab[ cnt + count.s1 ]);") \
+            < h.index("gml_v_Bary[gml_u] = BaryTab[gml_x[gml_u]];") < h.index("gml_s_Bary[gml_j] = gml_v_Bary[gml_u];") \
+            < h.index("__syncthreads();") < h.index("if(cnt >= count.s0)")
+        assert "gml_stg<float4, 32> Bary{gml_s_Bary, gml_l_VerTetBal, BaryDeg, BaryNul};" in s
         assert "gml_VerTetBal[k] = v >> 4;" not in s
     else:
         assert "Bary[23] = (BaryDeg >= 24) ? BaryTab[ gml_VerTetBal[23] ] : BaryNul;" in s
@@ -280,8 +282,8 @@ def test_reference_heat_bodies_compile():
 
 def test_paired_form_emitted_only_where_it_is_safe(mesh):
     # element loops over downlinks get a paired form (two rows per thread);
-    # uplink kernels, bodies writing the parameter block and bodies using
-    # work-item builtins keep the one-row form only
+    # kernels gathering through wide balls, bodies writing the parameter
+    # block and bodies using work-item builtins keep the one-row form only
     crd, ref, tet = mesh
     with G.Gml(host_only=True) as g:
         vi, ti = g.set_mesh(crd, ref, tet)
@@ -307,3 +309,40 @@ def test_paired_form_emitted_only_where_it_is_safe(mesh):
     assert "if(gml_has1 && gml_r1.gml_done) gml_write(gml_r1);" in src
     assert "VerCrd[0] = VerCrdTab[ TetVer.s0 ];" in src
     assert "gml_x.gml_done = true;" in rsrc
+
+
+def test_short_shells_gather_directly_and_pair(mesh):
+    # a face's two cells (tri -> tet shell, degree <= 2) are gathered
+    # directly, not staged in LDS, and the kernel gets a paired form; a body
+    # that assigns to staged data falls back to the eager LDS reads
+    from gmlib_amd import heat as H
+    crd, vref, tet, tri, side = H.heat_cube(3, 0.2, 1)
+    with G.Gml(host_only=True) as g:
+        L, h = g.L, g.h
+        vi, ti = g.set_mesh(crd, vref, tet)
+        g.new_parameters(C.sizeof(H.HeatParams), H.body("heat_params"))
+        nt = len(tri)
+        assert L.GmlNewMeshData(h, G.GmlTriangles, nt)
+        assert L.GmlSetDataBlock(h, G.GmlTriangles, 0, nt - 1, G.ptr(tri), C.c_void_p(tri.ctypes.data + (nt - 1) * 12),
+                                 G.ptr(side), C.c_void_p(side.ctypes.data + (nt - 1) * 4))
+        assert L.GmlExtractFaces(h) > nt
+        idx = C.c_int(0)
+        L.GmlGetMeshInfo(h, G.GmlTriangles, None, C.byref(idx))
+        sol = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "SolTet")
+        sx = g.solution(G.GmlTriangles, 1, G.GmlFlt, "SolExt")
+        R = G.GmlReadMode
+        k = g.compile(H.body("heat_face_value"), "fv", G.GmlTriangles,
+                      [(idx.value, R | G.GmlRefFlag, 0), (sol, R, 0), (sx, G.GmlWriteMode, 0)])
+        one, two = src_of(g, k), L.GmlxGetKernelPairSource(h, k)
+        assert "gml_lds" not in one and two is not None
+        two = two.decode()
+        assert "SolTet[1] = (SolTetDeg >= 2) ? SolTetTab[ gml_TriTetBal[1] ] : SolTetNul;" in two
+        assert two.index("gml_read(gml_r1);") < two.index("gml_body(gml_r0);")
+
+        b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+        out = g.solution(G.GmlVertices, 1, G.GmlFlt4, "Out")
+        ke = g.compile("Bary[0] = Bary[1] + Bary[2];\nOut = Bary[0];", "e", G.GmlVertices,
+                       [(b, R, 0), (out, G.GmlWriteMode, 0)])
+        se = src_of(g, ke)
+    assert "gml_stg<" not in se and "gml_v_Bary[gml_u] = BaryTab[gml_x[gml_u]];" in se
+    assert "Bary[0] = (BaryDeg >= 1) ? gml_s_Bary[ gml_l_VerTetBal[0] ] : BaryNul;" in se

@@ -73,13 +73,15 @@ def test_paired_early_return_and_partial_group():
         assert np.array_equal(v, want), key
 
 
-def test_paired_heat_cell_kernels_and_tuner():
+def test_paired_heat_kernels_and_tuner():
+    # face kernels gather their two cells through the tri -> tet shell, cell
+    # kernels their faces through the tet -> tri downlink
     crd, vref, tet, tri, side = H.heat_cube(6, 0.2, 4)
     outs = []
     for rows in (1, 2):
         s = H.HeatSolver(crd, vref, tet, tri, side)
         try:
-            for k in (s.k_grad, s.k_bal, s.k_step):
+            for k in (s.k_face, s.k_grad, s.k_fgrad, s.k_bal, s.k_step):
                 assert s.g.L.GmlxGetKernelPairSource(s.g.h, k) is not None
                 assert s.g.L.GmlxSetKernelShape(s.g.h, k, 256, 1, rows) == 1
             s.iterate(15)
