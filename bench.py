@@ -240,10 +240,10 @@ def run_smoothing(args):
                 raise RuntimeError("no sweep plan")
             log("sweep plan: mode %d, tile %d, %d tiles, tile tets / T = %.3f, tile vertices / V = %.3f, "
                 "1-ring slots %d, %.2f GB" % (info[8], info[0], info[1], info[3] / T, info[2] / V, info[7], info[6] / 1e9))
-            # operator form: 1-ring slots without the vertex itself (2 B) + degree and ring length
-            # (4 B) + ref (4 B) + old and new coordinates (16 B each), every datum once; exact fused
-            # form: tet->vertex rows as the plan's 8 B + ball slots 2 B x 4T + 40 V
-            sweep_alg = 2 * info[7] + 40 * V if info[8] == 2 else fused_bytes
+            # operator form: 1-ring slots (2 B; the degree is their count sum / 3) + ref (4 B) +
+            # old and new coordinates (16 B each), every datum once; exact fused form:
+            # tet->vertex rows as the plan's 8 B + ball slots 2 B x 4T + 40 V
+            sweep_alg = 2 * info[7] + 36 * V if info[8] == 2 else fused_bytes
             per = {"sweep": (sweep_alg, ek[0] / nl), "quality": (qual_bytes, ek[1] / args.steps)}
         kern = {k: {"avg_us": round(v[1] * 1e6, 2), "alg_GBs": round(v[0] / v[1] / 1e9, 1),
                     "frac": round(v[0] / v[1] / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": v[0]}

@@ -124,9 +124,9 @@ struct SweepPlan {
    unsigned short *lidx = nullptr;          // ball slot -> tile-local tet index (SELL-64)
    // operator form: per vertex, its 1-ring without itself as 16-bit
    // (tile-local index << 4 | count of ball tets containing the neighbour),
-   // SELL-64 over the vertices, and (deg << 16 | ring length)
+   // SELL-64 over the vertices (deg v = sum of the ring's counts / 3)
    unsigned short *nbr = nullptr;
-   unsigned int *noff = nullptr, *meta = nullptr;
+   unsigned int *noff = nullptr;
    long long nslab = 0;
    size_t bytes = 0;
 };

@@ -204,12 +204,15 @@ __global__ void __launch_bounds__(256) k_sweep_fused(long long nv, const unsigne
 // coordinates in LDS (its own vertices at local 0..TV-1 straight from the
 // coordinate array, the listed halo from TV on), then
 // x_v += omega (sum_u c_vu x_u / (4 deg v) - x_v) with the 1-ring read as
-// (tile-local index << 4 | c_vu) slots.  The vertex's own data (degree
-// word, ref, coordinates, first 16 ring slots) are loaded before the halo
-// staging so that both latencies overlap.  One vertex per thread, TV =
-// work-group size (64..1024).
+// (tile-local index << 4 | c_vu) slots.  Every lane reads its SELL-64
+// slab to the slab's width (zero slots past its ring are skipped), and
+// deg v = (sum of its counts) / 3: each ball tet of v contributes its three
+// other vertex positions (a repeat of v among them is a self slot).  The
+// vertex's own data (ref, coordinates, first 16 ring slots) are loaded
+// before the halo staging so that both latencies overlap.  One vertex per
+// thread, TV = work-group size (64..1024).
 __global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned int *__restrict__ vofs,
-                                                    const int *__restrict__ vlist, const unsigned int *__restrict__ meta,
+                                                    const int *__restrict__ vlist,
                                                     const unsigned int *__restrict__ noff,
                                                     const unsigned short *__restrict__ nbr, const int *__restrict__ ref,
                                                     const v4f *__restrict__ xo, v4f *__restrict__ xn, float omega,
@@ -220,9 +223,14 @@ __global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned 
    const int tid = threadIdx.x, TV = blockDim.x;
    const long long v = (long long)tile * TV + tid;
    const bool act = v < nv;
-   const unsigned int mt = act ? meta[v] : 0u;
-   const int d = (int)(mt >> 16), m = (int)(mt & 0xffffu);
-   const unsigned short *row = nbr + (act ? (size_t)noff[v >> 6] * 64 + (v & 63) : 0);
+   int m = 0;
+   size_t o = 0;
+   if(act)
+   {
+      o = noff[v >> 6];
+      m = (int)(noff[(v >> 6) + 1] - o);
+   }
+   const unsigned short *row = nbr + (act ? o * 64 + (v & 63) : 0);
    unsigned int e0[16];
 #pragma unroll
    for(int u = 0; u < 16; u++)
@@ -250,10 +258,15 @@ __global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned 
    }
    __syncthreads();
    if(!act) return;
-   v4f s = p * (float)d;                       // c_vv x_v, c_vv = deg v
+   v4f s = {0.f, 0.f, 0.f, 0.f};
+   int cs = 0;
 #pragma unroll
    for(int u = 0; u < 16; u++)
-      if(u < m) s += X[e0[u] >> 4] * (float)(e0[u] & 15u);
+      if(u < m && e0[u])
+      {
+         s += X[e0[u] >> 4] * (float)(e0[u] & 15u);
+         cs += (int)(e0[u] & 15u);
+      }
    for(int k = 16; k < m; k += 8)
    {
       unsigned int e[8];
@@ -262,8 +275,14 @@ __global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned 
          e[u] = k + u < m ? __builtin_nontemporal_load(&row[(k + u) * 64]) : 0u;
 #pragma unroll
       for(int u = 0; u < 8; u++)
-         if(k + u < m) s += X[e[u] >> 4] * (float)(e[u] & 15u);
+         if(k + u < m && e[u])
+         {
+            s += X[e[u] >> 4] * (float)(e[u] & 15u);
+            cs += (int)(e[u] & 15u);
+         }
    }
+   const int d = cs / 3;                       // c_vv x_v, c_vv = deg v
+   s += p * (float)d;
    if(!rf && d > 0)
    {
       s *= 0.25f / (float)d;
@@ -366,7 +385,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
          v4f *xn = (v4f *)g->xalt;
          int rc = TimedLaunch(g, GmlxBuiltinSweep, [&]() {
             hipLaunchKernelGGL(k_sweep_op, dim3((unsigned int)p->ntile), dim3(p->TV), lds, g->stream, nv,
-                               p->vofs, p->vlist, p->meta, p->noff, p->nbr, ref, xo, xn, (float)omega, sf.remap);
+                               p->vofs, p->vlist, p->noff, p->nbr, ref, xo, xn, (float)omega, sf.remap);
          });
          if(rc != 1) return rc;
          void *t = vd->dev; vd->dev = g->xalt; g->xalt = t;

@@ -18,10 +18,12 @@
 // a fixed sparse operator.  Its tile-local numbering puts the tile's own
 // vertices first (local index v - v0, staged with coalesced loads) and the
 // halo (the other 1-ring vertices, ascending) from TV on; only the halo is
-// listed.  The ring of v without v itself is stored as 16-bit entries
-// (tile-local index << 4 | c_vu), ascending local index (a count above 15
-// continues in the next entry); one 32-bit word per vertex holds
-// (deg v << 16 | ring length).
+// listed.  The ring of v is stored as 16-bit entries (tile-local index << 4
+// | c_vu), ascending local index (a count above 15 continues in the next
+// entry), SELL-64, zero past a ring's end.  Each ball tet of v contributes
+// its three other vertex positions, so the counts sum to 3 deg v and the
+// kernel derives deg v from them; v itself appears only when a ball tet
+// lists it more than once (the count beyond c_vv = deg v).
 
 #include "gml_internal.h"
 
@@ -107,12 +109,13 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
                      size_t j = i;
                      while(j < cnt.size() && cnt[j] == cnt[i]) j++;
                      if(cnt[i] > 4095) fits = false;
-                     if(cnt[i] != self)
-                        for(size_t c = j - i; c > 0; c -= std::min<size_t>(c, 15))
-                           R.push_back((unsigned short)((cnt[i] << 4) | std::min<size_t>(c, 15)));
+                     size_t n = j - i;
+                     if(cnt[i] == self) n = n > (size_t)deg[v] ? n - deg[v] : 0;
+                     for(size_t c = n; c > 0; c -= std::min<size_t>(c, 15))
+                        R.push_back((unsigned short)((cnt[i] << 4) | std::min<size_t>(c, 15)));
                      i = j;
                   }
-                  if(R.size() > 65535 || deg[v] > 65535) fits = false;
+                  if(R.size() > 65535) fits = false;
                }
                tv[t] = halo;
                continue;
@@ -153,7 +156,6 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
       for(long long t = 0; t < ntile; t++)
          std::copy(tv[t].begin(), tv[t].end(), vlist.begin() + vofs[t]);
       std::vector<unsigned int> noff((size_t)nchunk + 1, 0);
-      std::vector<unsigned int> meta((size_t)nv);
       for(long long c = 0; c < nchunk; c++)
       {
          unsigned int w = 0;
@@ -164,7 +166,6 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
       std::vector<unsigned short> nbr((size_t)p->nslab * 64 + 64, 0);
       for(long long v = 0; v < nv; v++)
       {
-         meta[v] = ((unsigned int)deg[v] << 16) | (unsigned int)ring[v].size();
          p->sum_ring += (long long)ring[v].size();
          for(size_t k = 0; k < ring[v].size(); k++)
             nbr[((size_t)noff[v >> 6] + k) * 64 + (v & 63)] = ring[v][k];
@@ -172,19 +173,17 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
       p->vofs = (unsigned int *)DevAlloc(g, vofs.size() * 4);
       p->vlist = (int *)DevAlloc(g, vlist.size() * 4);
       p->noff = (unsigned int *)DevAlloc(g, noff.size() * 4);
-      p->meta = (unsigned int *)DevAlloc(g, meta.size() * 4);
       p->nbr = (unsigned short *)DevAlloc(g, nbr.size() * 2);
-      if(!p->vofs || !p->vlist || !p->noff || !p->meta || !p->nbr
+      if(!p->vofs || !p->vlist || !p->noff || !p->nbr
       || !DevCopy(g, p->vofs, vofs.data(), vofs.size() * 4, 1)
       || !DevCopy(g, p->vlist, vlist.data(), vlist.size() * 4, 1)
       || !DevCopy(g, p->noff, noff.data(), noff.size() * 4, 1)
-      || !DevCopy(g, p->meta, meta.data(), meta.size() * 4, 1)
       || !DevCopy(g, p->nbr, nbr.data(), nbr.size() * 2, 1))
       {
          FreeSweepPlan(g, p);
          return nullptr;
       }
-      p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + meta.size() * 4 + nbr.size() * 2;
+      p->bytes = vofs.size() * 4 + vlist.size() * 4 + noff.size() * 4 + nbr.size() * 2;
       p->sum_nv = vofs[ntile] + nv;
       g->MemSiz += p->bytes;
       DbgPrint(g, "Operator sweep plan: %lld tiles of %d vertices, %u tile vertices, %lld ring slabs, %zu MB\n",
@@ -308,7 +307,7 @@ void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
    if(!p) return;
    DevFree(g, p->vofs); DevFree(g, p->tofs); DevFree(g, p->vlist);
    DevFree(g, p->tloc); DevFree(g, p->lidx);
-   DevFree(g, p->nbr); DevFree(g, p->noff); DevFree(g, p->meta);
+   DevFree(g, p->nbr); DevFree(g, p->noff);
    g->MemSiz -= p->bytes;
    delete p;
 }

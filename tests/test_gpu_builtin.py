@@ -82,3 +82,31 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
     for w in range(5):
         G.lib().GmlxSetBuiltinShape(w, 256, 1)
     G.lib().GmlxSetSmoothMode(2)
+
+
+
+@pytest.mark.parametrize("degenerate", [False, True])
+def test_operator_sweep_degree_from_counts(degenerate):
+    # the operator form derives deg v from its 1-ring counts (each ball tet
+    # gives its three other vertex positions); a tet listing a vertex twice
+    # adds a self slot, so the sweep stays the mean ball barycentre of the
+    # reference's ball (the tet counted once per position)
+    crd, ref, tet = G.cube_mesh(6, 0.2, 7)
+    if degenerate:
+        inner = np.nonzero((ref[tet] == 0).all(axis=1))[0][0]
+        a, b, c, _ = tet[inner]
+        tet = np.vstack([tet, [[a, a, b, c]]]).astype(np.int32)
+    omega = 0.3
+    assert G.lib().GmlxSetSmoothMode(2)
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        bary = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+        assert g.L.GmlxSmoothSweeps(g.h, bary, 2, omega) == 1
+        info = (C.c_longlong * 9)()
+        assert g.L.GmlxGetSweepPlanInfo(g.h, info) and info[8] == 2
+        c, _ = g.vertices(len(crd))
+    deg, ball, _ = O.vertex_ball(tet, len(crd), 32)
+    if degenerate:
+        assert deg[a] == np.count_nonzero(tet == a)            # the repeated vertex's ball holds the tet twice
+    c64 = O.smooth64(tet, ref, deg, ball, crd, 2, omega)
+    assert np.abs(c[:, :3] - c64).max() <= 2e-6
