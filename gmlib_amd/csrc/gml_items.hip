@@ -394,9 +394,13 @@ bool KeysOf(GmlCtx *g, const Source &sr, const KeyForm &kf, long long nv, int *b
 // 12) go into HS slots, so a slot is always free and every round ends.
 constexpr int kDedupThreads = 256;
 
+// LDS hash slot of a key: the xor-shift folds the word's high bits down
+// before the multiply, whose bits 40.. then depend on every key bit
 __device__ __forceinline__ unsigned int dedup_slot(unsigned long long lo, unsigned long long hi)
 {
-   const unsigned long long h = (lo ^ (hi * 0xc2b2ae3d27d4eb4full)) * 0x9e3779b97f4a7c15ull;
+   unsigned long long h = lo ^ (hi * 0xc2b2ae3d27d4eb4full);
+   h ^= h >> 31;
+   h *= 0x9e3779b97f4a7c15ull;
    return (unsigned int)(h >> 40);
 }
 
