@@ -87,7 +87,7 @@ def body(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract", "heat"])
+    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract", "heat", "bandwidth"])
     ap.add_argument("--qps", type=float, default=40.0, help="serve workload: arrival rate (jobs/s)")
     ap.add_argument("--jobs", type=int, default=400, help="serve workload: jobs in the timed stream")
     ap.add_argument("--pool", type=int, default=24, help="serve workload: meshes in the pool")
@@ -132,6 +132,8 @@ def main():
         run_extract(args)
     elif args.workload == "heat":
         run_heat(args)
+    elif args.workload == "bandwidth":
+        run_bandwidth(args)
     else:
         run_smoothing(args)
 
@@ -1010,6 +1012,152 @@ def run_heat(args):
                                "cores": 1, "kind": "port",
                                "sample": "%d iterations of the numpy FP64 restatement (links and geometry set up "
                                          "once per call) on a %d^3 cube (%d tets), %.2f s" % (k, m, len(tt), ct)}
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+def bandwidth_models(V, T):
+    """per kernel: (algorithmic bytes, each datum once) and the reference
+    example's own figures (gpu_bandwidth.c:151-176: direct 32V + 32T for the
+    vertex and tet kernels together; indirect 96T raw, 16V + 32T unique;
+    ragged 460V raw, 32V + 16T unique)"""
+    alg = {"direct_vertices": 32 * V, "direct_tets": 32 * T, "indirect_tets": 16 * V + 32 * T,
+           "vertices_ball": 32 * T + 20 * V}       # ball slots 16T, degree 4V, TetDat 16T, VerDat 16V
+    return alg
+
+
+def bandwidth_pass(G, crd, ref, tet, steps, warmup):
+    """the four GpuBandwidth kernels on one handle: per-kernel average
+    seconds (HIP events on the library's stream), GmlEvaluateNumbering"""
+    R, W = G.GmlReadMode, G.GmlWriteMode
+    g = G.Gml(0)
+    try:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        score = g.L.GmlEvaluateNumbering(g.h)
+        vdat = g.solution(G.GmlVertices, 1, G.GmlFlt4, "VerDat")
+        tdat = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "TetDat")
+        ks = {"direct_vertices": g.compile(body("bw_direct_vertices"), "direct_vertices", G.GmlVertices,
+                                           [(vi, R, 0), (vdat, W, 0)]),
+              "direct_tets": g.compile(body("bw_direct_tets"), "direct_tets", G.GmlTetrahedra,
+                                       [(ti, R, 0), (tdat, W, 0)]),
+              "indirect_tets": g.compile(body("bw_indirect_tets"), "indirect_tets", G.GmlTetrahedra,
+                                         [(vi, R, 0), (tdat, W, 0)]),
+              "vertices_ball": g.compile(body("bw_vertices_ball"), "vertices_ball", G.GmlVertices,
+                                         [(vdat, W, 0), (tdat, R, 0)])}
+        t = {}
+        for name, k in ks.items():           # the example's order: direct, indirect, ragged
+            # past the launch tuner's shapes (at most 11 launches)
+            for _ in range(max(warmup, 12)):
+                if g.L.GmlLaunchKernel(g.h, k) != 1:
+                    raise RuntimeError("launch failed")
+            g.L.GmlxSync(g.h)
+            t0 = g.kernel_time(k)
+            for _ in range(steps):
+                g.L.GmlLaunchKernel(g.h, k)
+            g.L.GmlxSync(g.h)
+            t[name] = (g.kernel_time(k) - t0) / steps
+        return t, score
+    finally:
+        g.close()
+
+
+def run_bandwidth(args):
+    import numpy as np
+    import gmlib_amd as G
+
+    n = args.n
+    t0 = time.time()
+    crd0, ref0, tet0 = G.cube_mesh(n, 0.2, 2024, False)
+    V, T = len(crd0), len(tet0)
+    rng = np.random.default_rng(7)
+    pv = rng.permutation(V)
+    inv = np.empty_like(pv)
+    inv[pv] = np.arange(V)
+    crdr = np.ascontiguousarray(crd0[pv])
+    refr = np.ascontiguousarray(ref0[pv])
+    tetr = np.ascontiguousarray(inv[tet0[rng.permutation(T)]].astype(np.int32))
+    del crd0, ref0, tet0, pv, inv
+    log("bandwidth: %d^3 cube (V=%d, T=%d), random numbering in %.1fs" % (n, V, T, time.time() - t0))
+    alg = bandwidth_models(V, T)
+    res = {}
+    tr, sr = bandwidth_pass(G, crdr, refr, tetr, args.steps, args.warmup)
+    res["random"] = (tr, sr)
+    crdh, refh, teth = crdr.copy(), refr.copy(), tetr.copy()
+    t0 = time.time()
+    if G.lib().GmlxHilbertRenumber(V, G.ptr(crdh), G.ptr(refh), T, 4, G.ptr(teth), None, None, None, 0) != 1:
+        raise RuntimeError("GmlxHilbertRenumber failed")
+    log("bandwidth: GPU Hilbert renumbering %.2fs" % (time.time() - t0))
+    del crdr, refr, tetr
+    th, sh = bandwidth_pass(G, crdh, refh, teth, args.steps, args.warmup)
+    res["hilbert"] = (th, sh)
+
+    def summary(t, score):
+        k = {name: {"avg_us": round(v * 1e6, 2), "alg_GBs": round(alg[name] / v / 1e9, 1),
+                    "frac": round(alg[name] / v / 1e9 / HBM_PEAK_GBS, 4)} for name, v in t.items()}
+        direct = t["direct_vertices"] + t["direct_tets"]
+        refm = {"direct_GBs": round((32 * V + 32 * T) / direct / 1e9, 1),
+                "indirect_GBs": round(96 * T / t["indirect_tets"] / 1e9, 1),
+                "indirect_unique_GBs": round((16 * V + 32 * T) / t["indirect_tets"] / 1e9, 1),
+                "ragged_GBs": round(460 * V / t["vertices_ball"] / 1e9, 1),
+                "ragged_unique_GBs": round((32 * V + 16 * T) / t["vertices_ball"] / 1e9, 1)}
+        return {"kernels": k, "reference_byte_model": refm, "evaluate_numbering": round(score, 4),
+                "GBs": round(sum(alg.values()) / sum(t.values()) / 1e9, 1)}
+
+    outr, outh = summary(*res["random"]), summary(*res["hilbert"])
+    log("random numbering: %s" % outr)
+    log("hilbert numbering: %s" % outh)
+    dom = max(th, key=th.get)
+    kg = alg[dom] / th[dom] / 1e9
+    step = sum(th.values())
+    out = {
+        "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
+        "value": outh["GBs"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered Kuhn cube mesh, randomly renumbered, then Hilbert-renumbered on the GPU)",
+        "config": {"workload": "GpuBandwidth access patterns through the generated kernels (direct vertex and tet "
+                               "copies, indirect tet -> vertex gather, ragged vertex -> tet ball gather), "
+                               "%d^3-cell Kuhn cube (%d vertices, %d tets)" % (n, V, T),
+                   "vertices": V, "tets": T, "bytes_per_step": sum(alg.values())},
+        "hilbert": outh,
+        "random": outr,
+        "roofline": {"bound": "hbm", "achieved": round(kg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kg / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "bytes_per_launch": alg[dom], "avg_us": round(th[dom] * 1e6, 2)},
+        "cpu_baseline": None,
+    }
+    tf = os.path.join(ROOT, "profiles", "traffic_bandwidth256.json")
+    if n == 256 and os.path.exists(tf):
+        out["roofline"]["traffic"] = json.load(open(tf)).get(dom)
+    if args.cpu_seconds > 0:
+        # the oracle's OpenMP loops of the same patterns on the Hilbert-numbered mesh
+        from tests import oracle_lib as O
+        OL = O.lib()
+        deg, ball, _ = O.vertex_ball(teth, V, 32)
+        tdat = np.empty((T, 4), np.float32)
+        vdat = np.empty((V, 4), np.float32)
+        loops = [("direct_vertices", lambda: np.copyto(vdat, crdh)),
+                 ("direct_tets", lambda: OL.orc_bw_direct_tets(T, O.p(teth), O.p(tdat))),
+                 ("indirect_tets", lambda: OL.orc_bw_indirect_tets(T, O.p(teth), O.p(crdh), O.p(tdat))),
+                 ("vertices_ball", lambda: OL.orc_bw_vertices_ball(V, O.p(deg), O.p(ball), 32, O.p(tdat), O.p(vdat)))]
+        ct, ns = 0.0, 0
+        while ns < 2 or ct < args.cpu_seconds:
+            for name, f in loops:
+                t0 = time.perf_counter()
+                f()
+                ct += time.perf_counter() - t0
+            ns += 1
+        out["cpu_baseline"] = {"value": round(ns * sum(alg.values()) / ct / 1e9, 2), "unit": "GB/s",
+                               "cores": OL.orc_num_threads(), "kind": "port",
+                               "sample": "%d passes of the four patterns over the same Hilbert-numbered %d^3 mesh, "
+                                         "oracle C + OpenMP (direct vertex copy: numpy), %.2f s" % (ns, n, ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 

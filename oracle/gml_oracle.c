@@ -709,6 +709,60 @@ long long orc_extract_items(int deg, int nsrc, const long long *ne, const int *s
    return nout;
 }
 
+/* ------------------------------------------------------------------------ */
+/* GpuBandwidth access patterns                                              */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * The bodies of examples/GpuBandwidth (driven by gpu_bandwidth.c:108-176),
+ * with the generated prologue's reads: direct_tets.cl converts each node
+ * index of a tet to float (TetDat = (float4){TetVer.s0..s3}); indirect_tets.cl
+ * sums the tet's four vertex coordinates left to right through the element
+ * table (VerCrd[k] = VerCrdTab[TetVer.sk], gmlib3.c:2777-2880); vertices_ball.cl
+ * sums TetDat over the vertex ball in ball order, slots past the degree
+ * reading the null value 0 (adding +0 leaves a sum that started at +0
+ * unchanged, so they are skipped here).  FP32, each addition rounded as
+ * written (-ffp-contract=off).
+ */
+void orc_bw_direct_tets(int ntet, const int *tet, float *out)
+{
+   long long i;
+#pragma omp parallel for schedule(static)
+   for(i = 0; i < 4LL * ntet; i++)
+      out[i] = (float)tet[i];
+}
+
+void orc_bw_indirect_tets(int ntet, const int *tet, const float *crd, float *out)
+{
+   int i;
+#pragma omp parallel for schedule(static)
+   for(i = 0; i < ntet; i++)
+   {
+      const int *t = &tet[(size_t)i * 4];
+      int c;
+      for(c = 0; c < 4; c++)
+         out[(size_t)i * 4 + c] = ((crd[(size_t)t[0] * 4 + c] + crd[(size_t)t[1] * 4 + c]) + crd[(size_t)t[2] * 4 + c])
+                                  + crd[(size_t)t[3] * 4 + c];
+   }
+}
+
+void orc_bw_vertices_ball(int nver, const int *deg, const int *ball, int width, const float *tdat, float *out)
+{
+   int v;
+#pragma omp parallel for schedule(static)
+   for(v = 0; v < nver; v++)
+   {
+      float a[4] = {0.f, 0.f, 0.f, 0.f};
+      int k, c;
+      for(k = 0; k < deg[v] && k < width; k++)
+      {
+         const float *x = &tdat[(size_t)(ball[(size_t)v * width + k] >> 4) * 4];
+         for(c = 0; c < 4; c++) a[c] += x[c];
+      }
+      for(c = 0; c < 4; c++) out[(size_t)v * 4 + c] = a[c];
+   }
+}
+
 /* Thread count the OpenMP regions above use (for the CPU baseline record). */
 int orc_num_threads(void)
 {

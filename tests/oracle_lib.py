@@ -51,6 +51,9 @@ def lib():
             "orc_bsr_spmv_f32": (None, [_I, _I, _P, _P, _P, _P, _P]),
             "orc_extract_items": (C.c_longlong, [_I, _I, _P, _P, _P, _P, _P, _I, _P, C.c_longlong, _P, _P,
                                                   C.c_longlong]),
+            "orc_bw_direct_tets": (None, [_I, _P, _P]),
+            "orc_bw_indirect_tets": (None, [_I, _P, _P, _P]),
+            "orc_bw_vertices_ball": (None, [_I, _P, _P, _I, _P, _P]),
             "orc_num_threads": (_I, []),
         }
         for n, (r, a) in sigs.items():
@@ -371,3 +374,25 @@ def heat_iterations(crd, tet, tri, triref, bc, niter, sol=None, dt=None):
         u = u + dt * rhs
         res.append(float((rhs * rhs).sum()))
     return u, res, dt
+
+
+def bw_direct_tets(tet):
+    tet = np.ascontiguousarray(tet, np.int32)
+    out = np.empty((len(tet), 4), np.float32)
+    lib().orc_bw_direct_tets(len(tet), p(tet), p(out))
+    return out
+
+
+def bw_indirect_tets(tet, crd):
+    tet = np.ascontiguousarray(tet, np.int32)
+    crd = np.ascontiguousarray(crd, np.float32)
+    out = np.empty((len(tet), 4), np.float32)
+    lib().orc_bw_indirect_tets(len(tet), p(tet), p(crd), p(out))
+    return out
+
+
+def bw_vertices_ball(deg, ball, tdat):
+    tdat = np.ascontiguousarray(tdat, np.float32)
+    out = np.empty((len(deg), 4), np.float32)
+    lib().orc_bw_vertices_ball(len(deg), p(deg), p(ball), ball.shape[1], p(tdat), p(out))
+    return out
