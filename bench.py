@@ -1026,9 +1026,10 @@ def bandwidth_models(V, T):
     return alg
 
 
-def bandwidth_pass(G, crd, ref, tet, steps, warmup):
+def bandwidth_pass(G, crd, ref, tet, steps, warmup, tag=""):
     """the four GpuBandwidth kernels on one handle: per-kernel average
-    seconds (HIP events on the library's stream), GmlEvaluateNumbering"""
+    seconds (HIP events on the library's stream), GmlEvaluateNumbering;
+    tag is appended to the kernel names (profiles tell the passes apart)"""
     R, W = G.GmlReadMode, G.GmlWriteMode
     g = G.Gml(0)
     try:
@@ -1036,13 +1037,13 @@ def bandwidth_pass(G, crd, ref, tet, steps, warmup):
         score = g.L.GmlEvaluateNumbering(g.h)
         vdat = g.solution(G.GmlVertices, 1, G.GmlFlt4, "VerDat")
         tdat = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "TetDat")
-        ks = {"direct_vertices": g.compile(body("bw_direct_vertices"), "direct_vertices", G.GmlVertices,
+        ks = {"direct_vertices": g.compile(body("bw_direct_vertices"), "direct_vertices" + tag, G.GmlVertices,
                                            [(vi, R, 0), (vdat, W, 0)]),
-              "direct_tets": g.compile(body("bw_direct_tets"), "direct_tets", G.GmlTetrahedra,
+              "direct_tets": g.compile(body("bw_direct_tets"), "direct_tets" + tag, G.GmlTetrahedra,
                                        [(ti, R, 0), (tdat, W, 0)]),
-              "indirect_tets": g.compile(body("bw_indirect_tets"), "indirect_tets", G.GmlTetrahedra,
+              "indirect_tets": g.compile(body("bw_indirect_tets"), "indirect_tets" + tag, G.GmlTetrahedra,
                                          [(vi, R, 0), (tdat, W, 0)]),
-              "vertices_ball": g.compile(body("bw_vertices_ball"), "vertices_ball", G.GmlVertices,
+              "vertices_ball": g.compile(body("bw_vertices_ball"), "vertices_ball" + tag, G.GmlVertices,
                                          [(vdat, W, 0), (tdat, R, 0)])}
         t = {}
         for name, k in ks.items():           # the example's order: direct, indirect, ragged
@@ -1080,7 +1081,7 @@ def run_bandwidth(args):
     log("bandwidth: %d^3 cube (V=%d, T=%d), random numbering in %.1fs" % (n, V, T, time.time() - t0))
     alg = bandwidth_models(V, T)
     res = {}
-    tr, sr = bandwidth_pass(G, crdr, refr, tetr, args.steps, args.warmup)
+    tr, sr = bandwidth_pass(G, crdr, refr, tetr, args.steps, args.warmup, "_random")
     res["random"] = (tr, sr)
     crdh, refh, teth = crdr.copy(), refr.copy(), tetr.copy()
     t0 = time.time()
