@@ -4,8 +4,9 @@ The same body compiled with staging (the default) and without must give
 bit-identical results: staging changes where the uplink data are read from
 (a per-tile LDS copy loaded with coalesced loads), not which values are read
 or in which order.  Covered: float4 and float uplink data, a body that also
-names the ball entries themselves (they are then read as well), and a random
-tet soup whose high-degree rows run in the unstaged twin kernel."""
+names the ball entries themselves (they are then read as well), a random
+tet soup whose high-degree rows run in the unstaged twin kernel, and
+isolated vertices filling whole tiles with nothing to stage."""
 import ctypes as C
 import os
 
@@ -126,3 +127,30 @@ def test_staged_with_high_degree_twin():
     # and both equal the brute-force ball minimum
     want = np.array([q[np.any(tet == v, axis=1)].min() if np.any(tet == v) else 0. for v in range(nv)], np.float32)
     assert np.array_equal(b[1][0], want)
+
+
+def test_staged_isolated_vertices_and_empty_tiles():
+    # vertices in no tet after a small cube: whole tiles of rows without
+    # ball entries (no staged entities), a partial last tile, rows past the
+    # end reading the last row before the barrier; every vertex gets the sum
+    # of its ball in ball order (0 for an isolated one), staged or not
+    from tests import oracle_lib as O
+    crd, ref, tet = G.cube_mesh(6, 0.2, 13)
+    extra = 600
+    rng = np.random.default_rng(3)
+    iso = np.zeros((extra, 4), np.float32)
+    iso[:, :3] = rng.uniform(2., 3., size=(extra, 3))
+    crd = np.concatenate([crd, iso])
+    ref = np.concatenate([ref, np.zeros(extra, np.int32)])
+    V = len(crd)
+    with G.Gml() as g:
+        vi, bary, qal = mesh_fields(g, crd, ref, tet)
+        out = g.solution(G.GmlVertices, 1, G.GmlFlt4, "Out")
+        ks = both(g, SUM_BODY, "sumbary", G.GmlVertices, [(bary, G.GmlReadMode, 0), (out, G.GmlWriteMode, 0)])
+        a = run_pair(g, ks, [out], [(np.float32, (V, 4))])
+        b4 = g.download_raw(bary, np.float32, (len(tet), 4))
+    assert np.array_equal(a[0][0], a[1][0])
+    deg, ball, _ = O.vertex_ball(tet, V, 32)
+    assert (deg[-extra:] == 0).all()
+    assert np.array_equal(a[1][0], O.bw_vertices_ball(deg, ball, b4))
+    assert (a[1][0][-extra:] == 0).all()
