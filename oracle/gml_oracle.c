@@ -7,7 +7,8 @@
  *
  * Each function cites the reference lines whose behaviour it restates
  * (paths relative to the GMlib source tree).  Every function is pinned by
- * tests/test_oracle_*.py against closed forms, brute force on tiny meshes,
+ * the CPU tests (tests/test_oracle_*.py, test_extract.py, test_bandwidth.py)
+ * against closed forms, brute force on tiny meshes,
  * invariants and an FP64 restatement (there are no reference-held golden
  * vectors for this path: the reference ships no tests and its tet sample
  * meshes are missing).
