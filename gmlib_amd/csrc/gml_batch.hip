@@ -111,24 +111,51 @@ __global__ void __launch_bounds__(kBatBlock) k_batch_quality(
    {
       const unsigned int v0 = vofs[t], nv = vofs[t + 1] - v0;
       const unsigned int t0 = tofs[t], nt = tofs[t + 1] - t0;
-      for(unsigned int i = threadIdx.x; i < nv; i += kBatBlock)
-         X[i] = crd[vlist[v0 + i]];
+      // the tile's coordinates: four per thread and pass, the four index
+      // loads, then the four gathers, in flight together
+      for(unsigned int i = threadIdx.x; i < nv; i += 4 * kBatBlock)
+      {
+         int vi[4];
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+            vi[u] = i + u * kBatBlock < nv ? vlist[v0 + i + u * kBatBlock] : -1;
+         v4f xv[4];
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+            xv[u] = vi[u] >= 0 ? crd[vi[u]] : (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+            if(vi[u] >= 0) X[i + u * kBatBlock] = xv[u];
+      }
       if(threadIdx.x == 0) Q[nt] = FLT_MAX;
       __syncthreads();
 
+      // tet qualities, four tets per thread and pass (their local rows
+      // loaded together); each thread's own tets are summed in index order
       float mn = FLT_MAX;
       double sm = 0.;
       const unsigned int q0 = tbeg[t], nown = tbeg[t + 1] - q0;
-      for(unsigned int j = threadIdx.x; j < nt; j += kBatBlock)
+      for(unsigned int j = threadIdx.x; j < nt; j += 4 * kBatBlock)
       {
-         const v4u16 a = tloc[t0 + j];
-         const float q = meanratio3(X[a.x], X[a.y], X[a.z], X[a.w]);
-         Q[j] = q;
-         if(j < nown)
+         v4u16 a[4];
+#pragma unroll
+         for(int u = 0; u < 4; u++)
+            a[u] = j + u * kBatBlock < nt ? tloc[t0 + j + u * kBatBlock] : (v4u16){0, 0, 0, 0};
+#pragma unroll
+         for(int u = 0; u < 4; u++)
          {
-            tq[q0 + j] = q;
-            mn = fminf(mn, q);
-            sm += (double)q;
+            const unsigned int jj = j + u * kBatBlock;
+            if(jj < nt)
+            {
+               const float q = meanratio3(X[a[u].x], X[a[u].y], X[a[u].z], X[a[u].w]);
+               Q[jj] = q;
+               if(jj < nown)
+               {
+                  tq[q0 + jj] = q;
+                  mn = fminf(mn, q);
+                  sm += (double)q;
+               }
+            }
          }
       }
       __syncthreads();
@@ -139,14 +166,16 @@ __global__ void __launch_bounds__(kBatBlock) k_batch_quality(
          const unsigned int v = vb + i;
          const int d = ldeg[v];
          const unsigned short *row = lball + (size_t)loff[c0 + (i >> 6)] * 64 + (i & 63);
+         // the padded ball row in groups of 16 slots loaded together (a
+         // slot past the row reads the sentinel, whose value is FLT_MAX)
          float m = FLT_MAX;
-         for(int k = 0; k < d; k += PAD)
+         for(int k = 0; k < d; k += 16)
          {
-            unsigned short s[PAD];
+            unsigned short sl[16];
 #pragma unroll
-            for(int u = 0; u < PAD; u++) s[u] = row[(k + u) * 64];
+            for(int u = 0; u < 16; u++) sl[u] = k + u < d ? row[(k + u) * 64] : (unsigned short)nt;
 #pragma unroll
-            for(int u = 0; u < PAD; u++) m = fminf(m, Q[s[u]]);
+            for(int u = 0; u < 16; u++) m = fminf(m, Q[sl[u]]);
          }
          vmin[v] = d > 0 ? m : 0.f;
       }
