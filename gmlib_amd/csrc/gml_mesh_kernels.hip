@@ -123,10 +123,24 @@ __global__ void __launch_bounds__(1024) k_ball_min(long long nv, const int *__re
    if(v >= nv) return;
    const int d = deg[v];
    const int *row = ent + (size_t)off[v >> 6] * 64 + (v & 63);
-   float m = d > 0 ? val[row[0] >> 4] : 0.f;
-   for(int k = 1; k < d; k++)
-      m = fminf(m, val[row[k * 64] >> 4]);
-   out[v] = m;
+   // slots 8 at a time: the 8 entry loads, then the 8 value gathers.  NaN
+   // is fminf's neutral element (fminf(NaN, x) = x), so starting from it and
+   // padding with it gives the first value's then the running minimum, as
+   // a loop from slot 0
+   const float nanv = __builtin_nanf("");
+   float m = nanv;
+   for(int k = 0; k < d; k += 8)
+   {
+      int e[8];
+#pragma unroll
+      for(int i = 0; i < 8; i++) e[i] = k + i < d ? __builtin_nontemporal_load(&row[(k + i) * 64]) >> 4 : -1;
+      float x[8];
+#pragma unroll
+      for(int i = 0; i < 8; i++) x[i] = e[i] >= 0 ? val[e[i]] : nanv;
+#pragma unroll
+      for(int i = 0; i < 8; i++) m = fminf(m, x[i]);
+   }
+   out[v] = d > 0 ? m : 0.f;
 }
 
 // Fused Jacobi sweep over one tile of TV = blockDim.x vertices (see the
