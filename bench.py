@@ -61,7 +61,16 @@ counted as 120 B per tet + 72 B per face + 32 B per vertex.  The CPU
 baseline is the numpy FP64 restatement of the same iteration on a smaller
 cube.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract|heat]
+Seventh workload (--workload bandwidth): the reference's GpuBandwidth
+example -- four one-line bodies through the generated kernels: a vertex copy
+(32 B per vertex), the tet indices as floats (32 B per tet), the indirect
+tet -> vertex gather (16 B per vertex + 32 B per tet) and the ragged vertex
+-> tet ball gather (32 B per tet + 20 B per vertex) -- on the 256^3 cube
+numbered at random, then after GmlxHilbertRenumber; the example's own byte
+figures are printed beside them.  The CPU baseline is the oracle's OpenMP
+loops of the same four patterns on the renumbered mesh.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract|heat|bandwidth]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
