@@ -910,6 +910,11 @@ def run_extract(args):
                      "bytes_per_launch": alg, "avg_us": round(dt * 1e6, 1)},
         "cpu_baseline": None,
     }
+    # bytes moved per step by all the step's kernels, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
+    tf = os.path.join(ROOT, "profiles", "traffic_extract256.json")
+    if n == 256 and os.path.exists(tf):
+        out["roofline"]["traffic"] = json.load(open(tf)).get("extract_step")
     g.close()
     if args.cpu_seconds > 0:
         from tests import oracle_lib as O
