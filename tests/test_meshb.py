@@ -228,3 +228,31 @@ def test_export_solution_binary_and_ascii(tmp_path):
     j = txt.index("ReferenceStrings")
     assert txt[j + 1] == "7" and txt[j + 2:j + 6] == ["66", "1", "qal", "1"]
     assert "vec.x" in txt and "grd.y" in txt
+
+
+SAMPLES = "/root/reference/sample_meshes"
+
+
+@pytest.mark.skipif(not os.path.isdir(SAMPLES), reason="reference tree not present")
+@pytest.mark.parametrize("name,kinds", [("edges", ("Edges",)), ("triangles", ("Triangles",))])
+def test_import_reference_3d_sample_meshes(tmp_path, name, kinds):
+    # the reference's other sample meshes (version 1, dimension 3), read
+    # where they lie and compared field by field with the independent parse
+    data = lzma.open(os.path.join(SAMPLES, name + ".meshb.xz")).read()
+    ver, dim, e, kw = read_meshb(data)
+    assert (ver, dim) == (1, 3) and set(kw) == {"Vertices"} | set(kinds)
+    path = str(tmp_path / (name + ".meshb"))
+    open(path, "wb").write(data)
+    with G.Gml(host_only=True) as g:
+        assert imp(g, path, GMF["Vertices"], *[GMF[k] for k in kinds]) == 1 + len(kinds)
+        t = mesh_tables(g)
+    n, at = kw["Vertices"]
+    v = np.frombuffer(data, np.dtype([("x", e + "f4"), ("y", e + "f4"), ("z", e + "f4"), ("r", e + "i4")]), n, at)
+    assert np.array_equal(t["Vertices"][0][:, 0], v["x"]) and np.array_equal(t["Vertices"][0][:, 2], v["z"])
+    assert (t["Vertices"][0][:, 3] == 0).all() and np.array_equal(t["Vertices"][1], v["r"])
+    for k in kinds:
+        nn = {"Edges": 2, "Triangles": 3}[k]
+        m, at = kw[k]
+        a = np.frombuffer(data, np.dtype(e + "i4"), m * (nn + 1), at).reshape(m, nn + 1)
+        assert np.array_equal(t[k][0], a[:, :nn] - 1) and np.array_equal(t[k][1], a[:, nn])
+        assert t[k][0].min() >= 0 and t[k][0].max() < n
