@@ -256,3 +256,31 @@ def test_import_reference_3d_sample_meshes(tmp_path, name, kinds):
         a = np.frombuffer(data, np.dtype(e + "i4"), m * (nn + 1), at).reshape(m, nn + 1)
         assert np.array_equal(t[k][0], a[:, :nn] - 1) and np.array_equal(t[k][1], a[:, nn])
         assert t[k][0].min() >= 0 and t[k][0].max() < n
+
+
+@pytest.mark.skipif(not os.path.isdir(SAMPLES), reason="reference tree not present")
+def test_extract_edges_of_reference_surface_mesh(tmp_path):
+    # GmlExtractEdges on the reference's triangle sample (a host-only handle:
+    # the ordered-map form) against the oracle's restatement of the
+    # reference hash: the same edges in the same order; every edge of a
+    # closed manifold surface is shared by two triangles (E = 3T / 2)
+    from tests import oracle_lib as O
+    data = lzma.open(os.path.join(SAMPLES, "triangles.meshb.xz")).read()
+    path = str(tmp_path / "triangles.meshb")
+    open(path, "wb").write(data)
+    with G.Gml(host_only=True) as g:
+        assert imp(g, path, GMF["Vertices"], GMF["Triangles"]) == 2
+        t = mesh_tables(g)
+        ne = g.L.GmlExtractEdges(g.h)
+        assert ne > 0
+        tab = mesh_tables(g)["Edges"]
+    tri = t["Triangles"][0]
+    want, wref = O.extract_items(2, [(G.GmlTriangles, tri)])
+    assert np.array_equal(tab[0], want) and ne == len(want) and (tab[1] == 0).all()
+    # the same edge set from a plain numpy count (each triangle side once)
+    V = len(t["Vertices"][0])
+    sides = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1).astype(np.int64)
+    keys, cnt = np.unique(sides[:, 0] * V + sides[:, 1], return_counts=True)
+    assert len(keys) == ne
+    assert np.array_equal(np.sort(want[:, 0].astype(np.int64) * V + want[:, 1]), keys)
+    assert (cnt == 2).all() and 2 * ne == 3 * len(tri)       # a closed surface
