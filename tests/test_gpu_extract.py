@@ -227,3 +227,28 @@ def test_gpu_extract_small_and_empty_meshes():
             assert g.L.GmlExtractEdges(g.h) == 0 and g.L.GmlExtractFaces(g.h) == 0
             assert not g.L.GmlGetMeshInfo(g.h, G.GmlEdges, None, None)
             assert not g.L.GmlGetMeshInfo(g.h, G.GmlTriangles, None, None)
+
+
+def test_gpu_extract_reference_sample_mesh(tmp_path):
+    # the reference's naca012 sample (36,624 vertices, 72,783 triangles and
+    # 465 boundary edges with refs), read on a device handle: the GPU
+    # extraction keeps the file's edges first with their refs, then the
+    # triangles' other edges in first-appearance order, as the oracle's
+    # restatement of the reference hash
+    import lzma
+    import os
+    from tests.test_meshb import GMF, imp, mesh_tables
+    data = lzma.open(os.path.join(os.path.dirname(__file__), "golden", "naca012_2D_triangles.meshb.xz")).read()
+    path = str(tmp_path / "naca.meshb")
+    open(path, "wb").write(data)
+    with G.Gml() as g:
+        assert imp(g, path, GMF["Vertices"], GMF["Edges"], GMF["Triangles"]) == 3
+        t = mesh_tables(g)
+        ne = g.L.GmlExtractEdges(g.h)
+        got = mesh_tables(g)["Edges"]
+    old, oref = t["Edges"]
+    tri = t["Triangles"][0]
+    want, wref = O.extract_items(2, [(G.GmlTriangles, tri)], old, oref)
+    assert ne == len(want) and np.array_equal(got[0], want) and np.array_equal(got[1], wref)
+    assert np.array_equal(got[0][:len(old)], np.sort(old, axis=1))          # the file's edges, nodes sorted
+    assert (got[1][:len(old)] == oref).all() and (got[1][len(old):] == 0).all()
