@@ -112,3 +112,34 @@ def test_gpu_links_large_cube_closed_forms():
     assert (back == t[inner][:, None]).any(axis=1).all()
     # zero entries: the 12 n^2 boundary faces, plus one per tet across a face from tet 0
     assert (k == 0).sum() == 12 * n * n + (ngb[0] != 0).sum()
+
+
+def test_gpu_links_reference_sample_mesh(tmp_path):
+    # the reference's naca012 sample (72,783 triangles, its boundary edges
+    # with refs, then the extracted interior edges): edge -> triangle shells,
+    # triangle -> edge downlinks and triangle neighbours, device against the
+    # host-only builders
+    import lzma
+    import os
+    from tests.test_meshb import GMF, imp, mesh_tables
+    data = lzma.open(os.path.join(os.path.dirname(__file__), "golden", "naca012_2D_triangles.meshb.xz")).read()
+    path = str(tmp_path / "naca.meshb")
+    open(path, "wb").write(data)
+    specs = [("up", EDG, TRI), ("down", TRI, EDG), ("ngb", TRI, TRI)]
+
+    def sample(g, n):
+        assert imp(g, path, GMF["Vertices"], GMF["Edges"], GMF["Triangles"]) == 3
+        assert g.L.GmlExtractEdges(g.h) == 109407
+        t = mesh_tables(g)
+        return {EDG: t["Edges"][0], TRI: t["Triangles"][0]}
+
+    dev, _ = links(False, sample, 0, specs)
+    host, _ = links(True, sample, 0, specs)
+    for k in specs:
+        key = k if k[0] != "ngb" else ("ngb", k[1], k[1])
+        a, b = dev[key], host[key]
+        if k[0] == "up":
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), k
+            assert a[0].min() >= 1 and a[0].max() == 2           # a boundary edge has one triangle
+        else:
+            assert np.array_equal(a, b), k
