@@ -94,6 +94,15 @@ def body(name):
     return open(os.path.join(ROOT, "gmlib_amd", "kernels", name + ".cl")).read()
 
 
+def new_handle(args):
+    """A device handle on GPU 0 with the --staging choice applied."""
+    import gmlib_amd as G
+    g = G.Gml(0)
+    if args.staging >= 0:
+        g.L.GmlxSetStaging(g.h, args.staging)
+    return g
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract", "heat", "bandwidth"])
@@ -124,9 +133,6 @@ def main():
                          "written), to calibrate HBM byte counters")
     args = ap.parse_args()
 
-    if args.staging >= 0:
-        import gmlib_amd as G
-        G.lib().GmlxSetStaging(args.staging)
     if args.gpus != 1:
         print("bench.py measures one GPU: the path is single-device (multi-GPU sharding is a design "
               "note only); run with --gpus 1", file=sys.stderr)
@@ -157,7 +163,7 @@ def run_smoothing(args):
     V, T = len(crd), len(tet)
     log("mesh %d^3: V=%d T=%d built in %.1fs" % (n, V, T, time.time() - t0))
 
-    g = G.Gml(0)
+    g = new_handle(args)
     L, h = g.L, g.h
     t0 = time.time()
     vi, ti = g.set_mesh(crd, ref, tet)
@@ -214,10 +220,10 @@ def run_smoothing(args):
         return [L.GmlxGetBuiltinRunTime(h, w, C.byref(cnt))
                 for w in (G.GmlxBuiltinSweep, G.GmlxBuiltinQuality)]
 
-    L.GmlxSetBuiltinShape(G.GmlxBuiltinSweep, args.sweep_tile, 1)
+    L.GmlxSetBuiltinShape(h, G.GmlxBuiltinSweep, args.sweep_tile, 1)
     if args.quality_shape:
         qg, qp = (int(x) for x in args.quality_shape.split("/"))
-        if not (L.GmlxSetBuiltinShape(G.GmlxBuiltinQuality, qg, 1) and L.GmlxSetBuiltinItems(G.GmlxBuiltinQuality, qp)):
+        if not (L.GmlxSetBuiltinShape(h, G.GmlxBuiltinQuality, qg, 1) and L.GmlxSetBuiltinItems(h, G.GmlxBuiltinQuality, qp)):
             raise SystemExit("invalid --quality-shape %s" % args.quality_shape)
     paths = {}
     for path, step in (("generated", step_generated), ("builtin", step_builtin)):
@@ -351,7 +357,7 @@ def run_cg(args):
         % (n, nv, nb, nb / nv, time.time() - t0))
     spmv_bytes = 40 * nb + 28 * nv
 
-    g = G.Gml(0)
+    g = new_handle(args)
     L, h = g.L, g.h
     m = L.GmlNewMatrix(h, nv, nb, 3, G.ptr(val), G.ptr(col), G.ptr(rp), G.GmlFlt)
     vx = L.GmlNewVector(h, nv, 3, G.ptr(rhs), G.GmlFlt)
@@ -387,7 +393,7 @@ def run_cg(args):
     shapes = {}
     for grp in (64, 128, 256):
         for rm in (0, 1):
-            L.GmlxSetBuiltinShape(G.GmlxBuiltinSpMV, grp, rm)
+            L.GmlxSetBuiltinShape(h, G.GmlxBuiltinSpMV, grp, rm)
             time_spmv(2)
             shapes["%d/%d" % (grp, rm)] = time_spmv(5)[1]
     best = min(shapes, key=shapes.get)
@@ -395,7 +401,7 @@ def run_cg(args):
     log("SpMV shapes (group/remap: kernel us): %s -> %s"
         % ({k: round(v * 1e6, 1) for k, v in shapes.items()}, best))
     for w in (G.GmlxBuiltinSpMV, G.GmlxBuiltinCgSpMV):
-        L.GmlxSetBuiltinShape(w, grp, rm)
+        L.GmlxSetBuiltinShape(h, w, grp, rm)
     for _ in range(args.warmup):
         L.GmlMultMatVec(h, m, vy, vx)
     dt, kavg = time_spmv(args.steps)
@@ -712,7 +718,7 @@ def run_batch(args):
     log("batch: %d meshes, n in [%d, %d], V=%d T=%d, generated in %.1fs" % (M, sizes.min(), sizes.max(), V, T, time.time() - t0))
     alg = 36 * T + 20 * V
 
-    g = G.Gml(0)
+    g = new_handle(args)
     L, h = g.L, g.h
     t0 = time.time()
     b = L.GmlxNewBatch(h, M, G.ptr(nver), G.ptr(ntet), G.ptr(crd), G.ptr(tet))
@@ -847,7 +853,7 @@ def run_extract(args):
     n = args.n
     crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
     V, T = len(crd), len(tet)
-    g = G.Gml(0)
+    g = new_handle(args)
     L, h = g.L, g.h
     g.set_mesh(crd, ref, tet)
     L.GmlxSync(h)
@@ -1045,7 +1051,7 @@ def bandwidth_pass(G, crd, ref, tet, steps, warmup, tag=""):
     seconds (HIP events on the library's stream), GmlEvaluateNumbering;
     tag is appended to the kernel names (profiles tell the passes apart)"""
     R, W = G.GmlReadMode, G.GmlWriteMode
-    g = G.Gml(0)
+    g = new_handle(args)
     try:
         vi, ti = g.set_mesh(crd, ref, tet)
         score = g.L.GmlEvaluateNumbering(g.h)

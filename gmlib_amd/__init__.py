@@ -92,7 +92,7 @@ _SIGS = {
     "GmlxGetStream": (_P, [_S]),
     "GmlxInvalidateHost": (_I, [_S, _I]),
     "GmlxBuildLink": (_I, [_S, _I, _I]),
-    "GmlxSetBuiltinItems": (_I, [_I, _I]),
+    "GmlxSetBuiltinItems": (_I, [_S, _I, _I]),
     "GmlxGetLinkData": (_I, [_S, _I, _I]),
     "GmlxGetUplinkTable": (_I, [_S, _I, _I, _P, _P, _I]),
     "GmlxCubeMesh": (_I, [_I, _D, C.c_ulonglong, _I, _P, _P, _P]),
@@ -105,9 +105,9 @@ _SIGS = {
     "GmlxTetQuality": (_I, [_S, _I]),
     "GmlxBallMin": (_I, [_S, _I, _I]),
     "GmlxSmoothSweeps": (_I, [_S, _I, _I, _D]),
-    "GmlxSetBuiltinShape": (_I, [_I, _I, _I]),
+    "GmlxSetBuiltinShape": (_I, [_S, _I, _I, _I]),
     "GmlxGetBuiltinRunTime": (_D, [_S, _I, C.POINTER(C.c_longlong)]),
-    "GmlxSetSmoothMode": (_I, [_I]),
+    "GmlxSetSmoothMode": (_I, [_S, _I]),
     "GmlxGetSweepPlanInfo": (_I, [_S, C.POINTER(C.c_longlong)]),
     "GmlxNewBatch": (_I, [_S, _I, _P, _P, _P, _P]),
     "GmlxBatchTune": (_I, [_S, _I, _IP, _IP]),
@@ -123,9 +123,10 @@ _SIGS = {
     "GmlxServeWait": (_I, [_S, C.c_longlong, _DP, _DP]),
     "GmlxServeAddMesh": (C.c_longlong, [_S, _I, _I, _P, _P, _P]),
     "GmlxServeSubmitMesh": (C.c_longlong, [_S, _I, C.c_longlong, _D]),
+    "GmlxServeRemoveMesh": (_I, [_S, C.c_longlong]),
     "GmlxServeInfo": (_I, [_S, _DP]),
     "GmlxServeStop": (None, [_S]),
-    "GmlxSetStaging": (_I, [_I]),
+    "GmlxSetStaging": (_I, [_S, _I]),
     "GmlxHilbertRenumber": (_I, [_I, _P, _P, _I, _I, _P, _P, _P, _P, _I]),
 }
 
@@ -329,6 +330,18 @@ class Gml:
         self.L.GmlxSync(self.h)
         _hip_d2h(out, dp)
         return out
+
+    def upload(self, idx, a):
+        """Copy a numpy array into a data item's device memory (its byte
+        size at most the item's) and mark the host mirror stale."""
+        a = np.ascontiguousarray(a)
+        nb = C.c_size_t(0)
+        dp = self.L.GmlxGetDevicePointer(self.h, idx, C.byref(nb))
+        if not dp or a.nbytes > nb.value:
+            raise RuntimeError("upload of %d bytes into data %d (%d bytes)" % (a.nbytes, idx, nb.value))
+        self.L.GmlxSync(self.h)
+        _hip_h2d(dp, a)
+        self.L.GmlxInvalidateHost(self.h, idx)
 
     def vertices(self, nv):
         crd = np.zeros((nv, 4), np.float32)   # GmlGetDataBlock writes x, y, z; w stays 0

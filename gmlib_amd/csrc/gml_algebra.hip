@@ -214,9 +214,10 @@ static unsigned int Blocks(long long n, int b = 256) { return (unsigned int)((n 
 
 // y = A x with A's values of type T and vectors of type X
 template<class T, class X>
-static int SpMV(hipStream_t s, const Matrix *m, const X *x, X *y)
+static int SpMV(const GmlCtx *g, const Matrix *m, const X *x, X *y)
 {
-   const Shape sh = BuiltinShape(GmlxBuiltinSpMV);
+   hipStream_t s = g->stream;
+   const Shape sh = BuiltinShape(g, GmlxBuiltinSpMV);
    const dim3 nb(Blocks(m->NmbLin, sh.grp)), gs(sh.grp);
    const int *deg = m->deg, *col = m->col;
    const unsigned int *off = m->off;
@@ -726,9 +727,9 @@ int GmlMultMatVec(size_t GmlIdx, int MatIdx, int VecIdx1, int VecIdx2)
    if(m->flt == GmlDbl && x->flt != GmlDbl) return -4;
    int rc = 1;
    int tl = TimedLaunch(g, GmlxBuiltinSpMV, [&]() {
-      if(m->flt == GmlFlt && x->flt == GmlFlt) rc = SpMV<float, float>(g->stream, m, (const float *)x->dev, (float *)y->dev);
-      else if(m->flt == GmlFlt)                rc = SpMV<float, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
-      else                                     rc = SpMV<double, double>(g->stream, m, (const double *)x->dev, (double *)y->dev);
+      if(m->flt == GmlFlt && x->flt == GmlFlt) rc = SpMV<float, float>(g, m, (const float *)x->dev, (float *)y->dev);
+      else if(m->flt == GmlFlt)                rc = SpMV<float, double>(g, m, (const double *)x->dev, (double *)y->dev);
+      else                                     rc = SpMV<double, double>(g, m, (const double *)x->dev, (double *)y->dev);
    });
    if(tl != 1) return tl;
    if(rc != 1) return rc;
@@ -909,7 +910,7 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
    const int nrow = A->NmbLin, B = A->b;
    const long long n = (long long)nrow * B;
    hipStream_t s = g->stream;
-   const Shape sh = BuiltinShape(GmlxBuiltinCgSpMV), su = BuiltinShape(GmlxBuiltinCgUpdate);
+   const Shape sh = BuiltinShape(g, GmlxBuiltinCgSpMV), su = BuiltinShape(g, GmlxBuiltinCgUpdate);
    const unsigned int nbs = Blocks(nrow, sh.grp);
    CgWork *ws = CgWorkspace(g, A);
    if(!ws) return -5;

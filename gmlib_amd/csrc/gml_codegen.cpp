@@ -228,8 +228,6 @@ static std::string ArgDecl(const Arg &a)
 // tile-local index (StagePlan).  Values and their order are those of the
 // direct gather; only where the bytes come from changes.
 // ---------------------------------------------------------------------------
-int g_stage_uplinks = 1;
-
 // the ball whose data are staged (the first ball with stageable data), or
 // -1; *staged receives the staged data arguments
 static int StagedArgs(const std::vector<Arg> &args, std::vector<int> *staged)
@@ -1053,7 +1051,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
    bool any_lazy = false;
    for(auto &a : args) if(a.CntIdx >= 0 && a.NmbItm > 1) any_lazy = true;
    std::vector<int> staged;
-   const int sball = g_stage_uplinks ? StagedArgs(args, &staged) : -1;
+   const int sball = g->stage_uplinks ? StagedArgs(args, &staged) : -1;
    std::string log, src;
    int KrnIdx = 0;
    for(int st = sball >= 0 ? 1 : 0; st >= 0 && !KrnIdx; st--)
@@ -1226,10 +1224,11 @@ double GmlGetKernelRunTime(size_t GmlIdx, int idx)
    return t;
 }
 
-int GmlxSetStaging(int on)
+int GmlxSetStaging(size_t GmlIdx, int on)
 {
-   const int old = g_stage_uplinks;
-   g_stage_uplinks = on ? 1 : 0;
+   GML_CTX(g, GmlIdx);
+   const int old = g->stage_uplinks;
+   g->stage_uplinks = on ? 1 : 0;
    return old;
 }
 

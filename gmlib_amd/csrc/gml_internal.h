@@ -133,6 +133,11 @@ struct SweepPlan {
 
 struct ScratchBlock { void *p; size_t bytes; bool busy; };
 
+// launch shape of a built-in kernel (work-group size, XCD block remap,
+// items per thread where a kernel takes several), per handle
+const int kNmbBuiltin = 16;
+struct Shape { int grp = 256, remap = 1, per = 2; };
+
 struct GmlCtx {
    int          dev = 0;
    hipStream_t  stream = nullptr;
@@ -170,6 +175,12 @@ struct GmlCtx {
    bool         host_only = false;
    // cached temporaries of the item kernels (gml_items.hip)
    std::vector<ScratchBlock> iscratch;
+   // tuning state of this handle: built-in launch shapes
+   // (GmlxSetBuiltinShape / Items), sweep form (GmlxSetSmoothMode), LDS
+   // staging of uplink data in kernels compiled later (GmlxSetStaging)
+   Shape        shape[kNmbBuiltin];
+   int          smooth_mode = 2;
+   int          stage_uplinks = 1;
 };
 
 // --- helpers ---------------------------------------------------------------
@@ -223,11 +234,8 @@ void FreeAlgebra(GmlCtx *g);
 
 void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
 
-// launch shape of a built-in kernel (work-group size, XCD block remap),
-// process-wide, set by GmlxSetBuiltinShape
-const int kNmbBuiltin = 16;
-struct Shape { int grp = 256, remap = 1, per = 2; };   // per: items per thread where a kernel takes several
-Shape BuiltinShape(int which);
+// launch shape of a built-in kernel on a handle
+inline Shape BuiltinShape(const GmlCtx *g, int which) { return g->shape[which]; }
 
 // Block index of this work-group in launch order, or, with remap, remapped
 // so that each of the 8 XCDs (which take dispatched blocks round-robin) runs
@@ -283,7 +291,6 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows);
 void FreeStagePlan(GmlCtx *g, StagePlan *p);
-extern int g_stage_uplinks;   // GmlxSetStaging
 
 // launcher (gml_codegen.cpp)
 int  RunKernel(GmlCtx *g, Kernel *k);

@@ -305,10 +305,6 @@ __global__ void __launch_bounds__(1024) k_sweep_op(long long nv, const unsigned 
    __builtin_nontemporal_store(p, &xn[v]);
 }
 
-static Shape g_shape[kNmbBuiltin];
-
-Shape BuiltinShape(int which) { return g_shape[which]; }
-static int g_smooth_mode = 2;
 
 static bool MeshReady(GmlCtx *g)
 {
@@ -329,7 +325,7 @@ int GmlxTetQuality(size_t GmlIdx, int QalIdx)
    || g->dat[QalIdx].NmbItm != 1 || g->dat[QalIdx].NmbLin != g->NmbEle[GmlTetrahedra])
       return -2;
    long long nt = g->NmbEle[GmlTetrahedra];
-   const Shape sh = g_shape[GmlxBuiltinQuality];
+   const Shape sh = g->shape[GmlxBuiltinQuality];
    const v4i *tet = (const v4i *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
    const v4f *crd = (const v4f *)g->dat[g->TypIdx[GmlVertices]].dev;
    float *q = (float *)g->dat[QalIdx].dev;
@@ -356,7 +352,7 @@ int GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx)
    if(!g->up[GmlVertices][GmlTetrahedra].BalIdx && !BuildUplink(g, GmlVertices, GmlTetrahedra)) return -3;
    Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
    long long nv = g->NmbEle[GmlVertices];
-   const Shape sh = g_shape[GmlxBuiltinBallMin];
+   const Shape sh = g->shape[GmlxBuiltinBallMin];
    o->host_valid = false;
    if(!nv) return 1;
    return TimedLaunch(g, GmlxBuiltinBallMin, [&]() {
@@ -382,9 +378,9 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    vd->host_valid = false;
    if(!nv || !nsweep) return 1;
 
-   if(g_smooth_mode == 2)
+   if(g->smooth_mode == 2)
    {
-      const Shape sf = g_shape[GmlxBuiltinSweep];
+      const Shape sf = g->shape[GmlxBuiltinSweep];
       if(g->oplan && g->oplan->TV != sf.grp) { FreeSweepPlan(g, g->oplan); g->oplan = nullptr; }
       if(!g->oplan && !(g->oplan = NewSweepPlan(g, sf.grp, 1))) return -4;
       if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
@@ -407,9 +403,9 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
       return 1;
    }
 
-   if(g_smooth_mode == 1)
+   if(g->smooth_mode == 1)
    {
-      const Shape sf = g_shape[GmlxBuiltinSweep];
+      const Shape sf = g->shape[GmlxBuiltinSweep];
       if(sf.grp > 256) return -4;              // the exact fused kernel takes tiles of up to 256
       if(g->plan && g->plan->TV != sf.grp) { FreeSweepPlan(g, g->plan); g->plan = nullptr; }
       if(!g->plan && !(g->plan = NewSweepPlan(g, sf.grp, 0))) return -4;
@@ -437,7 +433,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    }
 
    v4f *crd = (v4f *)vd->dev;
-   const Shape ss = g_shape[GmlxBuiltinScatter], sg = g_shape[GmlxBuiltinGather];
+   const Shape ss = g->shape[GmlxBuiltinScatter], sg = g->shape[GmlxBuiltinGather];
    b->host_valid = false;
    for(int i = 0; i < nsweep; i++)
    {
@@ -456,37 +452,40 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    return 1;
 }
 
-int GmlxSetSmoothMode(int mode)
+int GmlxSetSmoothMode(size_t GmlIdx, int mode)
 {
+   GML_CTX(g, GmlIdx);
    if(mode < 0 || mode > 2) return 0;
-   g_smooth_mode = mode;
+   g->smooth_mode = mode;
    return 1;
 }
 
 int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
 {
    GML_CTX(g, GmlIdx);
-   SweepPlan *p = g_smooth_mode == 2 ? g->oplan : g->plan;
+   SweepPlan *p = g->smooth_mode == 2 ? g->oplan : g->plan;
    if(!p || !info) return 0;
    info[0] = p->TV; info[1] = p->ntile; info[2] = p->sum_nv; info[3] = p->sum_nt;
    info[4] = p->max_nv; info[5] = p->max_nt; info[6] = (long long)p->bytes;
-   info[7] = p->sum_ring; info[8] = g_smooth_mode;
+   info[7] = p->sum_ring; info[8] = g->smooth_mode;
    return 1;
 }
 
-int GmlxSetBuiltinItems(int which, int PerThread)
+int GmlxSetBuiltinItems(size_t GmlIdx, int which, int PerThread)
 {
+   GML_CTX(g, GmlIdx);
    if(which != GmlxBuiltinQuality || (PerThread != 1 && PerThread != 2 && PerThread != 4)) return 0;
-   g_shape[which].per = PerThread;
+   g->shape[which].per = PerThread;
    return 1;
 }
 
-int GmlxSetBuiltinShape(int which, int GrpSiz, int Remap)
+int GmlxSetBuiltinShape(size_t GmlIdx, int which, int GrpSiz, int Remap)
 {
+   GML_CTX(g, GmlIdx);
    if(which < 0 || which > GmlxBuiltinCgUpdate || GrpSiz < 64 || GrpSiz > 1024 || (GrpSiz & 63)) return 0;
    if(which > GmlxBuiltinSweep && GrpSiz > 256) return 0;
-   g_shape[which].grp = GrpSiz;
-   g_shape[which].remap = Remap ? 1 : 0;
+   g->shape[which].grp = GrpSiz;
+   g->shape[which].remap = Remap ? 1 : 0;
    return 1;
 }
 

@@ -20,11 +20,19 @@
 // independent of what else ran: the same alone and in the mix.
 //
 // Meshes are either seeded cube meshes (cells per side, seed) or meshes the
-// caller registered with GmlxServeAddMesh (the server keeps their host copy
-// to load them again after an eviction).
+// caller registered with GmlxServeAddMesh.  The server keeps the host copy
+// of every mesh it has seen (a cube mesh is generated on its first job) and,
+// once built, its CG matrix and right-hand side, so a mesh evicted from the
+// device is loaded again by an upload alone.
 //
 // Resident = 0 gives the plain per-job path for comparison: every job loads
-// its mesh into a fresh handle, runs, and frees it, one job at a time.
+// its mesh into a fresh handle (GmlInit, upload, links and plans built on
+// the device), runs, and frees it, one job at a time.
+//
+// Lifetime: a job's record lives until every caller blocked in
+// GmlxServeWait on it has returned (the first to return retires its id);
+// GmlxServeStop lets running jobs end, finishes pending ones with -8 and
+// waits for blocked waiters to leave before freeing the server.
 
 #include "gml_internal.h"
 
@@ -39,11 +47,17 @@
 
 namespace gml {
 
-// a mesh registered by the caller
+// host copy of a mesh (registered by the caller, or a generated cube mesh),
+// with its CG system once a CG job has built it
 struct UserMesh {
    int nver = 0, ntet = 0;
+   unsigned long long seed = 0;    // seed of the CG matrix weights and right-hand side
    std::vector<float> crd;
    std::vector<int> ref, tet;
+   std::mutex sys_mtx;
+   long long nnz = 0;             // > 0 once the CG system is built
+   std::vector<int> rowptr, col;
+   std::vector<float> val, rhs;
 };
 
 // n > 0: cube mesh (n, seed); n == 0: user mesh number seed
@@ -54,7 +68,7 @@ struct Resident {
    int nver = 0, ntet = 0;
    int VerIdx = 0, TetIdx = 0, qal = 0, vqal = 0, bary = 0, crd0 = 0;
    int mat = 0, rhs = 0, sol = 0;
-   std::vector<int> tet;          // host copy of the tets (CG matrix build)
+   std::shared_ptr<UserMesh> mesh;   // host copy
    size_t bytes = 0;              // device bytes in use (or reserved before the job ends)
    bool busy = false, dirty = false;
    long long last = 0;
@@ -69,6 +83,8 @@ struct ServeJob {
    double res[4] = {0., 0., 0., 0.};
    int rc = 0;
    bool loaded = false, done = false;
+   int waiters = 0;               // callers blocked in GmlxServeWait on this job
+   bool retired = false;          // id removed from the job map
 };
 
 struct Server {
@@ -80,8 +96,10 @@ struct Server {
    std::deque<ServeJob *> pending;
    std::map<long long, ServeJob *> jobs;
    std::vector<Resident *> cache;
-   std::map<unsigned long long, std::shared_ptr<UserMesh>> meshes;
+   std::map<unsigned long long, std::shared_ptr<UserMesh>> meshes;           // registered
+   std::map<std::pair<int, unsigned long long>, std::shared_ptr<UserMesh>> cubes;   // generated
    unsigned long long next_mesh = 1;
+   int nwaiters = 0;
    std::vector<std::thread> workers;
    bool stop = false;
    long long next_id = 1, clock = 0, loads = 0, evictions = 0, ndone = 0;
@@ -118,40 +136,50 @@ static void Unload(Resident *r)
    r->h = 0;
 }
 
-// load the mesh of r into a fresh handle; false on failure
-static bool Load(Server *s, Resident *r, std::shared_ptr<UserMesh> um)
+// the host copy of a cube mesh, generated on first use (lock not held)
+static std::shared_ptr<UserMesh> CubeHost(Server *s, int n, unsigned long long seed)
 {
-   const int n = r->n;
-   const long long V = n > 0 ? (long long)(n + 1) * (n + 1) * (n + 1) : (um ? um->nver : 0);
-   const long long T = n > 0 ? 6LL * n * n * n : (um ? um->ntet : 0);
-   if(V < 1 || V >= (1LL << 31) || T >= (1LL << 31)) return false;
-   std::vector<float> crd;
-   std::vector<int> ref, tref((size_t)T, 0);
-   if(n > 0)
    {
-      crd.resize((size_t)V * 4);
-      ref.resize((size_t)V);
-      r->tet.resize((size_t)T * 4);
-      if(!GmlxCubeMesh(n, 0.2, r->seed, 1, crd.data(), ref.data(), r->tet.data())) return false;
+      std::lock_guard<std::mutex> lk(s->mtx);
+      auto it = s->cubes.find({n, seed});
+      if(it != s->cubes.end()) return it->second;
    }
-   else
-   {
-      crd = um->crd;
-      ref = um->ref;
-      r->tet = um->tet;
-   }
-   if(T == 0) return false;
+   const long long V = (long long)(n + 1) * (n + 1) * (n + 1), T = 6LL * n * n * n;
+   if(V >= (1LL << 31) || T >= (1LL << 31)) return nullptr;
+   auto m = std::make_shared<UserMesh>();
+   m->nver = (int)V;
+   m->ntet = (int)T;
+   m->seed = seed;
+   m->crd.resize((size_t)V * 4);
+   m->ref.resize((size_t)V);
+   m->tet.resize((size_t)T * 4);
+   if(!GmlxCubeMesh(n, 0.2, seed, 1, m->crd.data(), m->ref.data(), m->tet.data())) return nullptr;
+   std::lock_guard<std::mutex> lk(s->mtx);
+   auto ins = s->cubes.emplace(std::make_pair(n, seed), m);
+   return ins.first->second;
+}
+
+// load the host mesh of r into a fresh handle on the server's device; false
+// on failure
+static bool Load(const Server *s, Resident *r)
+{
+   UserMesh *m = r->mesh.get();
+   if(!m) return false;
+   const long long V = m->nver, T = m->ntet;
+   if(V < 1 || T < 1) return false;
+   std::vector<int> tref((size_t)T, 0);
    r->nver = (int)V;
    r->ntet = (int)T;
    r->h = GmlInit(s->dev);
    if(!r->h) return false;
    const size_t h = r->h;
+   float *crd = m->crd.data();
+   int *ref = m->ref.data(), *tet = m->tet.data();
    r->VerIdx = GmlNewMeshData(h, GmlVertices, (int)V);
    r->TetIdx = GmlNewMeshData(h, GmlTetrahedra, (int)T);
    if(!r->VerIdx || !r->TetIdx
-   || !GmlSetDataBlock(h, GmlVertices, 0, (int)V - 1, crd.data(), crd.data() + (V - 1) * 4, ref.data(), ref.data() + V - 1)
-   || !GmlSetDataBlock(h, GmlTetrahedra, 0, (int)T - 1, r->tet.data(), r->tet.data() + (T - 1) * 4, tref.data(),
-                       tref.data() + T - 1))
+   || !GmlSetDataBlock(h, GmlVertices, 0, (int)V - 1, crd, crd + (V - 1) * 4, ref, ref + V - 1)
+   || !GmlSetDataBlock(h, GmlTetrahedra, 0, (int)T - 1, tet, tet + (T - 1) * 4, tref.data(), tref.data() + T - 1))
       return false;
    r->qal = GmlNewSolutionData(h, GmlTetrahedra, 1, GmlFlt, (char *)"qal");
    r->vqal = GmlNewSolutionData(h, GmlVertices, 1, GmlFlt, (char *)"vqal");
@@ -223,19 +251,29 @@ static int RunJob(Resident *r, ServeJob *j)
    {
       if(!r->mat)
       {
-         const int V = r->nver;
-         std::vector<int> rowptr((size_t)V + 1);
-         const long long nnz = GmlxEdgeLaplacian3(V, r->ntet, r->tet.data(), r->seed, 0.01, rowptr.data(), nullptr,
-                                                  nullptr, 0);
-         if(nnz <= 0) return -5;
-         std::vector<int> col((size_t)nnz);
-         std::vector<float> val((size_t)nnz * 9);
-         if(GmlxEdgeLaplacian3(V, r->ntet, r->tet.data(), r->seed, 0.01, rowptr.data(), col.data(), val.data(), nnz) != nnz)
-            return -5;
-         std::vector<float> b((size_t)V * 3);
-         for(long long i = 0; i < (long long)V * 3; i++) b[i] = (float)HashNormal(r->seed, i);
-         r->mat = GmlNewMatrix(r->h, V, (int)nnz, 3, val.data(), col.data(), rowptr.data(), GmlFlt);
-         r->rhs = GmlNewVector(r->h, V, 3, b.data(), GmlFlt);
+         UserMesh *m = r->mesh.get();
+         const int V = m->nver;
+         {
+            // the host system is built once per mesh and kept with it
+            std::lock_guard<std::mutex> lk(m->sys_mtx);
+            if(m->nnz <= 0)
+            {
+               m->rowptr.resize((size_t)V + 1);
+               const long long nnz = GmlxEdgeLaplacian3(V, m->ntet, m->tet.data(), m->seed, 0.01, m->rowptr.data(),
+                                                        nullptr, nullptr, 0);
+               if(nnz <= 0) return -5;
+               m->col.resize((size_t)nnz);
+               m->val.resize((size_t)nnz * 9);
+               if(GmlxEdgeLaplacian3(V, m->ntet, m->tet.data(), m->seed, 0.01, m->rowptr.data(), m->col.data(),
+                                     m->val.data(), nnz) != nnz)
+                  return -5;
+               m->rhs.resize((size_t)V * 3);
+               for(long long i = 0; i < (long long)V * 3; i++) m->rhs[i] = (float)HashNormal(m->seed, i);
+               m->nnz = nnz;
+            }
+         }
+         r->mat = GmlNewMatrix(r->h, V, (int)m->nnz, 3, m->val.data(), m->col.data(), m->rowptr.data(), GmlFlt);
+         r->rhs = GmlNewVector(r->h, V, 3, m->rhs.data(), GmlFlt);
          r->sol = GmlNewVector(r->h, V, 3, nullptr, GmlDbl);
          if(!r->mat || !r->rhs || !r->sol) return -5;
       }
@@ -305,6 +343,11 @@ static ServeJob *Admit(Server *s, Resident **out)
          r = new Resident();
          r->n = j->n;
          r->seed = j->seed;
+         if(j->n == 0)
+         {
+            auto mi = s->meshes.find(j->seed);
+            if(mi != s->meshes.end()) r->mesh = mi->second;
+         }
          s->cache.push_back(r);
          j->loaded = true;
          s->loads++;
@@ -335,16 +378,11 @@ static void Worker(Server *s)
       s->running++;
       s->max_running = std::max(s->max_running, s->running);
       j->t_admit = s->Now();
-      std::shared_ptr<UserMesh> um;
-      if(r->n == 0)
-      {
-         auto it = s->meshes.find(r->seed);
-         if(it != s->meshes.end()) um = it->second;
-      }
       lk.unlock();
 
       int rc = 1;
-      if(!r->h && !Load(s, r, um)) rc = -5;
+      if(!r->mesh && r->n > 0) r->mesh = CubeHost(s, r->n, r->seed);
+      if(!r->h && !Load(s, r)) rc = -5;
       if(rc == 1) rc = RunJob(r, j);
 
       lk.lock();
@@ -429,8 +467,37 @@ long long GmlxServeAddMesh(size_t SrvIdx, int NmbVer, int NmbTet, const float *c
    m->tet.assign(tet, tet + (size_t)NmbTet * 4);
    std::lock_guard<std::mutex> lk(s->mtx);
    const unsigned long long key = s->next_mesh++;
+   m->seed = key;
    s->meshes[key] = m;
    return (long long)key;
+}
+
+int GmlxServeRemoveMesh(size_t SrvIdx, long long MeshKey)
+{
+   Server *s = reinterpret_cast<Server *>(SrvIdx);
+   if(!s || MeshKey < 1) return 0;
+   std::lock_guard<std::mutex> lk(s->mtx);
+   const unsigned long long key = (unsigned long long)MeshKey;
+   if(!s->meshes.count(key)) return 0;
+   // refused while a job on the mesh is pending or running
+   for(ServeJob *j : s->pending)
+      if(j->n == 0 && j->seed == key) return 0;
+   for(Resident *r : s->cache)
+      if(r->n == 0 && r->seed == key && r->busy) return 0;
+   for(auto it = s->cache.begin(); it != s->cache.end();)
+   {
+      Resident *r = *it;
+      if(r->n == 0 && r->seed == key)
+      {
+         s->used -= r->bytes;
+         Unload(r);
+         delete r;
+         it = s->cache.erase(it);
+      }
+      else ++it;
+   }
+   s->meshes.erase(key);
+   return 1;
 }
 
 long long GmlxServeSubmitMesh(size_t SrvIdx, int type, long long MeshKey, double param)
@@ -452,6 +519,8 @@ int GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim)
    auto it = s->jobs.find(JobIdx);
    if(it == s->jobs.end()) return -1;
    ServeJob *j = it->second;
+   j->waiters++;
+   s->nwaiters++;
    s->cv_done.wait(lk, [&]() { return j->done; });
    if(res) for(int i = 0; i < 4; i++) res[i] = j->res[i];
    if(tim)
@@ -462,8 +531,15 @@ int GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim)
       tim[3] = j->loaded ? 1. : 0.;
    }
    const int rc = j->rc;
-   s->jobs.erase(it);
-   delete j;
+   // the first waiter to return retires the id; the last one frees the record
+   if(!j->retired)
+   {
+      s->jobs.erase(JobIdx);
+      j->retired = true;
+   }
+   s->nwaiters--;
+   if(--j->waiters == 0) delete j;
+   s->cv_done.notify_all();
    return rc;
 }
 
@@ -485,9 +561,22 @@ void GmlxServeStop(size_t SrvIdx)
    {
       std::lock_guard<std::mutex> lk(s->mtx);
       s->stop = true;
+      // jobs not yet admitted end with -8; running ones finish
+      for(ServeJob *j : s->pending)
+      {
+         j->t_admit = s->Now();
+         Finish(s, j, -8);
+      }
+      s->pending.clear();
       s->cv_work.notify_all();
    }
    for(auto &t : s->workers) t.join();
+   {
+      // every job is done now: wake blocked waiters and let them leave
+      std::unique_lock<std::mutex> lk(s->mtx);
+      s->cv_done.notify_all();
+      s->cv_done.wait(lk, [&]() { return s->nwaiters == 0; });
+   }
    for(Resident *r : s->cache) { Unload(r); delete r; }
    for(auto &p : s->jobs) delete p.second;
    delete s;

@@ -33,8 +33,8 @@ int    GmlxCompileKernelv(size_t GmlIdx, const char *KrnSrc, const char *PrcNam,
    single-item data reached through a ball (e.g. tet data read by a vertex
    kernel) are loaded once per work-group tile into LDS with coalesced loads
    and read from there; results are identical either way.  Applies to
-   kernels compiled afterwards; returns the previous setting. */
-int    GmlxSetStaging(int on);
+   kernels the handle compiles afterwards; returns the previous setting. */
+int    GmlxSetStaging(size_t GmlIdx, int on);
 
 /* Generated HIP source of a compiled kernel (NUL-terminated, owned by the
    library, valid until GmlStop); NULL if the index is invalid.  The paired
@@ -146,19 +146,19 @@ int    GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx);
    coordinates: after a sweep the vertex table's device address is the other
    buffer (query it again with GmlxGetDevicePointer). */
 int    GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega);
-int    GmlxSetSmoothMode(int mode);
+int    GmlxSetSmoothMode(size_t GmlIdx, int mode);
 /* tile plan of the current sweep mode (info: 9 values): {tile size, tiles,
    sum of tile vertices, sum of tile tets, max vertices per tile, max tets per
    tile, plan bytes, total 1-ring slots (operator form), mode} */
 int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
-/* launch shape of a built-in kernel for every handle of the process: GrpSiz a
+/* launch shape of a built-in kernel on one handle: GrpSiz a
    multiple of 64, up to 1024 (quality, ball-min, scatter, gather, sweep: the
    tile of the operator-form sweep; the exact fused sweep takes up to 256) or
    256 (SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
-int    GmlxSetBuiltinShape(int which, int GrpSiz, int Remap);
+int    GmlxSetBuiltinShape(size_t GmlIdx, int which, int GrpSiz, int Remap);
 /* items per thread of a built-in kernel that takes several: the quality
    kernel (GmlxBuiltinQuality) takes 1, 2 (default) or 4 tets per thread */
-int    GmlxSetBuiltinItems(int which, int PerThread);
+int    GmlxSetBuiltinItems(size_t GmlIdx, int which, int PerThread);
 /* summed device time (s) and launch count of a built-in kernel on a handle
    (GmlxBuiltinSpMV: the block SpMV of GmlMultMatVec; GmlxBuiltinCgSpMV and
    GmlxBuiltinCgUpdate: the two kernels of a GmlxSolveCG iteration) */
@@ -220,15 +220,23 @@ long long GmlxServeSubmit(size_t SrvIdx, int type, int n, unsigned long long see
 long long GmlxServeAddMesh(size_t SrvIdx, int NmbVer, int NmbTet, const float *crd, const int *ref, const int *tet);
 /* a job on a registered mesh (same types and param as GmlxServeSubmit) */
 long long GmlxServeSubmitMesh(size_t SrvIdx, int type, long long MeshKey, double param);
+/* forget a registered mesh (its host copy, and its device copy if resident);
+   returns 1, or 0 if the key is unknown or a job on the mesh is pending or
+   running */
+int       GmlxServeRemoveMesh(size_t SrvIdx, long long MeshKey);
 /* block until the job is done.  res[4]: quality and smooth = (min, mean
    quality), cg = (FP64 relative residual, iterations); tim[4] = (submitted,
    admitted, done) in seconds since GmlxServeStart, and 1 if the job loaded
-   its mesh.  Returns 1, or a negative error (-7: the mesh cannot fit the
-   budget). */
+   its mesh.  Several callers may wait on one job; each gets its result, and
+   the id is retired once they return (a later wait on it returns -1).
+   Returns 1, or a negative error (-7: the mesh cannot fit the budget; -8:
+   the server stopped before the job was admitted). */
 int       GmlxServeWait(size_t SrvIdx, long long JobIdx, double *res, double *tim);
 /* 8 values: resident meshes, bytes in use, mesh loads, evictions, jobs done,
    most jobs running at once, budget bytes, peak bytes */
 int       GmlxServeInfo(size_t SrvIdx, double *info);
+/* stop: running jobs finish, pending jobs end with -8, callers blocked in
+   GmlxServeWait return; then every resident mesh is freed */
 void      GmlxServeStop(size_t SrvIdx);
 
 /* ---- batches of meshes: many independent tet meshes kept on the device and

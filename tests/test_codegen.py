@@ -51,18 +51,15 @@ def test_downlink_kernel_contract(mesh):
 @pytest.mark.parametrize("staged", [0, 1])
 def test_uplink_kernel_contract_with_ref_and_params(mesh, staged):
     crd, ref, tet = mesh
-    old = G.lib().GmlxSetStaging(staged)
-    try:
-        with G.Gml(host_only=True) as g:
-            vi, ti = g.set_mesh(crd, ref, tet)
-            g.new_parameters(16, body("smooth_params"))
-            b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
-            k = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
-                          [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
-            s = src_of(g, k)
-            linkinfo = g.link_info(G.GmlVertices, G.GmlTetrahedra)
-    finally:
-        G.lib().GmlxSetStaging(old)
+    with G.Gml(host_only=True) as g:
+        G.lib().GmlxSetStaging(g.h, staged)
+        vi, ti = g.set_mesh(crd, ref, tet)
+        g.new_parameters(16, body("smooth_params"))
+        b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+        k = g.compile(body("smooth_gather"), "smooth_gather", G.GmlVertices,
+                      [(b, G.GmlReadMode, 0), (vi, G.GmlReadMode | G.GmlWriteMode | G.GmlRefFlag, 0)])
+        s = src_of(g, k)
+        linkinfo = g.link_info(G.GmlVertices, G.GmlTetrahedra)
     assert "#define   BaryDegMax 32" in s
     assert "int16 VerTetBal[2];" in s and "int VerTetDeg;" in s and "int BaryDeg;" in s
     if staged:

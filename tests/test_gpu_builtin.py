@@ -22,11 +22,7 @@ def body(name):
                                                 (20, 256, 1, 2), (13, 64, 0, 2), (20, 512, 1, 2),
                                                 (11, 1024, 0, 2)])
 def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused):
-    for w in range(4):
-        assert G.lib().GmlxSetBuiltinShape(w, grp, remap)
     tile = grp if fused == 2 else min(grp, 256)        # the operator form takes tiles up to 1024
-    assert G.lib().GmlxSetBuiltinShape(G.GmlxBuiltinSweep, tile, remap)
-    assert G.lib().GmlxSetSmoothMode(fused)
     crd, ref, tet = G.cube_mesh(n, 0.2, 3)
     omega = 0.2
     res = {}
@@ -50,6 +46,11 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
                 for _ in range(5):
                     g.launch(ks); g.launch(kg)
             else:
+                # launch shapes and sweep form are this handle's own
+                for w in range(4):
+                    assert g.L.GmlxSetBuiltinShape(g.h, w, grp, remap)
+                assert g.L.GmlxSetBuiltinShape(g.h, G.GmlxBuiltinSweep, tile, remap)
+                assert g.L.GmlxSetSmoothMode(g.h, fused)
                 assert g.L.GmlxTetQuality(g.h, q) == 1
                 assert g.L.GmlxBallMin(g.h, q, vq) == 1
                 q0 = g.download(q, np.float32, (len(tet),)); v0 = g.download(vq, np.float32, (len(crd),))
@@ -79,9 +80,6 @@ def test_builtin_quality_ballmin_smoothing_match_generated(n, grp, remap, fused)
     assert np.array_equal(res["builtin"][1], O.ball_min(d0, b0, res["builtin"][0]))
     c64 = O.smooth64(tet, ref, d0, b0, crd, 5, omega)
     assert np.abs(res["builtin"][2][:, :3] - c64).max() <= 1e-5
-    for w in range(5):
-        G.lib().GmlxSetBuiltinShape(w, 256, 1)
-    G.lib().GmlxSetSmoothMode(2)
 
 
 
@@ -97,8 +95,8 @@ def test_operator_sweep_degree_from_counts(degenerate):
         a, b, c, _ = tet[inner]
         tet = np.vstack([tet, [[a, a, b, c]]]).astype(np.int32)
     omega = 0.3
-    assert G.lib().GmlxSetSmoothMode(2)
     with G.Gml() as g:
+        assert g.L.GmlxSetSmoothMode(g.h, 2)
         vi, ti = g.set_mesh(crd, ref, tet)
         bary = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
         assert g.L.GmlxSmoothSweeps(g.h, bary, 2, omega) == 1

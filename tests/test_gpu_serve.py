@@ -114,3 +114,66 @@ def test_serve_registered_mesh_matches_cube_job():
     L.GmlxServeStop(srv)
     assert out[0][:2] == out[3][:2] and out[1][:2] == out[4][:2]
     assert out[2][0] <= 1e-8 and out[2][1] > 0
+
+
+def _wait_thread(L, srv, jid, out, tag):
+    import threading
+
+    def w():
+        res, tim = (C.c_double * 4)(), (C.c_double * 4)()
+        out[tag] = (L.GmlxServeWait(srv, jid, res, tim), tuple(res))
+    t = threading.Thread(target=w)
+    t.start()
+    return t
+
+
+def test_serve_two_waiters_one_job_then_id_retired():
+    L = G.lib()
+    srv = L.GmlxServeStart(0, 1.0, 1, 1)
+    jid = L.GmlxServeSubmit(srv, G.GmlxJobSmooth, 10, 3, 20.0)
+    out = {}
+    ts = [_wait_thread(L, srv, jid, out, k) for k in ("a", "b")]
+    for t in ts:
+        t.join(60)
+    assert out["a"] == out["b"] and out["a"][0] == 1
+    res, tim = (C.c_double * 4)(), (C.c_double * 4)()
+    assert L.GmlxServeWait(srv, jid, res, tim) == -1            # retired
+    L.GmlxServeStop(srv)
+
+
+def test_serve_stop_ends_pending_jobs_and_wakes_waiters():
+    L = G.lib()
+    import time
+    srv = L.GmlxServeStart(0, 4.0, 1, 1)
+    # one slot: a smoothing job of 100000 sweeps (at least half a second of
+    # launches) holds it while the quality jobs behind it are still pending
+    # when the server stops; the waiters are blocked in GmlxServeWait by then
+    first = L.GmlxServeSubmit(srv, G.GmlxJobSmooth, 24, 9, 100000.0)
+    rest = [L.GmlxServeSubmit(srv, G.GmlxJobQuality, 6, 9, 0.0) for _ in range(8)]
+    out = {}
+    ts = [_wait_thread(L, srv, j, out, j) for j in [first] + rest]
+    time.sleep(0.2)
+    L.GmlxServeStop(srv)
+    for t in ts:
+        t.join(60)
+        assert not t.is_alive()
+    assert out[first][0] == 1
+    rcs = [out[j][0] for j in rest]
+    assert set(rcs) <= {1, -8} and rcs[-1] == -8, rcs
+
+
+def test_serve_remove_mesh():
+    L = G.lib()
+    crd, ref, tet = G.cube_mesh(4, 0.2, 5)
+    srv = L.GmlxServeStart(0, 0.05, 2, 1)
+    key = L.GmlxServeAddMesh(srv, len(crd), len(tet), G.ptr(crd), G.ptr(ref), G.ptr(tet))
+    jid = L.GmlxServeSubmitMesh(srv, G.GmlxJobQuality, key, 0.0)
+    res, tim = (C.c_double * 4)(), (C.c_double * 4)()
+    assert L.GmlxServeWait(srv, jid, res, tim) == 1
+    info = (C.c_double * 8)()
+    assert L.GmlxServeInfo(srv, info) and info[0] == 1          # resident
+    assert L.GmlxServeRemoveMesh(srv, key) == 1
+    assert L.GmlxServeInfo(srv, info) and info[0] == 0 and info[1] == 0
+    assert L.GmlxServeRemoveMesh(srv, key) == 0
+    assert L.GmlxServeSubmitMesh(srv, G.GmlxJobQuality, key, 0.0) == -1
+    L.GmlxServeStop(srv)

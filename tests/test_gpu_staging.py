@@ -45,13 +45,13 @@ def src_of(g, k):
 def both(g, src, name, typ, args):
     """(unstaged, staged) kernels of one body."""
     L = G.lib()
-    old = L.GmlxSetStaging(0)
+    old = L.GmlxSetStaging(g.h, 0)
     try:
         k0 = g.compile(src, name, typ, args)
-        L.GmlxSetStaging(1)
+        L.GmlxSetStaging(g.h, 1)
         k1 = g.compile(src, name, typ, args)
     finally:
-        L.GmlxSetStaging(old)
+        L.GmlxSetStaging(g.h, old)
     assert "gml_lds" not in src_of(g, k0) and "gml_lds" in src_of(g, k1)
     return k0, k1
 
