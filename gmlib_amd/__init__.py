@@ -101,6 +101,7 @@ _SIGS = {
     "GmlxFreeVector": (_I, [_S, _I]),
     "GmlxFreeMatrix": (_I, [_S, _I]),
     "GmlxDotVec": (_I, [_S, _I, _I, _DP]),
+    "GmlxReduceVectorNoAlloc": (_I, [_S, _I, _I, _DP]),
     "GmlxSolveCG": (_I, [_S, _I, _I, _I, _D, _I, _I, _DP, _IP]),
     "GmlxTetQuality": (_I, [_S, _I]),
     "GmlxBallMin": (_I, [_S, _I, _I]),

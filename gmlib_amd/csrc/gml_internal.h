@@ -228,6 +228,7 @@ void FreeItemPool(GmlCtx *g, bool idle_only = false);
 // reductions (gml_reduce.hip)
 int  ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bool timed);
 size_t ReduceScratchBytes();
+int  NoAllocScratchInit(int dev);   // device current; once per device (GmlInit)
 
 // block algebra (gml_algebra.hip)
 void FreeAlgebra(GmlCtx *g);

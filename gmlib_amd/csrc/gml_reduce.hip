@@ -15,6 +15,8 @@
 
 #include "gml_internal.h"
 
+#include <mutex>
+
 namespace gml {
 
 static const int kRedBlock = 256;
@@ -167,11 +169,17 @@ __global__ void __launch_bounds__(kFinBlock) k_reduce_finish(RedScratch *__restr
    if(threadIdx.x == 0) s->result = a;
 }
 
+// work-groups of a reduction over n floats (one partial each)
+static int RedGrid(long long n)
+{
+   const long long want = (n / 4 + kRedBlock * 4 - 1) / (kRedBlock * 4);
+   return (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
+}
+
 static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s)
 {
    // about four float4 per thread, up to 16384 work-groups
-   long long want = (n / 4 + kRedBlock * 4 - 1) / (kRedBlock * 4);
-   int grid = (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
+   const int grid = RedGrid(n);
 #define GML_RED(O) do { hipLaunchKernelGGL(k_reduce<O>, dim3(grid), dim3(kRedBlock), 0, st, x, n, s); \
                          hipLaunchKernelGGL(k_reduce_finish<O>, dim3(1), dim3(kFinBlock), 0, st, s, grid); } while(0)
    switch(op)
@@ -187,6 +195,45 @@ static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, Red
    }
 #undef GML_RED
    return hipGetLastError() == hipSuccess ? 1 : -6;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation form (GmlxReduceVectorNoAlloc): the partials
+// and the result live in this file-scope device array instead of the
+// handle's scratch.  Every GPU the library runs on holds its own copy
+// (sizeof(RedScratch) = 131,080 bytes) until the process ends.  In host code
+// the name `scratch` is a host-side stand-in, not a device address, so it is
+// never passed to a kernel: NoAllocScratchInit looks up each device's address
+// once, with that device current, and kernels take that address.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) RedScratch scratch[1];
+
+static const int kScrMaxDev = 64;
+static std::mutex g_scr_mtx;
+static RedScratch *g_scr_ptr[kScrMaxDev] = {nullptr};
+
+int NoAllocScratchInit(int dev)
+{
+   if(dev < 0 || dev >= kScrMaxDev) return 0;
+   std::lock_guard<std::mutex> lk(g_scr_mtx);
+   if(g_scr_ptr[dev]) return 1;
+   void *p = nullptr;
+   const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+   if(e != hipSuccess || !p)
+   {
+      fprintf(stderr, "gml: no address for the reduction scratch on device %d: %s\n", dev, hipGetErrorString(e));
+      (void)hipGetLastError();
+      return 0;
+   }
+   g_scr_ptr[dev] = (RedScratch *)p;
+   return 1;
+}
+
+static RedScratch *NoAllocScratch(int dev)
+{
+   if(dev < 0 || dev >= kScrMaxDev) return nullptr;
+   std::lock_guard<std::mutex> lk(g_scr_mtx);
+   return g_scr_ptr[dev];
 }
 
 // Reduce n floats at device address x; *res receives the result.
@@ -239,6 +286,54 @@ int GmlReduceVector(size_t GmlIdx, int DatIdx, int RedOpp, double *nrm)
    }
    if(!nrm) return -4;
    return ReduceDevice(g, (const float *)d->dev, d->NmbLin, RedOpp, nrm, true);
+}
+
+// GmlReduceVector without any allocation at call time: the same kernels
+// and result, with the partials in the file-scope scratch of the handle's
+// device; no timing events.  Calls that overlap on one GPU race on that
+// scratch.  Every launch writes each partial its finish reads, so nothing
+// has to be cleared between calls.
+int GmlxReduceVectorNoAlloc(size_t GmlIdx, int DatIdx, int RedOpp, double *nrm)
+{
+   GML_CTX(g, GmlIdx);
+   if(DatIdx < 1 || DatIdx > GmlMaxDat || !g->dat[DatIdx].use)
+   {
+      printf("Invalid data index: %d\n", DatIdx);
+      return -1;
+   }
+   Data *d = &g->dat[DatIdx];
+   if(d->ItmTyp != GmlFlt || d->NmbItm != 1 || d->ItmLen != 1)
+   {
+      printf("Invalid data structure: count %d, type %s, length %d\n", d->NmbItm, OclTypStr[d->ItmTyp], d->ItmLen);
+      return -2;
+   }
+   if(RedOpp < 0 || RedOpp >= GmlMaxRed)
+   {
+      printf("Invalid operation code %d\n", RedOpp);
+      return -3;
+   }
+   if(!nrm) return -4;
+   if(g->host_only) return -6;
+   RedScratch *s = NoAllocScratch(g->dev);
+   if(!s)
+   {
+      printf("No reduction scratch on device %d\n", g->dev);
+      return -6;
+   }
+   // the scratch holds kRedMaxGrid partials; the grid never takes more
+   const int grid = RedGrid(d->NmbLin);
+   if(grid > kRedMaxGrid)
+   {
+      printf("Reduction of %lld lines needs %d partials, the scratch holds %d\n", d->NmbLin, grid, kRedMaxGrid);
+      return -4;
+   }
+   const int rc = ReduceLaunch(g->stream, (const float *)d->dev, d->NmbLin, RedOpp, s);
+   if(rc != 1) return rc;
+   if(hipMemcpyAsync(g->red_host, &s->result, sizeof(double), hipMemcpyDeviceToHost, g->stream) != hipSuccess
+   || hipStreamSynchronize(g->stream) != hipSuccess)
+      return -6;
+   *nrm = *g->red_host;
+   return 1;
 }
 
 double GmlGetReduceRunTime(size_t GmlIdx, int RedOpp)

@@ -268,6 +268,9 @@ size_t GmlInit(int DevIdx)
       delete g;
       return 0;
    }
+   // device address of the no-allocation reduction scratch (once per device)
+   if(!NoAllocScratchInit(DevIdx))
+      puts("GmlxReduceVectorNoAlloc is unavailable on this device.");
    HIPCHK(hipStreamSynchronize(g->stream));
    return reinterpret_cast<size_t>(g);
 }

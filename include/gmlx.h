@@ -21,6 +21,22 @@ extern "C" {
    GmlLaunchKernel returns -1 and GmlReduceVector -6.  Release with GmlStop. */
 size_t GmlxInitHostOnly(void);
 
+/* --- reductions without allocation ---------------------------------------- */
+
+/* GmlReduceVector (sources/gmlib3.c:3229-3332) with nothing allocated at
+   call time, for callers that cannot allocate (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+   the reference.  Same arguments, result and error codes as
+   GmlReduceVector, from the same kernels in the same order (so the same
+   double); the work-group partials go to a file-scope device array of
+   16,384 partials + the result (131,080 bytes per GPU, held until the
+   process ends) instead of the handle's scratch, and no timing events are
+   recorded.  The grid is capped at those 16,384 work-groups, so every
+   vector length fits.  Calls that overlap on one GPU (two handles' streams
+   at once) race on that array; calls in sequence do not, since each launch
+   writes every partial it reads. */
+int    GmlxReduceVectorNoAlloc(size_t GmlIdx, int DatIdx, int RedOpp, double *res);
+
 /* --- FFI-friendly forms of variadic entry points ------------------------- */
 
 /* GmlCompileKernel with the (DatIdx, flags, LnkIdx) triplets in an array of
