@@ -137,20 +137,71 @@ def test_get_cnt_vec_table():
         assert (a.value, b.value) == (c, v), n
 
 
-def test_evaluate_numbering_restatement():
-    rng = np.random.default_rng(1)
-    ele = rng.integers(0, 5000, size=(700, 4)).astype(np.int32)
-    tag = [0] * 64
-    hit = mis = 0
-    for v in ele[:, 0]:
-        pos, adr = (int(v) >> 4) & 15, int(v) >> 10
-        if tag[pos] == adr:
-            hit += 1
-        else:
-            mis += 1
-            tag[pos] = adr
-    got = O.lib().orc_evaluate_numbering(len(ele), 4, O.p(ele))
-    assert got == pytest.approx(100. * hit / (hit + mis), rel=1e-6)
+def numbering(first_nodes):
+    """orc_evaluate_numbering over elements whose first node is given (the
+    reference reads only the first node of each element line: NmbItm = 1,
+    gmlib3.c:589 and :4423-4426)."""
+    e = np.zeros((len(first_nodes), 4), np.int32)
+    e[:, 0] = first_nodes
+    e[:, 1:] = 987654                       # never read
+    return O.lib().orc_evaluate_numbering(len(e), 4, O.p(e))
+
+
+def test_evaluate_numbering_closed_forms():
+    # gmlib3.c:4409-4442: a 16-entry direct-mapped tag table indexed by bits
+    # 4..7 of the vertex id, tag = id >> 10, tags start at 0
+    n = 4096
+    # one vertex id below 1024: its tag is the initial 0, every access hits
+    assert numbering(np.full(n, 700)) == 100.0
+    # one id above 1024: one miss, then hits
+    assert numbering(np.full(n, 5000)) == pytest.approx(100.0 * (n - 1) / n, rel=1e-6)
+    # ids 1024 apart share a slot with a new tag each time: every access misses
+    assert numbering(1024 * np.arange(1, n + 1)) == 0.0
+    # consecutive ids 0..n-1: the first 1024 hit (tag 0); in each later block
+    # of 1024 the first access to each of the 16 slots misses
+    miss = 16 * (n // 1024 - 1)
+    assert numbering(np.arange(n)) == pytest.approx(100.0 * (n - miss) / n, rel=1e-6)
+    # ids 16 apart walk the 16 slots: 64 ids per slot per block of 1024 ids,
+    # the first of each slot in each block >= 1 misses
+    ids = 16 * np.arange(n)
+    blocks = (ids[-1] >> 10) + 1
+    assert numbering(ids) == pytest.approx(100.0 * (n - 16 * (blocks - 1)) / n, rel=1e-6)
+    # two ids in different blocks whose bits 4..7 differ (slots 0 and 1) but
+    # whose bits 5..8 agree: alternating them hits after the first two misses
+    ab = np.tile(np.array([1024, 2048 + 16]), n // 2)
+    assert numbering(ab) == pytest.approx(100.0 * (n - 2) / n, rel=1e-6)
+    # the same two ids in one slot (bits 4..7 equal) evict each other
+    assert numbering(np.tile(np.array([1024, 2048]), n // 2)) == 0.0
+    # only the first node counts: identical first nodes with varied others
+    e = np.random.default_rng(2).integers(0, 10 ** 6, size=(n, 4)).astype(np.int32)
+    e[:, 0] = 300
+    assert O.lib().orc_evaluate_numbering(n, 4, O.p(e)) == 100.0
+
+
+def test_evaluate_numbering_shares_tags_across_kinds():
+    # GmlEvaluateNumbering runs one tag table over every element kind in turn
+    # (gmlib3.c:4416-4438): edges whose first node is 5000 leave the tag for
+    # the tets that follow, so only the very first access misses
+    import ctypes as C
+    import gmlib_amd as G
+    ne, nt = 50, 70
+    with G.Gml(host_only=True) as g:
+        L, h = g.L, g.h
+        vi = L.GmlNewMeshData(h, G.GmlVertices, 6000)
+        ei = L.GmlNewMeshData(h, G.GmlEdges, ne)
+        ti = L.GmlNewMeshData(h, G.GmlTetrahedra, nt)
+        assert vi and ei and ti
+        edg = np.tile(np.array([[5000, 17, 0]], np.int32), (ne, 1))
+        tet = np.tile(np.array([[5000, 1, 2, 3]], np.int32), (nt, 1))
+        r = np.zeros(max(ne, nt), np.int32)
+        assert L.GmlSetDataBlock(h, G.GmlEdges, 0, ne - 1, G.ptr(edg), C.c_void_p(edg.ctypes.data + (ne - 1) * 8),
+                                 G.ptr(r), C.c_void_p(r.ctypes.data + (ne - 1) * 4))
+        assert L.GmlSetDataBlock(h, G.GmlTetrahedra, 0, nt - 1, G.ptr(tet), C.c_void_p(tet.ctypes.data + (nt - 1) * 16),
+                                 G.ptr(r), C.c_void_p(r.ctypes.data + (nt - 1) * 4))
+        got = L.GmlEvaluateNumbering(h)
+    assert got == pytest.approx(100.0 * (ne + nt - 1) / (ne + nt), rel=1e-6)
+    # the oracle over the concatenated first nodes gives the same figure
+    assert numbering(np.concatenate([edg[:, 0], tet[:, 0]])) == pytest.approx(got, rel=1e-6)
 
 
 @pytest.mark.parametrize("n", [0, 1, 3, 4, 1027])
@@ -239,6 +290,125 @@ def test_reference_nodesmooth_ideal_apex_of_regular_tet():
     for f in range(4):
         c = np.delete(p, f, axis=0).mean(axis=0)
         assert np.allclose(opt[0, f], 2 * c - p[f], atol=2e-5)
+
+
+def test_reference_nodesmooth_scatter_closed_form_corner_tet():
+    # corner tet (0, e1, e2, e3), positively oriented.  Face opposite vertex 0
+    # is equilateral with side sqrt(2): centroid (1/3,1/3,1/3), outward normal
+    # (1,1,1)/sqrt(3), height 0.816496 * sqrt(2) -> ideal position ~(1,1,1).
+    # Face opposite vertex 1 is a right isoceles triangle with legs 1: the
+    # height uses its longest edge, the hypotenuse sqrt(2), along -x.
+    p = np.array([[0., 0., 0.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]]) + 3.0
+    tet = np.array([[0, 1, 2, 3]], np.int32)
+    deg, ball, _ = O.vertex_ball(tet, 4, 32)
+    _, opt, _ = O.ref_nodesmooth64(tet, deg, ball, np.hstack([p, np.zeros((4, 1))]), 1)
+    hgt = 0.816496 * math.sqrt(2.0)
+    assert np.allclose(opt[0, 0], 3.0 + 1.0 / 3.0 + hgt / math.sqrt(3.0), atol=1e-12)
+    assert np.allclose(opt[0, 1], [3.0 - hgt, 3.0 + 1.0 / 3.0, 3.0 + 1.0 / 3.0], atol=1e-12)
+    assert np.allclose(opt[0, 2], [3.0 + 1.0 / 3.0, 3.0 - hgt, 3.0 + 1.0 / 3.0], atol=1e-12)
+    assert np.allclose(opt[0, 3], [3.0 + 1.0 / 3.0, 3.0 + 1.0 / 3.0, 3.0 - hgt], atol=1e-12)
+
+
+def test_reference_nodesmooth_scatter_geometry_random_tets():
+    # for any positively oriented tet, each ideal position lies on the line
+    # through the centroid of the face opposite vertex f, perpendicular to
+    # that face, on the side away from vertex f, at 0.816496 times the
+    # longest of the face's three edges
+    rng = np.random.default_rng(8)
+    pts, tets = [], []
+    while len(tets) < 60:
+        q = rng.normal(size=(4, 3))
+        if np.dot(np.cross(q[1] - q[0], q[2] - q[0]), q[3] - q[0]) < 0.05:
+            continue
+        tets.append(np.arange(4) + 4 * len(tets))
+        pts.append(q)
+    P = np.vstack(pts)
+    tet = np.array(tets, np.int32)
+    deg, ball, _ = O.vertex_ball(tet, len(P), 32)
+    _, opt, _ = O.ref_nodesmooth64(tet, deg, ball, np.hstack([P, np.zeros((len(P), 1))]), 1)
+    for t in range(len(tet)):
+        q = P[tet[t]]
+        for f in range(4):
+            face = np.delete(q, f, axis=0)
+            c = face.mean(axis=0)
+            longest = max(np.linalg.norm(face[i] - face[j]) for i, j in ((0, 1), (0, 2), (1, 2)))
+            nrm = np.cross(face[1] - face[0], face[2] - face[0])
+            nrm /= np.linalg.norm(nrm)
+            if np.dot(nrm, q[f] - c) > 0:
+                nrm = -nrm
+            assert np.allclose(opt[t, f], c + 0.816496 * longest * nrm, atol=1e-10), (t, f)
+
+
+def test_reference_nodesmooth_gather_closed_form_bipyramid():
+    # two regular tets sharing face (a, b, c), apexes d and e = mirror of d.
+    # In a regular tet the ideal position of a vertex is its mirror image
+    # through the opposite face (up to 0.816496 / sqrt(2/3)).  Gather: the
+    # ball mean of the ideal positions chosen by each entry's local index,
+    # moved 0.2 of the way, residual |old - new|^2 = 0.04 |mean - old|^2.
+    k = 0.816496 / math.sqrt(2.0 / 3.0)
+    a, b, c, d = (np.array(x) for x in REG)
+    ctr = (a + b + c) / 3.0
+    e = 2 * ctr - d
+    P = np.array([a, b, c, d, e]) + 2.0
+    tet = np.array([[0, 1, 2, 3], [0, 2, 1, 4]], np.int32)      # both positively oriented
+    deg, ball, _ = O.vertex_ball(tet, 5, 32)
+    new, opt, res = O.ref_nodesmooth64(tet, deg, ball, np.hstack([P, np.zeros((5, 1))]), 1)
+
+    def ideal(t, j):
+        """mirror of vertex j of tet t through its opposite face, scaled by k"""
+        q = P[tet[t]]
+        f = np.delete(q, j, axis=0).mean(axis=0)
+        return f + k * (f - q[j])
+
+    for v in range(5):
+        entries = [(t, list(tet[t]).index(v)) for t in range(2) if v in tet[t]]
+        mean = np.mean([ideal(t, j) for t, j in entries], axis=0)
+        want = P[v] + 0.2 * (mean - P[v])
+        assert np.allclose(new[v], want, atol=1e-12), v
+        assert res[v] == pytest.approx(0.04 * np.sum((mean - P[v]) ** 2), rel=1e-10)
+    # the apexes: each moves 0.2 of the way to (k-scaled) the other apex
+    assert np.allclose(new[3], P[3] + 0.2 * (1 + k) / 2 * (P[4] - P[3]), atol=1e-12)
+    # the height of a regular tet of edge s is sqrt(2/3) s: residual of an apex
+    s = np.linalg.norm(a - b)
+    assert res[3] == pytest.approx(0.04 * ((1 + k) * math.sqrt(2.0 / 3.0) * s) ** 2, rel=1e-10)
+
+
+def test_bsr_spmv_f32_exact_small_integers():
+    # small-integer values and x: every FP32 product and sum is exact, so the
+    # FP32 product must equal the exact integer product, with non-symmetric
+    # blocks (a transposed block or swapped row/column shows)
+    rng = np.random.default_rng(4)
+    for b in (1, 3, 4, 5):
+        nb = 30
+        A = sp.random(nb, nb, density=0.2, random_state=b, format="csr")
+        A = (A + sp.eye(nb)).tocsr()
+        rowptr, col = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+        val = rng.integers(-8, 9, size=(len(col), b, b)).astype(np.float32)
+        x = rng.integers(-8, 9, size=nb * b).astype(np.float32)
+        y = np.empty(nb * b, np.float32)
+        O.lib().orc_bsr_spmv_f32(nb, b, O.p(rowptr), O.p(col), O.p(val), O.p(x), O.p(y))
+        want = np.zeros(nb * b, np.int64)
+        for r in range(nb):
+            for kk in range(rowptr[r], rowptr[r + 1]):
+                want[r * b:(r + 1) * b] += val[kk].astype(np.int64) @ x[col[kk] * b:(col[kk] + 1) * b].astype(np.int64)
+        assert np.array_equal(y.astype(np.int64), want), b
+
+
+def test_bsr_spmv_f32_within_rounding_of_f64():
+    # FP32 accumulation of a row with m terms: |y32 - y64| <= gamma_m sum |a||x|
+    rng = np.random.default_rng(5)
+    nb, b = 200, 3
+    A = (sp.random(nb, nb, density=0.08, random_state=2, format="csr") + sp.eye(nb)).tocsr()
+    rowptr, col = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    val = rng.normal(size=(len(col), b, b)).astype(np.float32)
+    x = rng.normal(size=nb * b).astype(np.float32)
+    y = np.empty(nb * b, np.float32)
+    O.lib().orc_bsr_spmv_f32(nb, b, O.p(rowptr), O.p(col), O.p(val), O.p(x), O.p(y))
+    y64 = O.bsr_spmv64(rowptr, col, val, x.astype(np.float64), b)
+    absv = O.bsr_spmv64(rowptr, col, np.abs(val), np.abs(x).astype(np.float64), b)
+    m = (np.diff(rowptr) * b).repeat(b)
+    u = np.finfo(np.float32).eps / 2
+    assert np.all(np.abs(y - y64) <= (m * u / (1 - m * u)) * absv + 1e-30)
 
 
 def test_bsr_spmv_vs_scipy():
