@@ -1,17 +1,27 @@
 """Benchmark of the GMlib kernel engine on one MI355X.
 
-Workload (one "step"): Laplacian node smoothing at scale -- a 256^3-cell Kuhn
-cube mesh (16,974,593 vertices, 100,663,296 tets, FP32 coordinates jittered
-by +-0.2h, seeded, boundary vertices fixed), 10 Jacobi sweeps (tet
-barycentre scatter + vertex-ball gather), then a tet-quality recompute and a
-GmlReduceVector min.  Kernels run through the drop-in C API (GmlCompileKernel
-/ GmlLaunchKernel / GmlReduceVector) with inputs resident in HBM.
+Headline workload (one "step"): Laplacian node smoothing at scale -- a
+256^3-cell Kuhn cube mesh (16,974,593 vertices, 100,663,296 tets, FP32
+coordinates jittered by +-0.2h, seeded, boundary vertices fixed), 10 Jacobi
+sweeps (tet barycentre scatter + vertex-ball gather), then a tet-quality
+recompute and a GmlReduceVector min.  The reported path is the drop-in C API
+(GmlCompileKernel / GmlLaunchKernel / GmlReduceVector over the generated
+kernels of gmlib_amd/kernels/*.cl) with inputs resident in HBM.
 
-Metric: algorithmic HBM GB/s of the whole step (each datum counted once:
-sweep 64T + 56V bytes, quality 20T + 16V, min 4T), plus the dominant
-kernel's roofline fraction against the 8 TB/s HBM peak, timed with HIP
-events on the library's stream, and the CPU baseline (the oracle's FP32
-restatement of the same sweep, OpenMP on the host cores, bounded sample).
+Metric: `value` is the drop-in step's algorithmic HBM bytes (each datum
+counted once: scatter 32T + 16V, gather 32T + 40V per sweep, quality
+20T + 16V, min 4T -- the bytes this path moves at least) per second.  The
+hand-written path (GmlxSmoothSweeps + GmlxTetQuality) is timed beside it
+from the same initial coordinates, with its own bytes per second (its
+operator-form sweep moves far fewer bytes) and, under model_equivalent_GBs,
+the drop-in byte model divided by its step time -- a work rate, not an HBM
+rate.  roofline: the dominant kernel's algorithmic bytes over its average
+duration (HIP events on the library's stream) against the 8 TB/s peak.
+parity: after the timed steps each path's coordinates against the oracle's
+FP32 sweeps run from the same initial coordinates for as many sweeps, the
+min quality bit-exact against the GPU's own quality array, the quality
+against FP64 on the GPU's coordinates; that oracle run, timed on the host
+cores, is the CPU baseline.
 
 Second workload (--workload cg): block-sparse SpMV and block-Jacobi CG on the
 3x3-block vector Laplacian of the edge graph of a 128^3-cell Kuhn cube
@@ -70,8 +80,11 @@ numbered at random, then after GmlxHilbertRenumber; the example's own byte
 figures are printed beside them.  The CPU baseline is the oracle's OpenMP
 loops of the same four patterns on the renumbered mesh.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload smoothing|cg|batch|serve|extract|heat|bandwidth]
-Prints one JSON line on stdout; progress goes to stderr.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+                       [--workload all|smoothing|small|cg|batch|serve|extract|heat|bandwidth]
+The default (all) measures the smoothing workload as the headline and the
+20^3 quality, CG, batch and serve workloads as sub-records of the same line
+("configs").  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import ctypes as C
@@ -105,7 +118,10 @@ def new_handle(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="smoothing", choices=["smoothing", "cg", "batch", "serve", "extract", "heat", "bandwidth"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "smoothing", "small", "cg", "batch", "serve", "extract", "heat", "bandwidth"])
+    ap.add_argument("--configs", default="1,3,4,5",
+                    help="all workload: the other configurations measured after the headline one")
     ap.add_argument("--qps", type=float, default=40.0, help="serve workload: arrival rate (jobs/s)")
     ap.add_argument("--jobs", type=int, default=400, help="serve workload: jobs in the timed stream")
     ap.add_argument("--pool", type=int, default=24, help="serve workload: meshes in the pool")
@@ -137,30 +153,50 @@ def main():
         print("bench.py measures one GPU: the path is single-device (multi-GPU sharding is a design "
               "note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "cg":
-        run_cg(args)
-    elif args.workload == "batch":
-        run_batch(args)
-    elif args.workload == "serve":
-        run_serve(args)
-    elif args.workload == "extract":
-        run_extract(args)
-    elif args.workload == "heat":
-        run_heat(args)
-    elif args.workload == "bandwidth":
-        run_bandwidth(args)
-    else:
-        run_smoothing(args)
+    runs = {"smoothing": run_smoothing, "small": run_small, "cg": run_cg, "batch": run_batch, "serve": run_serve,
+            "extract": run_extract, "heat": run_heat, "bandwidth": run_bandwidth}
+    if args.workload != "all":
+        print(json.dumps(runs[args.workload](args)), flush=True)
+        return
+    # the headline configuration (256^3 smoothing), then each other
+    # configuration as a sub-record of the same line, with a bounded CPU
+    # baseline; a configuration that fails is recorded as such
+    out = run_smoothing(args)
+    sub = {}
+    names = {"1": ("config1_quality_20cube", run_small), "3": ("config3_spmv_cg_128cube", run_cg),
+             "4": ("config4_batch_512meshes", run_batch), "5": ("config5_served_stream", run_serve)}
+    for c in [x for x in args.configs.split(",") if x]:
+        name, fn = names[c]
+        a = argparse.Namespace(**vars(args))
+        a.cpu_seconds = min(args.cpu_seconds, 3.0)
+        t0 = time.time()
+        log("---- %s ----" % name)
+        try:
+            rec = fn(a)
+        except Exception as e:          # recorded, so the headline line still prints
+            log("%s failed: %r" % (name, e))
+            rec = {"error": repr(e)}
+        for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline"):
+            rec.pop(k, None)
+        rec["wall_s"] = round(time.time() - t0, 1)
+        sub[name] = rec
+    out["configs"] = sub
+    print(json.dumps(out), flush=True)
 
 
 def run_smoothing(args):
+    """The scale workload: both paths from the same initial coordinates, each
+    timed, each checked against the oracle's FP32 sweeps after every sweep
+    it ran (the oracle run is also the CPU baseline)."""
     import numpy as np
     import gmlib_amd as G
+    from tests import oracle_lib as O
 
     n = args.n
     t0 = time.time()
     crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
     V, T = len(crd), len(tet)
+    crd0 = np.ascontiguousarray(crd, np.float32)
     log("mesh %d^3: V=%d T=%d built in %.1fs" % (n, V, T, time.time() - t0))
 
     g = new_handle(args)
@@ -190,13 +226,22 @@ def run_smoothing(args):
         L.GmlxSync(h)
         log("copy_calib: %d launches of %d bytes read + %d bytes written" % (args.copy_calib, 16 * V, 16 * V))
 
-    crd0 = np.ascontiguousarray(crd)
-    sweep_bytes = 64 * T + 56 * V
+    def restore():
+        """the initial coordinates back into the vertex table"""
+        if not L.GmlSetDataBlock(h, G.GmlVertices, 0, V - 1, G.ptr(crd0),
+                                 C.c_void_p(crd0.ctypes.data + (V - 1) * 16), G.ptr(ref),
+                                 C.c_void_p(ref.ctypes.data + (V - 1) * 4)):
+            raise RuntimeError("GmlSetDataBlock failed")
+        L.GmlxSync(h)
+
+    # algorithmic bytes (each datum once) of the drop-in step, which moves
+    # exactly the byte model of the workload: scatter 32T + 16V, gather
+    # 32T + 40V, quality 20T + 16V, min 4T
     scat_bytes = 32 * T + 16 * V
     gath_bytes = 32 * T + 40 * V
     qual_bytes = 20 * T + 16 * V
     min_bytes = 4 * T
-    step_bytes = args.sweeps * sweep_bytes + qual_bytes + min_bytes
+    model_step = args.sweeps * (scat_bytes + gath_bytes) + qual_bytes + min_bytes
 
     def step_generated():
         for _ in range(args.sweeps):
@@ -211,8 +256,6 @@ def run_smoothing(args):
             raise RuntimeError("built-in launch failed")
         return g.reduce(qal, G.GmlMin)
 
-    fused_bytes = 16 * T + 40 * V      # tet->vertex table (as plan) + ball slots + coords r/w + deg + ref
-
     def ktimes(path):
         if path == "generated":
             return [L.GmlGetKernelRunTime(h, k) for k in (ks, kg, kq)]
@@ -225,20 +268,23 @@ def run_smoothing(args):
         qg, qp = (int(x) for x in args.quality_shape.split("/"))
         if not (L.GmlxSetBuiltinShape(h, G.GmlxBuiltinQuality, qg, 1) and L.GmlxSetBuiltinItems(h, G.GmlxBuiltinQuality, qp)):
             raise SystemExit("invalid --quality-shape %s" % args.quality_shape)
-    paths = {}
+    paths, final = {}, {}
+    nsteps = args.warmup + 1 + args.steps          # steps each path runs from the initial coordinates
     for path, step in (("generated", step_generated), ("builtin", step_builtin)):
         if path not in args.paths.split(","):
             continue
+        restore()
         for _ in range(args.warmup):
             qmin = step()
         if path == "generated":
-            # the launch-shape tuner of a generated kernel tries 11 shapes on
-            # its first launches: finish it for the quality kernel too, which a
+            # the launch-shape tuner of a generated kernel tries its shapes on
+            # the first launches: finish it for the quality kernel too, which a
             # step launches once, so that the timed steps are steady-state
+            # (quality launches do not move the coordinates)
             while L.GmlxGetKernelRows(h, kq) == 0:
                 if L.GmlLaunchKernel(h, kq) != 1:
                     raise RuntimeError("launch failed")
-            qmin = step()
+        qmin = step()
         sg = ktimes(path)
         L.GmlxSync(h)
         t0 = time.perf_counter()
@@ -251,92 +297,220 @@ def run_smoothing(args):
         if path == "generated":
             per = {"scatter": (scat_bytes, ek[0] / nl), "gather": (gath_bytes, ek[1] / nl),
                    "quality": (qual_bytes, ek[2] / args.steps)}
+            own_step = model_step
         else:
             info = (C.c_longlong * 9)()
             if not L.GmlxGetSweepPlanInfo(h, info):
                 raise RuntimeError("no sweep plan")
-            log("sweep plan: mode %d, tile %d, %d tiles, tile tets / T = %.3f, tile vertices / V = %.3f, "
-                "1-ring slots %d, %.2f GB" % (info[8], info[0], info[1], info[3] / T, info[2] / V, info[7], info[6] / 1e9))
-            # operator form: 1-ring slots (2 B; the degree is their count sum / 3) + ref (4 B) +
-            # old and new coordinates (16 B each), every datum once; exact fused form:
-            # tet->vertex rows as the plan's 8 B + ball slots 2 B x 4T + 40 V
-            sweep_alg = 2 * info[7] + 36 * V if info[8] == 2 else fused_bytes
+            log("sweep plan: mode %d, tile %d, %d tiles, tile vertices / V = %.3f, 1-ring slots %d, %.2f GB"
+                % (info[8], info[0], info[1], info[2] / V, info[7], info[6] / 1e9))
+            # operator form: 1-ring slots (2 B; the degree is their count sum
+            # / 3) + ref (4 B) + old and new coordinates (16 B each); exact
+            # fused form: tet->vertex rows as the plan's 8 B + ball slots 2 B
+            # x 4T + 40 V
+            sweep_alg = 2 * info[7] + 36 * V if info[8] == 2 else 16 * T + 40 * V
             per = {"sweep": (sweep_alg, ek[0] / nl), "quality": (qual_bytes, ek[1] / args.steps)}
+            own_step = args.sweeps * sweep_alg + qual_bytes + min_bytes
         kern = {k: {"avg_us": round(v[1] * 1e6, 2), "alg_GBs": round(v[0] / v[1] / 1e9, 1),
                     "frac": round(v[0] / v[1] / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": v[0]}
                 for k, v in per.items()}
         ms = dt / args.steps * 1e3
-        gbs = step_bytes / (dt / args.steps) / 1e9
-        paths[path] = {"ms_per_step": round(ms, 4), "GBs": round(gbs, 2), "kernels": kern, "min_quality": qmin}
-        log("%s: step %.3f ms, %.1f GB/s algorithmic; kernels %s; min quality %.6f" % (path, ms, gbs, kern, qmin))
+        rec = {"ms_per_step": round(ms, 4), "own_bytes_per_step": own_step,
+               "GBs": round(own_step / (dt / args.steps) / 1e9, 2),
+               "model_equivalent_GBs": round(model_step / (dt / args.steps) / 1e9, 2),
+               "kernels": kern, "min_quality": qmin}
+        # the final state: coordinates after nsteps steps, quality of the last
+        c, _ = g.vertices(V)
+        q = g.download_raw(qal, np.float32, (T,))
+        final[path] = (c[:, :3].copy(), q, qmin)
+        paths[path] = rec
+        log("%s: step %.3f ms, %.1f GB/s of its own bytes (%.1f GB/s model-equivalent); kernels %s; min quality %.6f"
+            % (path, ms, rec["GBs"], rec["model_equivalent_GBs"], kern, qmin))
     shapes = []
     for k in (ks, kg, kq):
         a, b = C.c_int(), C.c_int()
         L.GmlxGetKernelShape(h, k, C.byref(a), C.byref(b))
         shapes.append((a.value, b.value))
     log("generated kernel shapes (group, remap): %s" % shapes)
-    best = max(paths, key=lambda p: paths[p]["GBs"])
-    P = paths[best]
+    if "generated" in paths and "builtin" in paths:
+        paths["builtin"]["speedup_vs_dropin"] = round(paths["generated"]["ms_per_step"] / paths["builtin"]["ms_per_step"], 2)
+    head = "generated" if "generated" in paths else "builtin"
+    P = paths[head]
     dom = max(P["kernels"], key=lambda k: P["kernels"][k]["avg_us"] * (args.sweeps if k != "quality" else 1))
     D = P["kernels"][dom]
-    # HBM traffic per launch of the dominant kernel, as measured by the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs on this workload (null elsewhere)
+    # HBM traffic per launch of the dominant kernel, as measured by the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs on this workload
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_256cube.json")
     if n == 256 and os.path.exists(tf):
-        key = dom if dom != "quality" else ("quality_builtin" if best == "builtin" else "quality_generated")
+        key = dom if dom != "quality" else ("quality_builtin" if head == "builtin" else "quality_generated")
         traffic = json.load(open(tf)).get(key)
-    ms, gbs = P["ms_per_step"], P["GBs"]
+    ball_w = 32
 
-    # CPU baseline: the oracle's FP32 sweep on the host cores (bounded sample)
-    cpu = None
-    if args.cpu_seconds > 0:
-        from tests import oracle_lib as O
-        deg = np.empty(V, np.int32)
-        ball = np.empty((V, 32), np.int32)
-        if not L.GmlxGetUplinkTable(h, G.GmlVertices, G.GmlTetrahedra, G.ptr(deg), G.ptr(ball), 32):
-            raise RuntimeError("uplink download failed")
-        c = np.array(crd0, copy=True)
-        b4 = np.empty((T, 4), np.float32)
-        OL = O.lib()
-        OL.orc_smooth_sweep_f32(V, T, O.p(tet), O.p(ref), O.p(deg), O.p(ball), 32, O.p(c), O.p(b4), args.omega)
-        t0 = time.perf_counter()
-        ns = 0
-        while ns < 3 or time.perf_counter() - t0 < args.cpu_seconds:
-            OL.orc_smooth_sweep_f32(V, T, O.p(tet), O.p(ref), O.p(deg), O.p(ball), 32, O.p(c), O.p(b4), args.omega)
-            ns += 1
-        ct = time.perf_counter() - t0
-        cpu = {"value": round(ns * sweep_bytes / ct / 1e9, 2), "unit": "GB/s",
-               "cores": OL.orc_num_threads(), "kind": "port",
-               "sample": "%d Jacobi sweeps (scatter + gather) of the same %d^3 mesh, oracle FP32 C + OpenMP, "
-                         "%.2f s" % (ns, n, ct)}
-        log("cpu baseline: %s" % cpu)
+    # the oracle from the same initial coordinates, as many FP32 sweeps as
+    # each path ran; its wall time is the CPU baseline
+    S = nsteps * args.sweeps
+    deg = np.empty(V, np.int32)
+    ball = np.empty((V, ball_w), np.int32)
+    if not L.GmlxGetUplinkTable(h, G.GmlVertices, G.GmlTetrahedra, G.ptr(deg), G.ptr(ball), ball_w):
+        raise RuntimeError("uplink download failed")
+    g.close()
+    OL = O.lib()
+    c = np.array(crd0, copy=True)
+    b4 = np.empty((T, 4), np.float32)
+    t0 = time.perf_counter()
+    for _ in range(S):
+        OL.orc_smooth_sweep_f32(V, T, O.p(tet), O.p(ref), O.p(deg), O.p(ball), ball_w, O.p(c), O.p(b4), args.omega)
+    ct = time.perf_counter() - t0
+    cpu = {"value": round(S * (scat_bytes + gath_bytes) / ct / 1e9, 2), "unit": "GB/s",
+           "cores": OL.orc_num_threads(), "kind": "port",
+           "sample": "%d Jacobi sweeps (scatter + gather) of the same %d^3 mesh from the same initial coordinates, "
+                     "oracle FP32 C + OpenMP, %.2f s" % (S, n, ct)}
+    log("cpu baseline: %s" % cpu)
+    parity = {"oracle": "orc_smooth_sweep_f32, %d sweeps from the same initial coordinates" % S,
+              "coord_tolerance": 1e-5, "quality_rel_tolerance": 1e-5}
+    ok = True
+    for path, (cg, qg, qmin) in final.items():
+        dx = float(np.abs(cg - c[:, :3]).max())
+        # min quality: the GPU reduction against the same FP32 array
+        min_exact = qmin == float(qg.min())
+        # quality against FP64 on the GPU's own final coordinates
+        q64 = O.tet_quality64(tet, np.hstack([cg, np.zeros((V, 1), np.float32)]))
+        big = np.abs(q64) >= 0.05
+        qerr = float(np.max(np.abs(qg[big] - q64[big]) / np.abs(q64[big])))
+        parity[path] = {"max_abs_dx_vs_oracle": dx, "min_quality_bitexact": bool(min_exact),
+                        "quality_max_rel_err_vs_fp64": qerr}
+        ok &= dx <= 1e-5 and min_exact and qerr <= 1e-5
+    parity["ok"] = bool(ok)
+    log("parity: %s" % parity)
 
-    out = {
+    return {
         "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
-        "value": gbs,
+        "value": P["GBs"],
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": P["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered Kuhn cube mesh)",
         "config": {"workload": "Laplacian node smoothing, %d^3-cell Kuhn cube (%d vertices, %d tets), "
-                               "%d Jacobi sweeps + tet quality + min reduction" % (n, V, T, args.sweeps),
-                   "vertices": V, "tets": T, "sweeps": args.sweeps, "path": best,
-                   "bytes_per_step": step_bytes},
+                               "%d Jacobi sweeps + tet quality + min reduction, drop-in C API path "
+                               "(GmlCompileKernel / GmlLaunchKernel / GmlReduceVector)" % (n, V, T, args.sweeps),
+                   "vertices": V, "tets": T, "sweeps": args.sweeps, "path": head,
+                   "bytes_per_step": P["own_bytes_per_step"]},
         "roofline": {"bound": "hbm", "achieved": D["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": D["frac"], "traffic": traffic, "kernel": "%s (%s path)" % (dom, best),
+                     "frac": D["frac"], "traffic": traffic, "kernel": "%s (%s path)" % (dom, head),
                      "bytes_per_launch": D["bytes_per_launch"], "avg_us": D["avg_us"]},
         "paths": paths,
+        "parity": parity,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+
+
+def run_small(args):
+    """The CPU-runnable configuration: tet quality, ball minimum and
+    GmlReduceVector min / mean on a 20^3-cell cube (48,000 tets), through the
+    drop-in generated kernels and the hand-written ones, in microseconds and
+    GB/s, checked against the oracle."""
+    import math
+    import numpy as np
+    import gmlib_amd as G
+    from tests import oracle_lib as O
+
+    n = 20
+    crd, ref, tet = G.cube_mesh(n, 0.2, 2020, True)
+    V, T = len(crd), len(tet)
+    qb = 16 * T + 16 * V + 4 * T
+    bb = 16 * T + 4 * V + 4 * T + 4 * V
+    rb = 4 * T
+    g = new_handle(args)
+    L, h = g.L, g.h
+    vi, ti = g.set_mesh(crd, ref, tet)
+    q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+    vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+    kq = g.compile(body("tet_meanratio"), "tet_meanratio", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+    kb = g.compile(body("ball_min"), "ball_min", G.GmlVertices, [(q, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+    red = C.c_double()
+
+    def step(path):
+        if path == "generated":
+            if L.GmlLaunchKernel(h, kq) != 1 or L.GmlLaunchKernel(h, kb) != 1:
+                raise RuntimeError("launch failed")
+        elif L.GmlxTetQuality(h, q) != 1 or L.GmlxBallMin(h, q, vq) != 1:
+            raise RuntimeError("built-in launch failed")
+        L.GmlReduceVector(h, q, G.GmlMin, C.byref(red))
+        mn = red.value
+        L.GmlReduceVector(h, q, G.GmlSum, C.byref(red))
+        return mn, red.value / T
+
+    def times(path):
+        cnt = C.c_longlong()
+        if path == "generated":
+            k = [L.GmlGetKernelRunTime(h, kq), L.GmlGetKernelRunTime(h, kb)]
+        else:
+            k = [L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinQuality, C.byref(cnt)),
+                 L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinBallMin, C.byref(cnt))]
+        return k + [L.GmlGetReduceRunTime(h, G.GmlMin)]
+
+    steps = max(args.steps, 50)
+    res = {}
+    for path in ("generated", "builtin"):
+        for _ in range(20):                       # the generated kernels' shape tuners settle
+            step(path)
+        t0 = times(path)
+        w0 = time.perf_counter()
+        for _ in range(steps):
+            mn, mean = step(path)
+        wall = (time.perf_counter() - w0) / steps
+        dk = [(a - b) / steps for a, b in zip(times(path), t0)]
+        qg = g.download_raw(q, np.float32, (T,))
+        vg = g.download_raw(vq, np.float32, (V,))
+        res[path] = {"step_us": round(wall * 1e6, 2),
+                     "quality": {"avg_us": round(dk[0] * 1e6, 2), "GBs": round(qb / dk[0] / 1e9, 1)},
+                     "ball_min": {"avg_us": round(dk[1] * 1e6, 2), "GBs": round(bb / dk[1] / 1e9, 1)},
+                     "reduce_min": {"avg_us": round(dk[2] * 1e6, 2), "GBs": round(rb / dk[2] / 1e9, 1)},
+                     "min_quality": mn, "mean_quality": mean, "_q": qg, "_v": vg}
     g.close()
+    # parity: quality vs FP64, ball minimum and min bit-exact on the GPU's own
+    # FP32 qualities, mean vs the exact sum
+    q64 = O.tet_quality64(tet, crd)
+    deg, ball, _ = O.vertex_ball(tet, V, 32)
+    par = {}
+    ok = True
+    for path, r in res.items():
+        qg, vg = r.pop("_q"), r.pop("_v")
+        qerr = float(np.max(np.abs(qg - q64) / np.maximum(np.abs(q64), 1e-3)))
+        ball_exact = bool(np.array_equal(vg, O.ball_min(deg, ball, qg)))
+        min_exact = r["min_quality"] == float(qg.min())
+        mean_err = abs(r["mean_quality"] - math.fsum(q64) / T) / abs(math.fsum(q64) / T)
+        par[path] = {"quality_max_rel_err_vs_fp64": qerr, "ball_min_bitexact": ball_exact,
+                     "min_bitexact": bool(min_exact), "mean_rel_err_vs_fp64": mean_err}
+        ok &= qerr <= 1e-5 and ball_exact and min_exact and mean_err <= 1e-5
+    par["ok"] = bool(ok)
+    # CPU baseline: the oracle's FP32 quality, ball minimum and reductions
+    OL = O.lib()
+    t0 = time.perf_counter()
+    k = 0
+    while k < 5 or time.perf_counter() - t0 < min(args.cpu_seconds, 2.0):
+        q32 = O.tet_quality32(tet, crd)
+        O.ball_min(deg, ball, q32)
+        O.reduce(G.GmlMin, q32)
+        O.reduce(G.GmlSum, q32)
+        k += 1
+    ct = (time.perf_counter() - t0) / k
+    out = {"workload": "tet quality + ball min + GmlReduceVector min / mean, 20^3-cell Kuhn cube (%d vertices, "
+                       "%d tets)" % (V, T),
+           "vertices": V, "tets": T, "bytes": {"quality": qb, "ball_min": bb, "reduce": rb},
+           "paths": res, "parity": par,
+           "cpu_baseline": {"value": round((qb + bb + 2 * rb) / ct / 1e9, 2), "unit": "GB/s", "step_us": round(ct * 1e6, 1),
+                            "cores": OL.orc_num_threads(), "kind": "port",
+                            "sample": "%d oracle steps (FP32 quality, ball min, min and sum), C + OpenMP" % k}}
+    log("config 1 (20^3): %s" % out)
+    return out
 
 
 def run_cg(args):
@@ -513,12 +687,12 @@ def run_cg(args):
         "stream_probe_GBs": probe,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     g.close()
+    return out
 
 
 def serve_stream(args, seed=2025):
-    """(pool of (n, seed) meshes, job list of (type, n, seed, param), arrival times)."""
+    """(pool of (n, seed) meshes, job list of (type, pool index, param), arrival times)."""
     import numpy as np
     import gmlib_amd as G
     rng = np.random.default_rng(seed)
@@ -527,25 +701,25 @@ def serve_stream(args, seed=2025):
     kinds = rng.choice([G.GmlxJobQuality, G.GmlxJobSmooth, G.GmlxJobCG], size=args.jobs, p=[0.5, 0.3, 0.2])
     which = rng.integers(len(pool), size=args.jobs)
     param = {G.GmlxJobQuality: 0.0, G.GmlxJobSmooth: 20.0, G.GmlxJobCG: 1e-8}
-    jobs = [(int(k), pool[w][0], pool[w][1], param[int(k)]) for k, w in zip(kinds, which)]
+    jobs = [(int(k), int(w), param[int(k)]) for k, w in zip(kinds, which)]
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=args.jobs))
     return pool, jobs, arrivals
 
 
-def serve_run(L, srv, jobs, arrivals=None):
-    """Submit jobs (at their arrival times, or all at once) and wait for all:
-    list of (rc, res[4], tim[4])."""
+def serve_run(L, srv, keys, jobs, arrivals=None):
+    """Submit jobs on registered meshes (at their arrival times, or all at
+    once) and wait for all: list of (rc, res[4], tim[4])."""
     import threading
     ids = [None] * len(jobs)
     t0 = time.perf_counter()
 
     def submitter():
-        for i, (t, n, s, p) in enumerate(jobs):
+        for i, (t, w, p) in enumerate(jobs):
             if arrivals is not None:
                 d = t0 + arrivals[i] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-            ids[i] = L.GmlxServeSubmit(srv, t, n, s, p)
+            ids[i] = L.GmlxServeSubmitMesh(srv, t, keys[w], p)
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -571,26 +745,53 @@ def latency_stats(out):
             "service_p99_ms": round(float(np.percentile(svc, 99)) * 1e3, 2)}
 
 
+def register_pool(L, srv, meshes):
+    keys = []
+    for crd, ref, tet in meshes:
+        k = L.GmlxServeAddMesh(srv, len(crd), len(tet), G_ptr(crd), G_ptr(ref), G_ptr(tet))
+        if k <= 0:
+            raise RuntimeError("GmlxServeAddMesh failed")
+        keys.append(k)
+    return keys
+
+
+def G_ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
 def run_serve(args):
+    """Served stream against the plain per-job path.  Both servers get the
+    same host meshes, generated before either runs and registered with
+    GmlxServeAddMesh, and the plain server is warmed so that the host-side CG
+    systems of its jobs exist: a plain job is GmlInit, upload, links and
+    plans built on the device, run, reduce, GmlStop, one at a time."""
+    import math
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     import gmlib_amd as G
+    from tests import oracle_lib as O
     L = G.lib()
     pool, jobs, arrivals = serve_stream(args)
     tets = [6 * n ** 3 for n, _ in pool]
-    log("serve: pool of %d meshes, %d to %d tets (%.1fM total); %d jobs at %.1f jobs/s; %d slots, budget %.0f GB"
-        % (len(pool), min(tets), max(tets), sum(tets) / 1e6, len(jobs), args.qps, args.slots, args.budget_gb))
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        meshes = list(ex.map(lambda ps: G.cube_mesh(ps[0], 0.2, ps[1]), pool))
+    log("serve: pool of %d meshes, %d to %d tets (%.1fM total), generated in %.1fs; %d jobs at %.1f jobs/s; "
+        "%d slots, budget %.0f GB" % (len(pool), min(tets), max(tets), sum(tets) / 1e6, time.time() - t0, len(jobs),
+                                       args.qps, args.slots, args.budget_gb))
     srv = L.GmlxServeStart(0, args.budget_gb, args.slots, 1)
     if not srv:
         raise RuntimeError("GmlxServeStart failed")
-    # warm-up: every mesh loaded, its smoothing plan and CG matrix built
+    keys = register_pool(L, srv, meshes)
+    # warm-up: every mesh loaded, its smoothing plan and CG system built
     t0 = time.time()
-    warm = [(k, n, s, p) for n, s in pool for k, p in ((G.GmlxJobSmooth, 20.0), (G.GmlxJobCG, 1e-8))]
-    wout = serve_run(L, srv, warm)
+    warm = [(k, w, p) for w in range(len(pool)) for k, p in ((G.GmlxJobSmooth, 20.0), (G.GmlxJobCG, 1e-8))]
+    wout = serve_run(L, srv, keys, warm)
     if any(o[0] != 1 for o in wout):
         raise RuntimeError("warm-up jobs failed: %s" % [o[0] for o in wout])
-    log("warm-up (%d jobs: load, plans, matrices): %.1fs" % (len(warm), time.time() - t0))
+    log("warm-up (%d jobs: load, plans, CG systems): %.1fs" % (len(warm), time.time() - t0))
     # the timed open-loop stream at the fixed rate
-    out = serve_run(L, srv, jobs, arrivals)
+    out = serve_run(L, srv, keys, jobs, arrivals)
     if any(o[0] != 1 for o in out):
         raise RuntimeError("served jobs failed: %s" % sorted(set(o[0] for o in out)))
     st = latency_stats(out)
@@ -606,23 +807,74 @@ def run_serve(args):
     log("served at %.1f jobs/s offered: %s; by kind %s; resident %d meshes, %.2f GB, %d loads, %d evictions, "
         "up to %d jobs at once" % (args.qps, st, by_kind, info[0], info[1] / 1e9, info[2], info[3], info[5]))
     # saturated: the same jobs submitted at once
-    sat = latency_stats(serve_run(L, srv, jobs))
+    sat = latency_stats(serve_run(L, srv, keys, jobs))
     log("served, saturated (all %d jobs at once): %s" % (len(jobs), sat))
     L.GmlxServeStop(srv)
 
     # plain per-job path on the first jobs of the stream, results compared
     npl = min(len(jobs), 40)
     plain = L.GmlxServeStart(0, args.budget_gb, 1, 0)
-    pout = serve_run(L, plain, jobs[:npl])
+    pkeys = register_pool(L, plain, meshes)
+    cg_meshes = sorted(set(w for t, w, _ in jobs[:npl] if t == G.GmlxJobCG))
+    t0 = time.time()
+    if any(o[0] != 1 for o in serve_run(L, plain, pkeys, [(G.GmlxJobCG, w, 1e-8) for w in cg_meshes])):
+        raise RuntimeError("plain warm-up failed")
+    log("plain warm-up (host CG systems of %d meshes): %.1fs" % (len(cg_meshes), time.time() - t0))
+    pout = serve_run(L, plain, pkeys, jobs[:npl])
     L.GmlxServeStop(plain)
     if any(o[0] != 1 for o in pout):
         raise RuntimeError("plain jobs failed")
     pst = latency_stats(pout)
     same = all(a[1][:2] == b[1][:2] for a, b in zip(pout, out[:npl]))
     log("plain per-job path (%d jobs, closed loop): %s; results equal to the served ones: %s" % (npl, pst, same))
-    if not same:
-        raise RuntimeError("served and plain results differ")
-    outj = {
+
+    # served results against the oracle, on the jobs of meshes up to 2M tets:
+    # quality (min bit-exact is not expected across FP32 and FP64 -- min and
+    # mean within 1e-5 relative of the FP64 quality), smoothing (20 FP32
+    # oracle sweeps, then FP64 quality of those coordinates), CG (the GPU's
+    # FP64 relative residual at or below the tolerance)
+    checks = {"quality": [], "smooth": [], "cg": []}
+    cache = {}
+    t0 = time.time()
+    for (t, w, p), o in zip(jobs, out):
+        if tets[w] > 2e6:
+            continue
+        crd, ref, tet = meshes[w]
+        if t == G.GmlxJobCG:
+            checks["cg"].append(o[1][0] <= p)
+            continue
+        key = (t, w)
+        if key not in cache:
+            if t == G.GmlxJobQuality:
+                q = O.tet_quality64(tet, crd)
+            else:
+                deg, ball, _ = O.vertex_ball(tet, len(crd), 32)
+                c = O.smooth32(tet, ref, deg, ball, crd, int(p), 0.2)
+                q = O.tet_quality64(tet, c)
+            cache[key] = (float(q.min()), math.fsum(q) / len(q))
+        mn, mean = cache[key]
+        name = "quality" if t == G.GmlxJobQuality else "smooth"
+        checks[name].append((abs(o[1][0] - mn) / abs(mn), abs(o[1][1] - mean) / abs(mean)))
+
+    def worst(v, i):
+        return max(e[i] for e in v) if v else None
+    # a served smoothing runs the operator-form sweep, equal to the oracle's
+    # two-kernel sweep up to FP32 rounding of the coordinates; the minimum
+    # quality moves by about that rounding over the edge length, so its bound
+    # is 1e-4 where the mean's (errors average out) stays 1e-5
+    oracle = {"quality_jobs": len(checks["quality"]),
+              "quality_min_max_rel_err": worst(checks["quality"], 0),
+              "quality_mean_max_rel_err": worst(checks["quality"], 1),
+              "smooth_jobs": len(checks["smooth"]),
+              "smooth_min_max_rel_err": worst(checks["smooth"], 0),
+              "smooth_mean_max_rel_err": worst(checks["smooth"], 1),
+              "cg_jobs": len(checks["cg"]), "cg_all_at_tolerance": all(checks["cg"]),
+              "tolerances": {"quality": 1e-5, "smooth_min": 1e-4, "smooth_mean": 1e-5},
+              "seconds": round(time.time() - t0, 1)}
+    oracle["ok"] = bool(all(a <= 1e-5 and b <= 1e-5 for a, b in checks["quality"])
+                        and all(a <= 1e-4 and b <= 1e-5 for a, b in checks["smooth"]) and all(checks["cg"]))
+    log("served results vs the oracle: %s" % oracle)
+    return {
         "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
         "value": st["jobs_per_s"],
         "unit": "jobs/s",
@@ -642,10 +894,10 @@ def run_serve(args):
         "served": st, "served_by_kind": by_kind, "served_saturated": sat,
         "plain_per_job": dict(pst, jobs=npl, results_equal=same),
         "speedup_saturated_vs_plain": round(sat["jobs_per_s"] / pst["jobs_per_s"], 2),
+        "oracle": oracle,
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(outj), flush=True)
 
 
 def batch_sizes(nmesh, seed=2024):
@@ -835,8 +1087,8 @@ def run_batch(args):
                                          "(balls built beforehand), min and sum reductions, C + OpenMP, %.2f s"
                                          % (i, done_t, ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
-    print(json.dumps(out), flush=True)
     g.close()
+    return out
 
 
 def extract_bytes(T, E, F):
@@ -943,7 +1195,7 @@ def run_extract(args):
                                          "of the reference hash (GetItmNod / CalHshKey / AddHsh / GetHsh), one "
                                          "thread, %.2f s" % (m, ct_n, ce, cf, c)}
         log("cpu baseline: %s" % out["cpu_baseline"])
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def heat_bytes(T, F, V):
@@ -1033,7 +1285,7 @@ def run_heat(args):
                                "sample": "%d iterations of the numpy FP64 restatement (links and geometry set up "
                                          "once per call) on a %d^3 cube (%d tets), %.2f s" % (k, m, len(tt), ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bandwidth_models(V, T):
@@ -1180,7 +1432,7 @@ def run_bandwidth(args):
                                "sample": "%d passes of the four patterns over the same Hilbert-numbered %d^3 mesh, "
                                          "oracle C + OpenMP (direct vertex copy: numpy), %.2f s" % (ns, n, ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
