@@ -106,6 +106,111 @@ static std::string SinglePrecisionLiterals(const char *s)
 }
 
 // ---------------------------------------------------------------------------
+// OpenCL vector literals of several parts, (float4)(a, b, c, d) or
+// (float4)(v.xy, w.zw): a C++ compiler reads them as the cast of a comma
+// expression (every component = the last part), so they are rewritten to
+// gml_vmake<float4>(...), the prelude's component-wise constructor.  A
+// single part, (float4)(x), is a splat in both languages and stays.  Parts
+// are rewritten recursively; comments, strings and character literals are
+// copied as they are.
+// ---------------------------------------------------------------------------
+static bool IsVectorType(const std::string &t)
+{
+   static const char *base[] = {"char", "uchar", "short", "ushort", "int", "uint", "long", "ulong", "float", "double"};
+   for(const char *b : base)
+   {
+      const size_t n = strlen(b);
+      if(t.size() > n && t.compare(0, n, b) == 0)
+      {
+         const std::string w = t.substr(n);
+         if(w == "2" || w == "3" || w == "4" || w == "8" || w == "16") return true;
+      }
+   }
+   return false;
+}
+
+// end of the comment, string or character literal starting at i (i itself
+// if none starts there)
+static size_t SkipOpaque(const std::string &s, size_t i)
+{
+   const size_t n = s.size();
+   if(s[i] == '/' && i + 1 < n && s[i + 1] == '/')
+   {
+      while(i < n && s[i] != '\n') i++;
+      return i;
+   }
+   if(s[i] == '/' && i + 1 < n && s[i + 1] == '*')
+   {
+      size_t j = i + 2;
+      while(j + 1 < n && !(s[j] == '*' && s[j + 1] == '/')) j++;
+      return j + 2 <= n ? j + 2 : n;
+   }
+   if(s[i] == '"' || s[i] == '\'')
+   {
+      const char q = s[i];
+      size_t j = i + 1;
+      while(j < n && s[j] != q) { if(s[j] == '\\') j++; j++; }
+      return j < n ? j + 1 : n;
+   }
+   return i;
+}
+
+static std::string VectorLiterals(const std::string &s)
+{
+   std::string out;
+   const size_t n = s.size();
+   size_t i = 0;
+   auto isid = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
+   auto ws = [&](size_t j) { while(j < n && isspace((unsigned char)s[j])) j++; return j; };
+   while(i < n)
+   {
+      const size_t o = SkipOpaque(s, i);
+      if(o != i) { out.append(s, i, o - i); i = o; continue; }
+      if(s[i] == '(' && (i == 0 || !isid(s[i - 1])))
+      {
+         // "(" type ")" "(" parts ")"
+         size_t j = ws(i + 1), t0 = j;
+         while(j < n && isid(s[j])) j++;
+         const std::string typ = s.substr(t0, j - t0);
+         size_t k = ws(j);
+         if(IsVectorType(typ) && k < n && s[k] == ')')
+         {
+            size_t a = ws(k + 1);
+            if(a < n && s[a] == '(')
+            {
+               // matching parenthesis and top-level commas of the parts
+               int depth = 0, commas = 0;
+               size_t e = a;
+               for(; e < n; e++)
+               {
+                  const size_t q = SkipOpaque(s, e);
+                  if(q != e) { e = q - 1; continue; }
+                  const char c = s[e];
+                  if(c == '(' || c == '[' || c == '{') depth++;
+                  else if(c == ')' || c == ']' || c == '}') { if(--depth == 0) break; }
+                  else if(c == ',' && depth == 1) commas++;
+               }
+               if(e < n && commas > 0)
+               {
+                  out += "gml_vmake<" + typ + ">(" + VectorLiterals(s.substr(a + 1, e - a - 1)) + ")";
+                  i = e + 1;
+                  continue;
+               }
+            }
+         }
+      }
+      out.push_back(s[i]);
+      i++;
+   }
+   return out;
+}
+
+static std::string UserCode(const char *s)
+{
+   return VectorLiterals(SinglePrecisionLiterals(s));
+}
+
+// ---------------------------------------------------------------------------
 // Code-object cache shared by every handle of the process (hipRTC output for
 // gfx950 keyed by a 64-bit FNV-1a hash of source and options).
 // ---------------------------------------------------------------------------
@@ -357,12 +462,12 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    if(g->UsrTlk)
    {
       s += "// USER'S TOOLKIT\n";
-      s += SinglePrecisionLiterals(g->UsrTlk);
+      s += UserCode(g->UsrTlk);
       s += "\n";
    }
    s += "// USER'S ARGUMENTS STRUCTURE\n";
    if(ParSrc)
-      s += SinglePrecisionLiterals(ParSrc);
+      s += UserCode(ParSrc);
    else
       s += "typedef struct { int GmlNoParameters; } GmlParSct;";
    s += "\n";
@@ -598,7 +703,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    s = body_s;
 
    // --- body
-   const std::string user = "// USER'S KERNEL CODE\n" + SinglePrecisionLiterals(KrnSrc) + "\n";
+   const std::string user = "// USER'S KERNEL CODE\n" + UserCode(KrnSrc) + "\n";
 
    // --- writes (gmlib3.c:2903-2938): only the loop entity's own line
    std::string writes_s;

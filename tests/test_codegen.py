@@ -144,29 +144,23 @@ def test_compile_error_is_reported(mesh, capfd):
 
 
 def test_lazy_uplink_accessor_and_voyeurs(mesh):
-    # the NodeSmoothing pattern: 4 x float4 per tet read through the ball
-    # with voyeurs; only the selected item is fetched
+    # 4 x float4 per tet read through the ball with voyeurs, the item chosen
+    # by each entry's local index: only that item is fetched
+    from tests.test_gpu_parity import FACE_PULL_GATHER, FACE_PULL_SCATTER
     crd, ref, tet = mesh
-    bod = """{
-   float4 s = (float4)(0.f);
-   for(int i = 0; i < OptCrdDegMax; i++)
-   {
-      s += (VerTetVoy[i] == 0) ? OptCrd[i][0] : (float4)(0.f);
-      s += (VerTetVoy[i] == 1) ? OptCrd[i][1] : (float4)(0.f);
-      s += (VerTetVoy[i] == 2) ? OptCrd[i][2] : (float4)(0.f);
-      s += (VerTetVoy[i] == 3) ? OptCrd[i][3] : (float4)(0.f);
-   }
-   res = dot(s, s);
-}"""
     with G.Gml(host_only=True) as g:
         vi, ti = g.set_mesh(crd, ref, tet)
-        o = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "OptCrd")
-        r = g.solution(G.GmlVertices, 1, G.GmlFlt, "res")
-        k = g.compile(bod, "gat", G.GmlVertices, [(r, G.GmlWriteMode, 0), (o, G.GmlReadMode | G.GmlVoyeurs, 0)])
+        o = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "Pull")
+        r = g.solution(G.GmlVertices, 1, G.GmlFlt, "move")
+        ks = g.compile(FACE_PULL_SCATTER, "sca", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (o, G.GmlWriteMode, 0)])
+        k = g.compile(FACE_PULL_GATHER, "gat", G.GmlVertices,
+                      [(r, G.GmlWriteMode, 0), (o, G.GmlReadMode | G.GmlVoyeurs, 0), (vi, G.GmlReadMode | G.GmlWriteMode, 0)])
         s = src_of(g, k)
+        assert ks > 0
         assert "char      VerTetVoy[32];" in s
         assert "VerTetVoy[k] = (char)(v & 7);" in s
-        assert "gml_upn<float4, 4, 32> OptCrd{OptCrdTab, gml_VerTetBal, OptCrdDeg, OptCrdNul};" in s
+        assert "gml_upn<float4, 4, 32> Pull{PullTab, gml_VerTetBal, PullDeg, PullNul};" in s
+        assert "#define   PullDegMax 32" in s
 
 
 # --- the reference's example bodies, unmodified ------------------------------
@@ -343,3 +337,39 @@ def test_short_shells_gather_directly_and_pair(mesh):
         se = src_of(g, ke)
     assert "gml_stg<" not in se and "gml_v_Bary[gml_u] = BaryTab[gml_x[gml_u]];" in se
     assert "Bary[0] = (BaryDeg >= 1) ? gml_s_Bary[ gml_l_VerTetBal[0] ] : BaryNul;" in se
+
+
+def test_opencl_vector_literals_rewritten(mesh):
+    # (T)(a, b, ...) of several parts is an OpenCL vector literal; a C++
+    # compiler would read a comma expression.  The engine builds it with the
+    # prelude's component-wise gml_vmake; a one-part (T)(x) stays a splat
+    crd, ref, tet = mesh
+    bod = """{
+   float4 a = (float4)(1.f, 2.f, 3.f, 4.f);
+   float2 h = (float2)(a.x, 0.5);
+   float4 b = (float4)( h, (float2)(a.zw) );       /* vectors as parts, a nested literal */
+   int4   n = (int4)(1, 2, (int2)(3, 4));
+   float4 s = (float4)(0.25);
+   const char *str = "(float4)(1, 2)";            /* strings are left alone */
+   // (float4)(5, 6, 7, 8) in a comment too
+   qal = dot(a + b + s, (float4)(n.x, n.y, n.z, n.w)) + (str[0] == '(' ? 0.f : 1.f);
+}"""
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        k = g.compile(bod, "vlit", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        s = src_of(g, k)
+    user = s[s.index("// USER'S KERNEL CODE"):]
+    assert "gml_vmake<float4>(1.f, 2.f, 3.f, 4.f)" in user
+    assert "gml_vmake<float2>(a.x, 0.5f)" in user
+    assert "gml_vmake<float4>( h, (float2)(a.zw) )" in user
+    assert "gml_vmake<int4>(1, 2, gml_vmake<int2>(3, 4))" in user
+    assert "(float4)(0.25f)" in user
+    assert '"(float4)(1, 2)"' in user and "// (float4)(5, 6, 7, 8) in a comment too" in user
+    # a literal with the wrong number of components does not compile
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        tri = (C.c_int * 6)(vi, G.GmlReadMode, 0, q, G.GmlWriteMode, 0)
+        assert g.L.GmlxCompileKernelv(g.h, b"{ float4 a = (float4)(1.f, 2.f, 3.f); qal = a.x; }", b"bad",
+                                      G.GmlTetrahedra, 2, tri) == 0
