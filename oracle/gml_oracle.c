@@ -550,6 +550,64 @@ void orc_ref_nodesmooth_sweep_f64(int nver, int ntet, const int *tet,
    }
 }
 
+/*
+ * Volume-weighted opposite-face pull, FP64: the two-body uplink pattern the
+ * codegen tests run (a tet body writing 4 float4 items per tet, a vertex
+ * body reading them through the vertex -> tet ball with voyeurs, the way
+ * examples/NodeSmoothing/NodeSmoothing.c:137-141 wires its bodies).  Per tet
+ * i and local vertex j: item (i, j) = (centroid of the face opposite j,
+ * w = |volume|).  Per vertex with a non-empty ball: target = sum over its
+ * ball entries (i, j) of w_i * centroid_ij / sum of w_i; new = p + 0.25 *
+ * (target - p); res = |new - p|^2.  An empty ball leaves p and res = 0.
+ * crd: nver * 3 doubles (updated), item: ntet * 4 * 4 doubles.
+ */
+void orc_face_pull_sweep_f64(int nver, int ntet, const int *tet, const int *deg, const int *ball, int width,
+                             double *crd, double *item, double *res)
+{
+   int i;
+#pragma omp parallel for schedule(static)
+   for(i = 0; i < ntet; i++)
+   {
+      double P[4][3], s[3] = {0., 0., 0.}, u[3], v[3], w[3], vol;
+      int j, k;
+      for(j = 0; j < 4; j++)
+         for(k = 0; k < 3; k++)
+         {
+            P[j][k] = crd[(size_t)tet[(size_t)i * 4 + j] * 3 + k];
+            s[k] += P[j][k];
+         }
+      for(k = 0; k < 3; k++) { u[k] = P[1][k] - P[0][k]; v[k] = P[2][k] - P[0][k]; w[k] = P[3][k] - P[0][k]; }
+      vol = fabs((u[1] * v[2] - u[2] * v[1]) * w[0] + (u[2] * v[0] - u[0] * v[2]) * w[1]
+               + (u[0] * v[1] - u[1] * v[0]) * w[2]) / 6.;
+      for(j = 0; j < 4; j++)
+      {
+         for(k = 0; k < 3; k++) item[((size_t)i * 4 + j) * 4 + k] = (s[k] - P[j][k]) / 3.;
+         item[((size_t)i * 4 + j) * 4 + 3] = vol;
+      }
+   }
+#pragma omp parallel for schedule(static)
+   for(i = 0; i < nver; i++)
+   {
+      double a[3] = {0., 0., 0.}, ws = 0., r = 0.;
+      int k, j, d = deg[i];
+      for(k = 0; k < d; k++)
+      {
+         const int e = ball[(size_t)i * width + k];
+         const double *it = &item[((size_t)(e >> 4) * 4 + (e & 7)) * 4];
+         for(j = 0; j < 3; j++) a[j] += it[3] * it[j];
+         ws += it[3];
+      }
+      if(d > 0)
+         for(j = 0; j < 3; j++)
+         {
+            const double p = crd[(size_t)i * 3 + j], n = p + (a[j] / ws - p) * 0.25;
+            r += (n - p) * (n - p);
+            crd[(size_t)i * 3 + j] = n;
+         }
+      res[i] = r;
+   }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Block sparse algebra                                                      */
 /* ------------------------------------------------------------------------ */

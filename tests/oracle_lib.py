@@ -47,6 +47,7 @@ def lib():
             "orc_smooth_sweep_f32": (None, [_I, _I, _P, _P, _P, _P, _I, _P, _P, _F]),
             "orc_smooth_sweep_f64": (None, [_I, _I, _P, _P, _P, _P, _I, _P, _P, _D]),
             "orc_ref_nodesmooth_sweep_f64": (None, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
+            "orc_face_pull_sweep_f64": (None, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
             "orc_bsr_spmv_f64": (None, [_I, _I, _P, _P, _P, _P, _P]),
             "orc_bsr_spmv_f32": (None, [_I, _I, _P, _P, _P, _P, _P]),
             "orc_extract_items": (C.c_longlong, [_I, _I, _P, _P, _P, _P, _P, _I, _P, C.c_longlong, _P, _P,
@@ -137,6 +138,17 @@ def ref_nodesmooth64(tet, deg, ball, crd, nsweep):
         lib().orc_ref_nodesmooth_sweep_f64(len(c), len(tet), p(tet), p(deg), p(ball), ball.shape[1],
                                            p(c), p(opt), p(res))
     return c, opt, res
+
+
+def face_pull64(tet, deg, ball, crd, nsweep):
+    """nsweep sweeps of the volume-weighted opposite-face pull (FP64):
+    (coordinates, items of the last sweep, residuals of the last sweep)."""
+    c = np.ascontiguousarray(np.asarray(crd, np.float64)[:, :3])
+    item = np.empty((len(tet), 4, 4), np.float64)
+    res = np.zeros(len(c), np.float64)
+    for _ in range(nsweep):
+        lib().orc_face_pull_sweep_f64(len(c), len(tet), p(tet), p(deg), p(ball), ball.shape[1], p(c), p(item), p(res))
+    return c, item, res
 
 
 def bsr_spmv64(rowptr, col, val, x, b):

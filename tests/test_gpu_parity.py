@@ -141,59 +141,81 @@ def test_smoothing_sweeps_generated(n, sweeps):
     assert np.array_equal(rg, ref)
 
 
-def test_reference_nodesmooth_pattern_generated():
-    # the NodeSmoothing pattern of the reference (ideal apex per face,
-    # 4 x float4 per tet gathered through the ball with voyeurs, mix 0.2)
-    scatter = """{
-   float4 e[6], c, nrm; float l[6], m;
-   e[0] = VerCrd[0] - VerCrd[1]; e[1] = VerCrd[0] - VerCrd[2]; e[2] = VerCrd[0] - VerCrd[3];
-   e[3] = VerCrd[1] - VerCrd[2]; e[4] = VerCrd[1] - VerCrd[3]; e[5] = VerCrd[2] - VerCrd[3];
-   for(int i = 0; i < 6; i++) l[i] = dot(e[i], e[i]);
-   m = 0.816496 * sqrt(max(max(l[3], l[4]), l[5])); c = (VerCrd[1] + VerCrd[2] + VerCrd[3]) * (float4)(1./3.);
-   OptCrd[0] = c + fast_normalize(cross(e[4], e[5])) * (float4)(m);
-   m = 0.816496 * sqrt(max(max(l[1], l[2]), l[5])); c = (VerCrd[0] + VerCrd[2] + VerCrd[3]) * (float4)(1./3.);
-   OptCrd[1] = c + fast_normalize(cross(e[5], e[2])) * (float4)(m);
-   m = 0.816496 * sqrt(max(max(l[0], l[2]), l[4])); c = (VerCrd[1] + VerCrd[0] + VerCrd[3]) * (float4)(1./3.);
-   OptCrd[2] = c + fast_normalize(cross(e[2], e[4])) * (float4)(m);
-   m = 0.816496 * sqrt(max(max(l[0], l[1]), l[3])); c = (VerCrd[1] + VerCrd[2] + VerCrd[0]) * (float4)(1./3.);
-   OptCrd[3] = c + fast_normalize(cross(e[3], e[1])) * (float4)(m);
-}"""
-    gather = """{
-   float4 c = (float4)(0.);
-   for(int i = 0; i < OptCrdDegMax; i++)
+# Two bodies wired the way the reference's NodeSmoothing example wires its
+# scatter and gather kernels (NodeSmoothing.c:137-141): a tet body writing
+# four float4 items per tet, a vertex body reading them through the vertex
+# -> tet ball with voyeurs (each ball entry's local index selects its item),
+# looping to ...DegMax over the ...Nul-padded ball, testing ...Deg, and
+# relaxing with mix().  The scheme (volume-weighted pull towards the
+# centroids of the opposite faces) is the oracle's orc_face_pull_sweep_f64.
+FACE_PULL_SCATTER = """{
+   float4 p0 = VerCrd[0], p1 = VerCrd[1], p2 = VerCrd[2], p3 = VerCrd[3];
+   float4 sum = p0 + p1 + p2 + p3;
+   float  w = fabs(dot(cross(p1 - p0, p2 - p0), p3 - p0)) * (1.f / 6.f);
+   for(int j = 0; j < 4; j++)
    {
-      c += (VerTetVoy[i] == 0) ? OptCrd[i][0] : (float4)(0);
-      c += (VerTetVoy[i] == 1) ? OptCrd[i][1] : (float4)(0);
-      c += (VerTetVoy[i] == 2) ? OptCrd[i][2] : (float4)(0);
-      c += (VerTetVoy[i] == 3) ? OptCrd[i][3] : (float4)(0);
+      float4 f = (sum - VerCrd[j]) * (1.f / 3.f);
+      f.w = w;
+      Pull[j] = f;
    }
-   c *= (float4)(1./(OptCrdDeg));
-   c = mix(VerCrd, c, (float)0.2);
-   res = dot(VerCrd - c, VerCrd - c);
-   VerCrd = c;
 }"""
-    crd, ref, tet = G.cube_mesh(12, 0.2, 4)
-    sweeps = 3
+
+FACE_PULL_GATHER = """{
+   float4 acc = (float4)(0.f);
+   float  wsum = 0.f;
+   for(int k = 0; k < PullDegMax; k++)
+   {
+      float4 f = Pull[k][VerTetVoy[k]];      /* PullNul (zeros) past the degree */
+      acc += f * (float4)(f.w);
+      wsum += f.w;
+   }
+   move = 0.f;
+   if(PullDeg > 0)
+   {
+      float4 target = acc / (float4)(wsum);
+      target.w = 0.f;
+      float4 next = mix(VerCrd, target, 0.25f);
+      move = dot(next - VerCrd, next - VerCrd);
+      VerCrd = next;
+   }
+}"""
+
+
+@pytest.mark.parametrize("n,sweeps,isolated", [(12, 3, False), (7, 2, True)])
+def test_face_pull_uplink_pattern_generated(n, sweeps, isolated):
+    crd, ref, tet = G.cube_mesh(n, 0.2, 4)
+    if isolated:
+        # a vertex that no tet uses: its ball is empty (PullDeg == 0)
+        crd = np.vstack([crd, [[0.5, 0.5, 2.0, 0.0]]]).astype(np.float32)
+        ref = np.append(ref, 0).astype(np.int32)
     with G.Gml() as g:
         vi, ti = g.set_mesh(crd, ref, tet)
-        opt = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "OptCrd")
-        res = g.solution(G.GmlVertices, 1, G.GmlFlt, "res")
-        ks = g.compile(scatter, "scatter", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (opt, G.GmlWriteMode, 0)])
-        kg = g.compile(gather, "gather", G.GmlVertices,
-                       [(res, G.GmlWriteMode, 0), (opt, G.GmlReadMode | G.GmlVoyeurs, 0), (vi, G.GmlReadMode | G.GmlWriteMode, 0)])
-        assert "gml_upn<float4, 4, 32>" in g.L.GmlxGetKernelSource(g.h, kg).decode()
+        pull = g.solution(G.GmlTetrahedra, 4, G.GmlFlt4, "Pull")
+        move = g.solution(G.GmlVertices, 1, G.GmlFlt, "move")
+        ks = g.compile(FACE_PULL_SCATTER, "face_pull_scatter", G.GmlTetrahedra,
+                       [(vi, G.GmlReadMode, 0), (pull, G.GmlWriteMode, 0)])
+        kg = g.compile(FACE_PULL_GATHER, "face_pull_gather", G.GmlVertices,
+                       [(move, G.GmlWriteMode, 0), (pull, G.GmlReadMode | G.GmlVoyeurs, 0),
+                        (vi, G.GmlReadMode | G.GmlWriteMode, 0)])
+        src = g.L.GmlxGetKernelSource(g.h, kg).decode()
+        assert "gml_upn<float4, 4, 32>" in src and "VerTetVoy" in src
         for _ in range(sweeps):
             g.launch(ks)
             g.launch(kg)
         cg, _ = g.vertices(len(crd))
-        rsum = g.reduce(res, G.GmlSum)
+        mv = g.download_raw(move, np.float32, (len(crd),))
+        msum = g.reduce(move, G.GmlSum)
     d0, b0, _ = O.vertex_ball(tet, len(crd), 32)
-    c64, _, r64 = O.ref_nodesmooth64(tet, d0, b0, crd, sweeps)
+    c64, _, r64 = O.face_pull64(tet, d0, b0, crd, sweeps)
     e = np.abs(cg[:, :3] - c64).max()
-    print("reference NodeSmoothing pattern: max |gpu - fp64 oracle| = %.3e, residual gpu %.6e fp64 %.6e"
-          % (e, rsum, r64.sum()))
+    er = np.abs(mv - r64).max() / r64.max()
+    print("face pull n=%d: max |gpu - fp64 oracle| = %.3e, residual max rel %.3e, sum gpu %.6e fp64 %.6e"
+          % (n, e, er, msum, r64.sum()))
     assert e <= 1e-5
-    assert rsum == pytest.approx(r64.sum(), rel=1e-4)
+    assert er <= 1e-4
+    assert msum == pytest.approx(r64.sum(), rel=1e-4)
+    if isolated:
+        assert np.array_equal(cg[-1], crd[-1]) and mv[-1] == 0.0
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 5, 1025, 1 << 20, (1 << 22) + 3])

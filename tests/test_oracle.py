@@ -373,6 +373,46 @@ def test_reference_nodesmooth_gather_closed_form_bipyramid():
     assert res[3] == pytest.approx(0.04 * ((1 + k) * math.sqrt(2.0 / 3.0) * s) ** 2, rel=1e-10)
 
 
+def test_face_pull_closed_forms():
+    # one regular tet: each vertex moves a quarter of the way to the centroid
+    # of its opposite face, i.e. along the tet height sqrt(2/3) s
+    P = np.array(REG) + 1.0
+    s = np.linalg.norm(P[0] - P[1])
+    tet = np.array([[0, 1, 2, 3]], np.int32)
+    deg, ball, _ = O.vertex_ball(tet, 5, 32)                  # vertex 4 is isolated
+    crd = np.vstack([P, [9.0, 9.0, 9.0]])
+    new, item, res = O.face_pull64(tet, deg, ball, np.hstack([crd, np.zeros((5, 1))]), 1)
+    for j in range(4):
+        f = np.delete(P, j, axis=0).mean(axis=0)
+        assert np.allclose(item[0, j, :3], f, atol=1e-14)
+        assert np.allclose(new[j], P[j] + 0.25 * (f - P[j]), atol=1e-14)
+        assert res[j] == pytest.approx(0.0625 * (2.0 / 3.0) * s * s, rel=1e-12)
+    assert item[0, 0, 3] == pytest.approx(s ** 3 / (6 * math.sqrt(2)), rel=1e-12)   # regular tet volume
+    assert np.array_equal(new[4], crd[4]) and res[4] == 0.0
+
+    # two tets on face (a, b, c): apex d of a regular tet, apex e half as far
+    # on the other side (half the volume).  The shared vertex a takes the
+    # 2:1 volume-weighted mean of the centroids of faces (b, c, d) and (c, b, e)
+    a, b, c, d = (np.array(x) for x in REG)
+    ctr = (a + b + c) / 3.0
+    e = ctr - 0.5 * (d - ctr)
+    P = np.array([a, b, c, d, e])
+    tet = np.array([[0, 1, 2, 3], [0, 2, 1, 4]], np.int32)
+    deg, ball, _ = O.vertex_ball(tet, 5, 32)
+    new, item, res = O.face_pull64(tet, deg, ball, np.hstack([P, np.zeros((5, 1))]), 1)
+    assert item[1, 0, 3] == pytest.approx(0.5 * item[0, 0, 3], rel=1e-12)
+    f1, f2 = (b + c + d) / 3.0, (c + b + e) / 3.0
+    target = (2 * f1 + f2) / 3.0
+    assert np.allclose(new[0], a + 0.25 * (target - a), atol=1e-14)
+    # the apexes see one tet each: the centroid of (a, b, c)
+    assert np.allclose(new[3], d + 0.25 * (ctr - d), atol=1e-14)
+    assert np.allclose(new[4], e + 0.25 * (ctr - e), atol=1e-14)
+    # an inverted tet weighs by its absolute volume
+    inv = np.array([[0, 2, 1, 3]], np.int32)
+    _, item2, _ = O.face_pull64(inv, *O.vertex_ball(inv, 5, 32)[:2], np.hstack([P, np.zeros((5, 1))]), 1)
+    assert item2[0, 0, 3] == pytest.approx(item[0, 0, 3], rel=1e-12)
+
+
 def test_bsr_spmv_f32_exact_small_integers():
     # small-integer values and x: every FP32 product and sum is exact, so the
     # FP32 product must equal the exact integer product, with non-symmetric
