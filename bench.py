@@ -294,10 +294,23 @@ def run_smoothing(args):
         dt = time.perf_counter() - t0
         ek = [a - b for a, b in zip(ktimes(path), sg)]
         nl = args.steps * args.sweeps
+        forms = {}
         if path == "generated":
             per = {"scatter": (scat_bytes, ek[0] / nl), "gather": (gath_bytes, ek[1] / nl),
                    "quality": (qual_bytes, ek[2] / args.steps)}
-            own_step = model_step
+            # a kernel in the element-tile form reads its tile plan (index
+            # words per tet, tile offsets, tile vertex lists) and each tile's
+            # vertices once instead of its tet rows and the scattered vertex
+            # gathers: its own compulsory bytes replace the model's 16T + 16V
+            pi = (C.c_longlong * 6)()
+            have_plan = L.GmlxGetTilePlanInfo(h, G.GmlTetrahedra, pi) == 1
+            own = {}
+            for name, k, outb in (("scatter", ks, 16 * T), ("gather", kg, None), ("quality", kq, 4 * T)):
+                forms[name] = L.GmlxGetKernelForm(h, k)
+                if forms[name] == 3 and have_plan and outb is not None:
+                    own[name] = 4 * pi[4] * T + 4 * (pi[1] + 1) + 20 * pi[2] + outb
+            own_step = (args.sweeps * (own.get("scatter", scat_bytes) + own.get("gather", gath_bytes))
+                        + own.get("quality", qual_bytes) + min_bytes)
         else:
             info = (C.c_longlong * 9)()
             if not L.GmlxGetSweepPlanInfo(h, info):
@@ -314,6 +327,13 @@ def run_smoothing(args):
         kern = {k: {"avg_us": round(v[1] * 1e6, 2), "alg_GBs": round(v[0] / v[1] / 1e9, 1),
                     "frac": round(v[0] / v[1] / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_launch": v[0]}
                 for k, v in per.items()}
+        if path == "generated":
+            for k in kern:
+                kern[k]["form"] = {1: "one row", 2: "paired", 3: "element tiles"}.get(forms[k], str(forms[k]))
+                if k in own:
+                    t = per[k][1]
+                    kern[k].update({"own_bytes_per_launch": own[k], "own_GBs": round(own[k] / t / 1e9, 1),
+                                    "own_frac": round(own[k] / t / 1e9 / HBM_PEAK_GBS, 4)})
         ms = dt / args.steps * 1e3
         rec = {"ms_per_step": round(ms, 4), "own_bytes_per_step": own_step,
                "GBs": round(own_step / (dt / args.steps) / 1e9, 2),

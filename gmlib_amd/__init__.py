@@ -85,6 +85,9 @@ _SIGS = {
     "GmlxGetKernelSource": (_CP, [_S, _I]),
     "GmlxGetKernelPairSource": (_CP, [_S, _I]),
     "GmlxGetKernelRows": (_I, [_S, _I]),
+    "GmlxGetKernelForm": (_I, [_S, _I]),
+    "GmlxGetTilePlanInfo": (_I, [_S, _I, C.POINTER(C.c_longlong)]),
+    "GmlxGetKernelTileSource": (_CP, [_S, _I]),
     "GmlxSetKernelShape": (_I, [_S, _I, _I, _I, _I]),
     "GmlxGetKernelShape": (_I, [_S, _I, _IP, _IP]),
     "GmlxGetDevicePointer": (_P, [_S, _I, C.POINTER(C.c_size_t)]),

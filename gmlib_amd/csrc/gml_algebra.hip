@@ -838,6 +838,7 @@ int GmlNormVec(size_t GmlIdx, int VecIdx, int RedIdx, double *nrm)
                          (const double *)v->dev, out);
    if(hipGetLastError() != hipSuccess) return -6;
    g->dat[RedIdx].host_valid = false;
+   BumpVersion(&g->dat[RedIdx]);
    double s = 0.;
    int rc = ReduceDevice(g, out, v->NmbLin, GmlL1, &s, true);
    if(rc != 1) return rc;

@@ -399,6 +399,48 @@ static bool BodyWritesParams(const char *body)
    return false;
 }
 
+// ---------------------------------------------------------------------------
+// Element-tile form.  An element loop without uplinks that reads vertex data
+// through its own element table (VerCrd[k] of a tet kernel) and whose body
+// never names the table row can run over tiles of kDownTile elements: the
+// tile's distinct vertices (a DownPlan) are loaded once into LDS with
+// coalesced loads, and each row reads its slots there through one-byte
+// tile-local indices instead of reading its table row and gathering every
+// slot from memory.  Values are those of the direct form; only where the
+// bytes come from changes.  The form is one more candidate of the launch
+// tuner.  Returns the table argument and the staged data arguments, or -1.
+// ---------------------------------------------------------------------------
+static const char *kRowIds[] = {"barrier", "get_global_id", "get_local_id", "get_group_id", "get_local_size",
+                                "get_global_size", "get_num_groups", "__syncthreads", "gml_gid"};
+
+static int TileStagedArgs(const GmlCtx *g, const std::vector<Arg> &args, int MshTyp, const char *KrnSrc,
+                          std::vector<int> *staged)
+{
+   staged->clear();
+   if(MshTyp <= GmlVertices || MshTyp >= GmlMaxEleTyp) return -1;
+   int L = -1;
+   for(size_t i = 0; i < args.size(); i++)
+   {
+      const Arg &a = args[i];
+      if(a.kind == AkBall || a.kind == AkDeg || (a.FlgTab & GmlManual)) return -1;
+      if(a.kind == AkDown && a.DatIdx == g->TypIdx[MshTyp] && a.MshTyp == GmlVertices) L = (int)i;
+   }
+   if(L < 0 || BodyUses(KrnSrc, args[L].nam)) return -1;
+   const int w = args[L].NmbItm * args[L].ItmLen;
+   if(w < 1 || w > 16) return -1;
+   for(size_t i = 0; i < args.size(); i++)
+   {
+      const Arg &a = args[i];
+      if(a.LnkIdx != L) continue;
+      if(!(a.FlgTab & GmlReadMode) || (a.FlgTab & GmlWriteMode) || a.NmbItm != 1 || a.CntIdx >= 0) return -1;
+      staged->push_back((int)i);
+   }
+   if(staged->empty()) return -1;
+   for(const char *id : kRowIds)
+      if(BodyUses(KrnSrc, id)) return -1;
+   return L;
+}
+
 static bool PairEligible(const std::vector<Arg> &args, const char *KrnSrc, bool lazy, bool stage)
 {
    if(lazy || stage || BodyWritesParams(KrnSrc) || strstr(KrnSrc, "atomic")) return false;
@@ -411,15 +453,14 @@ static bool PairEligible(const std::vector<Arg> &args, const char *KrnSrc, bool 
       else if(a.CntIdx >= 0) words += args[a.LnkIdx].MaxDeg * a.NmbItm * ((OclTypSiz[a.ItmTyp] + 3) / 4);
    }
    if(words > 32) return false;
-   for(const char *id : {"barrier", "get_global_id", "get_local_id", "get_group_id", "get_local_size",
-                         "get_global_size", "get_num_groups", "__syncthreads", "gml_gid"})
+   for(const char *id : kRowIds)
       if(BodyUses(KrnSrc, id)) return false;
    return true;
 }
 
 static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam, int MshTyp,
                               const std::vector<Arg> &args, const char *ParSrc, bool lazy, bool stage,
-                              bool pair = false)
+                              bool pair = false, bool tile = false)
 {
    std::string s;
    struct Local { std::string type, name, dims; };
@@ -429,8 +470,10 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       return Fmt("   %s %s%s;\n", t.c_str(), n.c_str(), d.c_str());
    };
    s.reserve(16384);
-   std::vector<int> staged;
+   std::vector<int> staged, tstaged;
    const int sball = stage ? StagedArgs(args, &staged) : -1;
+   const int stab = tile ? TileStagedArgs(g, args, MshTyp, KrnSrc, &tstaged) : -1;
+   auto is_tstaged = [&](int i) { return std::find(tstaged.begin(), tstaged.end(), i) != tstaged.end(); };
    // the ball's own entries are read only when something needs them
    bool ball_ids = true;
    if(sball >= 0)
@@ -480,6 +523,9 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
    if(sball >= 0)
       s += "\n   const unsigned int *gml_tofs,\n   const int *gml_tlist,\n   const unsigned short *gml_lidx,"
            "\n   const int gml_stage_n,";
+   if(stab >= 0)
+      s += "\n   const unsigned int *gml_dofs,\n   const int *gml_dlist,\n   const unsigned int *gml_dlid,"
+           "\n   const int gml_dstage_n,\n   const int gml_drows,";
    s += "\n   GmlParSct *GmlPar,\n   const int2 count,\n   const int gml_remap)\n{\n";
 
    // --- locals (gmlib3.c:2670-2732)
@@ -528,7 +574,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
 
    // --- counters (gmlib3.c:2739-2748)
    const std::string idxnam = std::string(BalTypStr[MshTyp]) + "Idx";
-   if(!pair)
+   if(!pair && stab < 0)
    {
       s += "// KERNEL VARIABLES DEFINITION\n" + decl;
       s += "// KERNEL COUNTERS\n";
@@ -553,10 +599,16 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       }
       // U entities per thread and pass, all U loads in flight before the
       // LDS stores; a slot past the end is clamped to the last entity, whose
-      // value it loads and stores again (the same bytes)
+      // value it loads and stores again (the same bytes, from cache).  With
+      // 12 x 16 bytes a 256-row tile of a tet mesh's vertex balls (about
+      // 2,300 distinct tets) takes one pass instead of two: one index ->
+      // value round trip fewer (the drop-in smoothing gather at 256^3:
+      // 721 -> 666 us); 10 to 12 measured alike, 14 and 16 slower (register
+      // pressure), and loads predicated on the count instead of clamped
+      // slower.  Wider items keep the same register budget.
       int bytes = 0;
       for(int i : staged) bytes += OclTypSiz[args[i].ItmTyp];
-      const int U = bytes <= 16 ? 8 : (bytes <= 32 ? 4 : 2);
+      const int U = bytes <= 16 ? 12 : (bytes <= 32 ? 6 : 3);
       s += Fmt("   {\n"
                "      const unsigned int gml_t = (unsigned int)(cnt - (int)threadIdx.x) / blockDim.x;\n"
                "      const unsigned int gml_t0 = gml_tofs[gml_t], gml_nt = gml_tofs[gml_t + 1] - gml_t0;\n"
@@ -583,7 +635,7 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       s += "         }\n      }\n      __syncthreads();\n   }\n\n";
       std::swap(s, stage_s);
    }
-   if(!pair && sball < 0) s += "   if(cnt >= count.s0)\n      return;\n\n";
+   if(!pair && sball < 0 && stab < 0) s += "   if(cnt >= count.s0)\n      return;\n\n";
 
    // --- reads (gmlib3.c:2755-2885), emitted into their own string
    std::string body_s, staged_reads;
@@ -596,6 +648,19 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
       const Arg *L = a.LnkIdx >= 0 ? &args[a.LnkIdx] : nullptr;
       const Arg *C = a.CntIdx >= 0 ? &args[a.CntIdx] : nullptr;
       const char *n = a.nam.c_str();
+
+      // element-tile form: the table row is not read; each slot of a staged
+      // item comes from LDS through its byte of the row's tile-local indices
+      if((int)i == stab) continue;
+      if(stab >= 0 && is_tstaged((int)i))
+      {
+         const int siz = L->NmbItm * L->ItmLen;
+         for(int k = 0; k < siz; k++)
+            s += Fmt("   %s%s = gml_s_%s[(gml_lw[%d] >> %d) & 255u];\n", n, siz > 1 ? Fmt("[%d]", k).c_str() : "", n,
+                     k >> 2, 8 * (k & 3));
+         s += "\n";
+         continue;
+      }
 
       if(a.kind == AkBall)
       {
@@ -732,6 +797,69 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
            "   if(cnt >= count.s0)\n      return;\n\n// STAGED READINGS\n" + staged_reads + "\n" + user + writes + "}\n";
       return s;
    }
+   if(stab >= 0)
+   {
+      // element-tile form: the work-group's tile of kDownTile rows, each
+      // thread kDownTile / blockDim.x of them; the rows' index words first,
+      // then the tile's distinct vertices into LDS, then the rows
+      const int wq = (args[stab].NmbItm * args[stab].ItmLen + 3) / 4;
+      s += Fmt("// ELEMENT TILE: gml_drows rows (at most %d), their distinct vertex data staged in LDS\n"
+               "   const int gml_b = gml_blk(gml_remap);\n"
+               "   const int gml_R = gml_drows / (int)blockDim.x;\n"
+               "   unsigned int gml_lws[8][%d];\n"
+               "#pragma unroll\n"
+               "   for(int gml_q = 0; gml_q < 8; gml_q++)\n"
+               "      if(gml_q < gml_R)\n      {\n"
+               "         const int gml_r = min(gml_b * gml_drows + gml_q * (int)blockDim.x + (int)threadIdx.x, count.s0 - 1);\n"
+               "#pragma unroll\n"
+               "         for(int gml_j = 0; gml_j < %d; gml_j++)\n"
+               "            gml_lws[gml_q][gml_j] = __builtin_nontemporal_load(&gml_dlid[(size_t)gml_r * %d + gml_j]);\n"
+               "      }\n"
+               "   extern __shared__ __align__(16) char gml_lds[];\n"
+               "   size_t gml_lo = 0;\n", kDownTile, wq, wq, wq);
+      for(int i : tstaged)
+      {
+         const char *T = OclTypStr[args[i].ItmTyp];
+         s += Fmt("   %s *gml_s_%s = (%s *)(gml_lds + gml_lo);\n", T, args[i].nam.c_str(), T);
+         s += Fmt("   gml_lo += ((size_t)gml_dstage_n * sizeof(%s) + 15) & ~(size_t)15;\n", T);
+      }
+      // two entities per thread and pass (a tile has at most 256), both
+      // loads in flight before the LDS stores; a slot past the end is
+      // clamped to the last entity
+      s += "   {\n"
+           "      const unsigned int gml_t0 = gml_dofs[gml_b], gml_nt = gml_dofs[gml_b + 1] - gml_t0;\n"
+           "      for(unsigned int gml_i = threadIdx.x; gml_i < gml_nt; gml_i += 2 * blockDim.x)\n      {\n"
+           "         int gml_x[2];\n"
+           "#pragma unroll\n"
+           "         for(int gml_u = 0; gml_u < 2; gml_u++)\n"
+           "            gml_x[gml_u] = gml_dlist[gml_t0 + min(gml_i + gml_u * blockDim.x, gml_nt - 1)];\n";
+      for(int i : tstaged)
+      {
+         const char *n = args[i].nam.c_str();
+         s += Fmt("         %s gml_v_%s[2];\n#pragma unroll\n"
+                  "         for(int gml_u = 0; gml_u < 2; gml_u++)\n"
+                  "            gml_v_%s[gml_u] = %sTab[gml_x[gml_u]];\n", OclTypStr[args[i].ItmTyp], n, n, n);
+      }
+      s += "#pragma unroll\n"
+           "         for(int gml_u = 0; gml_u < 2; gml_u++)\n"
+           "         {\n"
+           "            const unsigned int gml_j = min(gml_i + gml_u * blockDim.x, gml_nt - 1);\n";
+      for(int i : tstaged)
+         s += Fmt("            gml_s_%s[gml_j] = gml_v_%s[gml_u];\n", args[i].nam.c_str(), args[i].nam.c_str());
+      s += "         }\n      }\n      __syncthreads();\n   }\n\n";
+      s += "// ONE ROW: its locals, reads, the user's code and its writes\n"
+           "   auto gml_row = [&](int cnt, const unsigned int *gml_lw) {\n";
+      s += Fmt("   int       %s = cnt + count.s1;\n", idxnam.c_str());
+      s += decl + reads + user + writes + "   };\n";
+      s += "#pragma unroll\n"
+           "   for(int gml_q = 0; gml_q < 8; gml_q++)\n"
+           "      if(gml_q < gml_R)\n      {\n"
+           "         const int gml_c = gml_b * gml_drows + gml_q * (int)blockDim.x + (int)threadIdx.x;\n"
+           "         if(gml_c < count.s0)\n"
+           "            gml_row(gml_c, gml_lws[gml_q]);\n"
+           "      }\n}\n";
+      return s;
+   }
    if(!pair)
    {
       s += reads + user + writes + "}\n";
@@ -786,6 +914,7 @@ static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std:
       return 0;
    Kernel *k = &g->krn[idx];
    *k = Kernel();
+   SetShapes(k);
    if(g->host_only)
    {
       k->use = true;
@@ -813,14 +942,23 @@ static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std:
 // ---------------------------------------------------------------------------
 // Launch
 // ---------------------------------------------------------------------------
-// launch shapes the tuner tries: {group size, XCD remap, two rows per
-// thread}; the paired shapes only for kernels with a paired form.  The first
-// shape is launched again last, once code and caches are warm.
-static const int kShapes[][3] = {{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {256, 0, 0}, {128, 0, 0}, {64, 0, 0},
-                                 {256, 1, 1}, {128, 1, 1}, {256, 0, 1}, {128, 0, 1}};
-static const int kNmbOne = 6, kNmbAll = 10;
-static int NmbShapes(const Kernel *k) { return (k->fn2 || !k->src2.empty() ? kNmbAll : kNmbOne) + 1; }
-static const int *ShapeAt(const Kernel *k, int i) { return kShapes[i < NmbShapes(k) - 1 ? i : 0]; }
+// launch shapes the tuner tries: {group size, XCD remap, form}; the paired
+// shapes for kernels with a paired form, the element-tile shapes for
+// kernels with that form (dropped at the first launch if the table has no
+// tile plan).  The first shape is launched again last, once code and caches
+// are warm.
+void SetShapes(Kernel *k)
+{
+   k->shapes = {{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {256, 0, 0}, {128, 0, 0}, {64, 0, 0}};
+   if(k->fn2 || !k->src2.empty())
+      for(std::array<int, 3> sh : {std::array<int, 3>{256, 1, 1}, {128, 1, 1}, {256, 0, 1}, {128, 0, 1}})
+         k->shapes.push_back(sh);
+   if(k->fn3 || !k->src3.empty())
+      for(std::array<int, 3> sh : {std::array<int, 3>{128, 1, 2}, {256, 1, 2}, {64, 1, 2}, {128, 0, 2}, {256, 0, 2}})
+         k->shapes.push_back(sh);
+}
+static int NmbShapes(const Kernel *k) { return (int)k->shapes.size() + 1; }
+static const int *ShapeAt(const Kernel *k, int i) { return k->shapes[i < NmbShapes(k) - 1 ? i : 0].data(); }
 
 void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
 {
@@ -871,15 +1009,32 @@ int RunKernel(GmlCtx *g, Kernel *k)
    if(n <= 0) return 1;
    int count[2] = {(int)k->NmbLin[0], (int)k->NmbLin[1]};
 
-   int grp = k->grp, remap = k->remap, pair = k->pair, tuning = 0;
+   // element-tile form: the table's tile plan, if it has one; without it
+   // the form's shapes leave the tuner, and a kernel tuned to the form
+   // falls back to one row per thread
+   DownPlan *dp = nullptr;
+   if(k->fn3)
+   {
+      dp = GetDownPlan(g, k->dtyp);
+      if(dp && (dp->nrows != n || k->NmbLin[1] != 0)) dp = nullptr;
+      if(!dp)
+      {
+         if(!k->tuned && k->tune_step == 0)
+            k->shapes.erase(std::remove_if(k->shapes.begin(), k->shapes.end(),
+                                           [](const std::array<int, 3> &x) { return x[2] == 2; }), k->shapes.end());
+         if(k->tuned && k->form == 2) { k->form = 0; k->grp = 256; k->remap = 1; }
+      }
+   }
+   int grp = k->grp, remap = k->remap, form = k->form, tuning = 0;
    if(!k->tuned)
    {
       const int *sh = ShapeAt(k, k->tune_step);
       grp = sh[0];
       remap = sh[1];
-      pair = sh[2] && k->fn2;
+      form = sh[2] == 1 ? (k->fn2 ? 1 : 0) : (sh[2] == 2 && dp ? 2 : 0);
       tuning = 1;
    }
+   const int pair = form == 1;
    // staged uplink data: the tile plan of this work-group size; the tuner
    // keeps the largest size whose LDS fits 64 KB and varies the remap only
    StagePlan *sp = nullptr;
@@ -930,6 +1085,21 @@ int RunKernel(GmlCtx *g, Kernel *k)
       for(auto &p : stage_ptr) kp.push_back((void *)&p);
       kp.push_back((void *)&stage_n);
    }
+   void *tile_ptr[3] = {dp ? (void *)dp->tofs : nullptr, dp ? (void *)dp->tlist : nullptr, dp ? (void *)dp->lid : nullptr};
+   int tile_n = dp ? dp->max_nv : 0, tile_rows = dp ? dp->rows : 0;
+   if(form == 2)
+   {
+      // a work-group covers one tile: at most its rows of threads, at
+      // most 8 rows per thread
+      grp = std::min(grp, tile_rows);
+      if(tile_rows / grp > 8) grp = tile_rows / 8;
+      for(auto &p : tile_ptr) kp.push_back((void *)&p);
+      kp.push_back((void *)&tile_n);
+      kp.push_back((void *)&tile_rows);
+      size_t b = 0;
+      for(int z : k->dsiz) b += ((size_t)tile_n * z + 15) & ~(size_t)15;
+      lds = (unsigned int)b;
+   }
    kp.push_back((void *)&par);
    kp.push_back((void *)count);
    kp.push_back((void *)&remap_arg);
@@ -939,9 +1109,10 @@ int RunKernel(GmlCtx *g, Kernel *k)
    bool tun = tuning && NewRec(&tr);
    if(have) (void)hipEventRecord(rec.beg, g->stream);
    if(tun) (void)hipEventRecord(tr.beg, g->stream);
-   const long long span = pair ? 2LL * grp : grp;
+   const long long span = form == 2 ? tile_rows : (pair ? 2LL * grp : grp);
    unsigned int nb = (unsigned int)((n + span - 1) / span);
-   hipError_t e = hipModuleLaunchKernel(pair ? k->fn2 : k->fn, nb, 1, 1, grp, 1, 1, lds, g->stream, kp.data(), nullptr);
+   hipFunction_t fn = form == 2 ? k->fn3 : (pair ? k->fn2 : k->fn);
+   hipError_t e = hipModuleLaunchKernel(fn, nb, 1, 1, grp, 1, 1, lds, g->stream, kp.data(), nullptr);
    if(e != hipSuccess)
    {
       (void)hipGetLastError();
@@ -983,14 +1154,17 @@ int RunKernel(GmlCtx *g, Kernel *k)
          const int *sh = ShapeAt(k, bi);
          k->grp = k->stage_src >= 0 ? k->stage_grp : sh[0];
          k->remap = sh[1];
-         k->pair = sh[2] && k->fn2;
+         k->form = sh[2] == 1 ? (k->fn2 ? 1 : 0) : (sh[2] == 2 && k->fn3 ? 2 : 0);
          k->tuned = 1;
       }
    }
 
-   // data written by the kernel: host mirrors are stale
+   // data written by the kernel: host mirrors are stale, versions new
    for(int di : k->wargs)
+   {
       g->dat[di].host_valid = false;
+      BumpVersion(&g->dat[di]);
+   }
    return 1;
 }
 
@@ -1214,6 +1388,35 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
          DbgPrint(g, "Kernel %s: the paired form does not compile, the one-row form is kept\n%s\n", PrcNam,
                   plog.c_str());
    }
+   // the element-tile form of an element loop reading vertex data through
+   // its own table; the launch tuner compares it with the other forms
+   std::vector<int> tstaged;
+   if(HghSrc < 0 && k->stage_src < 0 && g->stage_uplinks && TileStagedArgs(g, args, MshTyp, KrnSrc, &tstaged) >= 0)
+   {
+      std::string tsrc = EmitKernel(g, KrnSrc, PrcNam, MshTyp, args, ParSrc, false, false, false, true), code, tlog;
+      if(RtcCompile(tsrc, PrcNam, g->cflags, &code, &tlog))
+      {
+         k->src3 = tsrc;
+         k->dtyp = MshTyp;
+         for(int i : tstaged) k->dsiz.push_back(OclTypSiz[args[i].ItmTyp]);
+         if(!g->host_only
+         && (hipModuleLoadData(&k->mod3, code.data()) != hipSuccess
+             || hipModuleGetFunction(&k->fn3, k->mod3, PrcNam) != hipSuccess))
+         {
+            (void)hipGetLastError();
+            if(k->mod3) (void)hipModuleUnload(k->mod3);
+            k->mod3 = nullptr;
+            k->fn3 = nullptr;
+            k->src3.clear();
+            k->dtyp = -1;
+            k->dsiz.clear();
+         }
+      }
+      else
+         DbgPrint(g, "Kernel %s: the element-tile form does not compile, the other forms are kept\n%s\n", PrcNam,
+                  tlog.c_str());
+   }
+   SetShapes(k);
    for(auto &a : args)
    {
       k->args.push_back(a.DatIdx);
@@ -1255,6 +1458,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
       return 0;
    }
    Kernel *h = &g->krn[HghIdx];
+   SetShapes(h);
    h->args = g->krn[KrnIdx].args;
    h->wargs = g->krn[KrnIdx].wargs;
    h->NmbLin[0] = g->NmbEle[MshTyp] - u->MaxPos;
@@ -1349,8 +1553,19 @@ int GmlxGetKernelRows(size_t GmlIdx, int idx)
    GML_CTX(g, GmlIdx);
    if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return -1;
    const Kernel *k = &g->krn[idx];
-   if(k->src2.empty()) return 1;
-   return k->tuned ? (k->pair ? 2 : 1) : 0;
+   if(k->src2.empty() && k->src3.empty()) return 1;
+   if(!k->tuned) return 0;
+   return k->form == 2 ? kDownTile / k->grp : (k->form == 1 ? 2 : 1);
+}
+
+int GmlxGetKernelForm(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return -1;
+   const Kernel *k = &g->krn[idx];
+   if(k->src2.empty() && k->src3.empty()) return 1;
+   if(!k->tuned) return 0;
+   return k->form + 1;
 }
 
 int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
@@ -1358,8 +1573,9 @@ int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
    GML_CTX(g, GmlIdx);
    if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use) return 0;
    Kernel *k = &g->krn[idx];
-   if(GrpSiz < 64 || GrpSiz > 256 || (GrpSiz & 63) || (Rows != 1 && Rows != 2)) return 0;
+   if(GrpSiz < 64 || GrpSiz > 256 || (GrpSiz & 63) || (Rows != 1 && Rows != 2 && Rows != -1)) return 0;
    if(Rows == 2 && !k->fn2 && (g->host_only ? k->src2.empty() : true)) return 0;
+   if(Rows == -1 && !k->fn3 && (g->host_only ? k->src3.empty() : true)) return 0;
    if(k->stage_src >= 0 && GrpSiz != 64 && GrpSiz != 128 && GrpSiz != 256) return 0;
    for(int i : {idx, k->HghIdx})
    {
@@ -1370,7 +1586,7 @@ int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
       t->grp = GrpSiz;
       if(t->stage_src >= 0) t->stage_grp = GrpSiz;
       t->remap = Remap ? 1 : 0;
-      t->pair = Rows == 2 && (t->fn2 || !t->src2.empty());
+      t->form = Rows == 2 && (t->fn2 || !t->src2.empty()) ? 1 : (Rows == -1 && (t->fn3 || !t->src3.empty()) ? 2 : 0);
       t->tuned = 1;
    }
    return 1;
@@ -1381,6 +1597,13 @@ const char *GmlxGetKernelPairSource(size_t GmlIdx, int idx)
    GML_CTX(g, GmlIdx);
    if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use || g->krn[idx].src2.empty()) return nullptr;
    return g->krn[idx].src2.c_str();
+}
+
+const char *GmlxGetKernelTileSource(size_t GmlIdx, int idx)
+{
+   GML_CTX(g, GmlIdx);
+   if(idx < 1 || idx > GmlMaxKrn || !g->krn[idx].use || g->krn[idx].src3.empty()) return nullptr;
+   return g->krn[idx].src3.c_str();
 }
 
 int GmlxGetKernelShape(size_t GmlIdx, int idx, int *GrpSiz, int *Remap)

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <array>
 
 #include "../../include/gmlib3.h"
 #include "../../include/gmlx.h"
@@ -53,7 +54,12 @@ struct Data {
    int      RedIdx = 0;
    int      OffIdx = 0;              // DkBal: slot of the chunk offset table
    bool     host_valid = false;      // mirror holds the device contents
+   unsigned long long ver = 0;       // changes whenever the device contents may have changed
 };
+
+// give a data item a new, process-wide unique version (its device contents
+// changed: upload, kernel write, allocation, caller invalidation)
+void BumpVersion(Data *d);
 
 // Vertex/entity -> element uplink in SELL-64 layout: rows are grouped in
 // chunks of 64 consecutive source entities; chunk c stores its entries
@@ -90,12 +96,22 @@ struct Kernel {
    long long     NmbLin[2] = {0, 0}; // {count, offset}
    int           HghIdx = 0;
    bool          has_par = true;
-   // launch shape autotuning (group size x XCD remap x rows per thread)
-   int           grp = 256, remap = 1, tuned = 0, tune_step = 0, pair = 0;
+   // launch shape autotuning (group size x XCD remap x form); forms: 0 one
+   // row per thread, 1 paired (two rows), 2 element tiles (LDS-staged
+   // element -> vertex data, kDownTile rows per work-group)
+   int           grp = 256, remap = 1, tuned = 0, tune_step = 0, form = 0;
+   std::vector<std::array<int, 3>> shapes;   // the tuner's candidates {group, remap, form}
    // paired form (two rows per thread), when the kernel has one
    hipModule_t   mod2 = nullptr;
    hipFunction_t fn2 = nullptr;
    std::string   src2;
+   // element-tile form, when the kernel has one: the element kind whose
+   // table is staged and the byte size of each staged item
+   hipModule_t   mod3 = nullptr;
+   hipFunction_t fn3 = nullptr;
+   std::string   src3;
+   int           dtyp = -1;
+   std::vector<int> dsiz;
    // LDS-staged uplink data (stage_src >= 0): the uplink, the byte size of
    // each staged item, the work-group size its plan was built for
    int           stage_src = -1, stage_dst = -1, stage_grp = 0;
@@ -132,6 +148,30 @@ struct SweepPlan {
 };
 
 struct ScratchBlock { void *p; size_t bytes; bool busy; };
+
+// Tile plan of an element -> vertex gather staged in LDS (gml_plan.cpp):
+// tiles of `rows` consecutive elements (the largest of 512, 256, 128, 64
+// whose tiles each reach at most 256 distinct vertices: 512 for a
+// Hilbert-ordered tet mesh); per tile the ascending distinct vertex ids of
+// its rows (every slot of the element table's rows, as the generated reads
+// take them); per row its slots' tile-local indices, one byte each, packed
+// four to a word.  No plan when even 64-row tiles exceed 256 vertices or an
+// id is outside the vertex table.  Valid while the element table keeps the
+// version it was built from.
+const int kDownTile = 512;                  // largest tile
+struct DownPlan {
+   int typ = 0, w = 0, wq = 0, rows = 0;    // element kind, slots per row, index words per row, tile rows
+   long long nrows = 0, ntile = 0;
+   int max_nv = 0;                          // largest tile list
+   unsigned long long ver = 0;              // version of the element table
+   const void *src = nullptr;               // its device address
+   unsigned int *tofs = nullptr;            // [ntile + 1]
+   int *tlist = nullptr;                    // vertex ids, tile by tile
+   unsigned int *lid = nullptr;             // [nrows * wq]
+   long long sum_nv = 0;
+   size_t bytes = 0;
+   bool none = false;                       // not stageable (a tile over 256 vertices, or bad ids)
+};
 
 // launch shape of a built-in kernel (work-group size, XCD block remap,
 // items per thread where a kernel takes several), per handle
@@ -181,6 +221,12 @@ struct GmlCtx {
    Shape        shape[kNmbBuiltin];
    int          smooth_mode = 2;
    int          stage_uplinks = 1;
+   // element -> vertex tile plans, per element kind (built on first use)
+   DownPlan    *dplan[GmlMaxEleTyp] = {nullptr};
+   unsigned long long dplan_seen[GmlMaxEleTyp] = {0};   // table version of the last unplanned request
+   // the hand-written quality kernel defaults to its tile-staged form
+   // (items per thread 0) in work-groups of 128
+   GmlCtx() { shape[GmlxBuiltinQuality] = Shape{128, 1, 0}; }
 };
 
 // --- helpers ---------------------------------------------------------------
@@ -292,8 +338,16 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows);
 void FreeStagePlan(GmlCtx *g, StagePlan *p);
+// the current element -> vertex tile plan of element kind typ, rebuilt when
+// the element table changed; nullptr when the kind cannot be staged
+DownPlan *GetDownPlan(GmlCtx *g, int typ);
+// the plan if it exists, or built when the same table version was asked for
+// before (a table used once never pays for a plan); nullptr otherwise
+DownPlan *RepeatDownPlan(GmlCtx *g, int typ);
+void FreeDownPlan(GmlCtx *g, DownPlan *p);
 
 // launcher (gml_codegen.cpp)
 int  RunKernel(GmlCtx *g, Kernel *k);
+void SetShapes(Kernel *k);   // the launch tuner's candidates of a compiled kernel
 
 } // namespace gml

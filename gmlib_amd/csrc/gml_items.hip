@@ -761,6 +761,7 @@ int GpuDownlink(GmlCtx *g, int Src, int Dst, const std::string &nam)
       return 0;
    }
    d->host_valid = false;
+   BumpVersion(d);
    return idx;
 }
 
@@ -846,6 +847,7 @@ int GpuNeighbours(GmlCtx *g, int typ, int ngb, int idx, int rw)
       return 0;
    }
    g->dat[idx].host_valid = false;
+   BumpVersion(&g->dat[idx]);
    return 1;
 }
 

@@ -74,6 +74,44 @@ __global__ void __launch_bounds__(1024) k_tet_quality(long long nt, const v4i *_
          __builtin_nontemporal_store(meanratio3(c[u][0], c[u][1], c[u][2], c[u][3]), &q[t0 + u * blockDim.x]);
 }
 
+// Tile-staged form: a work-group takes one tile of the element tile plan
+// (R * blockDim.x consecutive tets, R per thread), loads the
+// tile's distinct vertices into LDS (ascending ids: coalesced, two loads in
+// flight per thread), and each tet reads its corners there through its four
+// one-byte tile-local indices: 4 bytes per tet instead of its 16-byte row
+// and four scattered 16-byte gathers.
+template <int R>
+__global__ void __launch_bounds__(256) k_tet_quality_tile(long long nt, const unsigned int *__restrict__ tofs,
+                                                           const int *__restrict__ tlist, const unsigned int *__restrict__ lid,
+                                                           const v4f *__restrict__ crd, float *__restrict__ q, int remap)
+{
+   __shared__ v4f s[256];
+   const unsigned int t = xcd_block(remap);
+   const long long base = (long long)t * R * blockDim.x;
+   unsigned int li[R];
+#pragma unroll
+   for(int r = 0; r < R; r++)
+      li[r] = __builtin_nontemporal_load(&lid[min(base + r * (long long)blockDim.x + threadIdx.x, nt - 1)]);
+   const unsigned int t0 = tofs[t], n = tofs[t + 1] - t0;
+   for(unsigned int i = threadIdx.x; i < n; i += 2 * blockDim.x)
+   {
+      const unsigned int j1 = min(i + blockDim.x, n - 1);
+      const int a = tlist[t0 + i], b = tlist[t0 + j1];
+      const v4f va = crd[a], vb = crd[b];
+      s[i] = va;
+      s[j1] = vb;
+   }
+   __syncthreads();
+#pragma unroll
+   for(int r = 0; r < R; r++)
+   {
+      const long long i = base + r * (long long)blockDim.x + threadIdx.x;
+      const unsigned int w = li[r];
+      if(i < nt)
+         __builtin_nontemporal_store(meanratio3(s[w & 255u], s[(w >> 8) & 255u], s[(w >> 16) & 255u], s[w >> 24]), &q[i]);
+   }
+}
+
 __global__ void __launch_bounds__(1024) k_smooth_scatter(long long nt, const v4i *__restrict__ tet,
                                                           const v4f *__restrict__ crd, v4f *__restrict__ bary, int remap)
 {
@@ -330,8 +368,29 @@ int GmlxTetQuality(size_t GmlIdx, int QalIdx)
    const v4f *crd = (const v4f *)g->dat[g->TypIdx[GmlVertices]].dev;
    float *q = (float *)g->dat[QalIdx].dev;
    g->dat[QalIdx].host_valid = false;
+   BumpVersion(&g->dat[QalIdx]);
    if(!nt) return 1;
-   const int per = sh.per;
+   int per = sh.per;
+   if(per == 0)
+   {
+      // tile-staged form, when the tet table has a tile plan (built on the
+      // second call for the same table)
+      DownPlan *p = RepeatDownPlan(g, GmlTetrahedra);
+      if(p && p->nrows == nt)
+      {
+         // R = tile rows / group size, 1 to 8
+         int grp = std::min(std::min(sh.grp, 256), p->rows);
+         if(p->rows / grp > 8) grp = p->rows / 8;
+         const int R = p->rows / grp;
+         auto kern = R == 8 ? k_tet_quality_tile<8> : R == 4 ? k_tet_quality_tile<4> : R == 2 ? k_tet_quality_tile<2>
+                                                                                           : k_tet_quality_tile<1>;
+         return TimedLaunch(g, GmlxBuiltinQuality, [&]() {
+            hipLaunchKernelGGL(kern, dim3((unsigned int)p->ntile), dim3(grp), 0, g->stream, nt, p->tofs, p->tlist,
+                               p->lid, crd, q, sh.remap);
+         });
+      }
+      per = 2;
+   }
    auto kern = per == 1 ? k_tet_quality<1> : per == 4 ? k_tet_quality<4> : k_tet_quality<2>;
    const long long span = (long long)(per == 1 || per == 4 ? per : 2) * sh.grp;
    return TimedLaunch(g, GmlxBuiltinQuality, [&]() {
@@ -354,6 +413,7 @@ int GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx)
    long long nv = g->NmbEle[GmlVertices];
    const Shape sh = g->shape[GmlxBuiltinBallMin];
    o->host_valid = false;
+   BumpVersion(o);
    if(!nv) return 1;
    return TimedLaunch(g, GmlxBuiltinBallMin, [&]() {
       hipLaunchKernelGGL(k_ball_min, dim3((unsigned int)((nv + sh.grp - 1) / sh.grp)), dim3(sh.grp), 0, g->stream,
@@ -376,6 +436,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    const v4i *tet = (const v4i *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
    const int *ref = (const int *)g->dat[g->RefIdx[GmlVertices]].dev;
    vd->host_valid = false;
+   BumpVersion(vd);
    if(!nv || !nsweep) return 1;
 
    if(g->smooth_mode == 2)
@@ -435,6 +496,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    v4f *crd = (v4f *)vd->dev;
    const Shape ss = g->shape[GmlxBuiltinScatter], sg = g->shape[GmlxBuiltinGather];
    b->host_valid = false;
+   BumpVersion(b);
    for(int i = 0; i < nsweep; i++)
    {
       int rc = TimedLaunch(g, GmlxBuiltinScatter, [&]() {
@@ -474,7 +536,7 @@ int GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info)
 int GmlxSetBuiltinItems(size_t GmlIdx, int which, int PerThread)
 {
    GML_CTX(g, GmlIdx);
-   if(which != GmlxBuiltinQuality || (PerThread != 1 && PerThread != 2 && PerThread != 4)) return 0;
+   if(which != GmlxBuiltinQuality || (PerThread != 0 && PerThread != 1 && PerThread != 2 && PerThread != 4)) return 0;
    g->shape[which].per = PerThread;
    return 1;
 }

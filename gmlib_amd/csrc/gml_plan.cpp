@@ -313,3 +313,156 @@ void FreeSweepPlan(GmlCtx *g, SweepPlan *p)
 }
 
 } // namespace gml
+
+namespace gml {
+
+// ---------------------------------------------------------------------------
+// Element -> vertex tile plan (DownPlan): the element table of kind typ in
+// tiles of kDownTile rows; per tile its distinct vertex ids, ascending, and
+// per row the tile-local index of each slot, one byte each.  Built on the
+// host from the table's device contents with up to 16 threads.  A table
+// whose tiles reach more than 256 distinct vertices (a poorly ordered mesh:
+// the tile would share little anyway) or that holds an id outside the
+// vertex table gets no plan.
+// ---------------------------------------------------------------------------
+// per tile of TR rows: the distinct ids and the rows' byte indices; false
+// if a tile exceeds 256 ids or holds an id outside [0, nv)
+static bool TilePlan(const std::vector<int> &ele, long long n, int w, int wq, long long nv, int TR,
+                     std::vector<std::vector<int>> *tl, std::vector<unsigned int> *lid)
+{
+   const long long ntile = (n + TR - 1) / TR;
+   tl->assign((size_t)ntile, std::vector<int>());
+   lid->assign((size_t)n * wq, 0u);
+   std::atomic<bool> ok(true);
+   const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   std::vector<std::thread> th;
+   for(unsigned t = 0; t < nth; t++)
+      th.emplace_back([&, t]() {
+         std::vector<int> s;
+         for(long long k = t; k < ntile && ok; k += nth)
+         {
+            const long long a = k * TR, b = std::min(n, a + TR);
+            s.assign(ele.begin() + a * w, ele.begin() + b * w);
+            for(int v : s) if(v < 0 || v >= nv) { ok = false; break; }
+            std::sort(s.begin(), s.end());
+            s.erase(std::unique(s.begin(), s.end()), s.end());
+            if(s.size() > 256) { ok = false; break; }
+            for(long long r = a; r < b; r++)
+               for(int j = 0; j < w; j++)
+               {
+                  const unsigned int l = (unsigned int)(std::lower_bound(s.begin(), s.end(), ele[r * w + j]) - s.begin());
+                  (*lid)[(size_t)r * wq + (j >> 2)] |= l << (8 * (j & 3));
+               }
+            (*tl)[k] = s;
+         }
+      });
+   for(auto &x : th) x.join();
+   return ok;
+}
+
+static DownPlan *BuildDownPlan(GmlCtx *g, int typ)
+{
+   const int idx = g->TypIdx[typ];
+   Data *d = &g->dat[idx];
+   DownPlan *p = new DownPlan();
+   p->typ = typ;
+   p->w = d->NmbItm * d->ItmLen;
+   p->wq = (p->w + 3) / 4;
+   p->nrows = d->NmbLin;
+   p->ver = d->ver;
+   p->src = d->dev;
+   const long long nv = g->NmbEle[GmlVertices], n = p->nrows, w = p->w;
+   if(w < 1 || w > 16 || n < 1 || !nv) { p->none = true; return p; }
+   std::vector<int> ele((size_t)n * w);
+   if(!DevCopy(g, ele.data(), d->dev, ele.size() * 4, 2)) { p->none = true; return p; }
+   for(int v : ele)
+      if(v < 0 || v >= nv) { p->none = true; return p; }
+   std::vector<std::vector<int>> tl;
+   std::vector<unsigned int> lid;
+   int TR = kDownTile;
+   while(TR >= 64 && !TilePlan(ele, n, (int)w, p->wq, nv, TR, &tl, &lid)) TR /= 2;
+   if(TR < 64) { p->none = true; return p; }
+   p->rows = TR;
+   const long long ntile = (long long)tl.size();
+   std::vector<unsigned int> tofs((size_t)ntile + 1, 0);
+   for(long long k = 0; k < ntile; k++)
+   {
+      tofs[k + 1] = tofs[k] + (unsigned int)tl[k].size();
+      p->max_nv = std::max(p->max_nv, (int)tl[k].size());
+   }
+   std::vector<int> tlist((size_t)tofs[ntile] + 1, 0);
+   for(long long k = 0; k < ntile; k++) std::copy(tl[k].begin(), tl[k].end(), tlist.begin() + tofs[k]);
+   p->ntile = ntile;
+   p->sum_nv = tofs[ntile];
+   p->tofs = (unsigned int *)DevAlloc(g, tofs.size() * 4);
+   p->tlist = (int *)DevAlloc(g, tlist.size() * 4);
+   p->lid = (unsigned int *)DevAlloc(g, lid.size() * 4);
+   if(!p->tofs || !p->tlist || !p->lid
+   || !DevCopy(g, p->tofs, tofs.data(), tofs.size() * 4, 1)
+   || !DevCopy(g, p->tlist, tlist.data(), tlist.size() * 4, 1)
+   || !DevCopy(g, p->lid, lid.data(), lid.size() * 4, 1))
+   {
+      DevFree(g, p->tofs); DevFree(g, p->tlist); DevFree(g, p->lid);
+      p->tofs = nullptr; p->tlist = nullptr; p->lid = nullptr;
+      p->none = true;
+      return p;
+   }
+   p->bytes = tofs.size() * 4 + tlist.size() * 4 + lid.size() * 4;
+   g->MemSiz += p->bytes;
+   DbgPrint(g, "Element tile plan of %s: %lld tiles of %d rows, %lld tile vertices (%.2f per vertex), max %d per tile\n",
+            BalTypStr[typ], ntile, TR, p->sum_nv, (double)p->sum_nv / (double)nv, p->max_nv);
+   return p;
+}
+
+DownPlan *GetDownPlan(GmlCtx *g, int typ)
+{
+   if(g->host_only || typ <= GmlVertices || typ >= GmlMaxEleTyp || !g->TypIdx[typ] || !g->TypIdx[GmlVertices])
+      return nullptr;
+   Data *d = &g->dat[g->TypIdx[typ]];
+   DownPlan *&p = g->dplan[typ];
+   if(p && (p->ver != d->ver || p->src != d->dev || p->nrows != d->NmbLin)) { FreeDownPlan(g, p); p = nullptr; }
+   if(!p) p = BuildDownPlan(g, typ);
+   return p->none ? nullptr : p;
+}
+
+DownPlan *RepeatDownPlan(GmlCtx *g, int typ)
+{
+   if(g->host_only || typ <= GmlVertices || typ >= GmlMaxEleTyp || !g->TypIdx[typ] || !g->TypIdx[GmlVertices])
+      return nullptr;
+   const Data *d = &g->dat[g->TypIdx[typ]];
+   const DownPlan *p = g->dplan[typ];
+   if(p && p->ver == d->ver && p->src == d->dev && p->nrows == d->NmbLin) return p->none ? nullptr : g->dplan[typ];
+   if(g->dplan_seen[typ] != d->ver)
+   {
+      g->dplan_seen[typ] = d->ver;
+      return nullptr;
+   }
+   return GetDownPlan(g, typ);
+}
+
+} // namespace gml
+
+using namespace gml;
+
+extern "C" int GmlxGetTilePlanInfo(size_t GmlIdx, int typ, long long *info)
+{
+   GML_CTX(g, GmlIdx);
+   if(!info || typ <= GmlVertices || typ >= GmlMaxEleTyp) return 0;
+   DownPlan *p = GetDownPlan(g, typ);
+   if(!p) return 0;
+   info[0] = p->rows; info[1] = p->ntile; info[2] = p->sum_nv; info[3] = p->max_nv;
+   info[4] = p->wq; info[5] = (long long)p->bytes;
+   return 1;
+}
+
+namespace gml {
+
+void FreeDownPlan(GmlCtx *g, DownPlan *p)
+{
+   if(!p) return;
+   DevFree(g, p->tofs); DevFree(g, p->tlist); DevFree(g, p->lid);
+   g->MemSiz -= p->bytes;
+   delete p;
+}
+
+} // namespace gml

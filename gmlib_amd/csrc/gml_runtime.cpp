@@ -11,6 +11,8 @@
 
 #include "gml_internal.h"
 
+#include <atomic>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -127,6 +129,7 @@ bool AllocData(GmlCtx *g, Data *d, bool zero)
    if(zero && !g->host_only)
       HIPCHK(hipMemsetAsync(d->dev, 0, siz, g->stream));
    g->MemSiz += d->MemSiz;
+   BumpVersion(d);
    return true;
 }
 
@@ -156,6 +159,10 @@ void FreeDataSlot(GmlCtx *g, int idx)
    *d = Data();
 }
 
+static std::atomic<unsigned long long> g_version{0};
+
+void BumpVersion(Data *d) { d->ver = ++g_version; }
+
 size_t Upload(GmlCtx *g, int idx)
 {
    if(idx < 1 || idx > GmlMaxDat) return 0;
@@ -167,6 +174,7 @@ size_t Upload(GmlCtx *g, int idx)
       return 0;
    }
    d->host_valid = true;
+   BumpVersion(d);
    g->MovSiz += d->MemSiz;
    return d->MemSiz;
 }
@@ -307,6 +315,7 @@ void GmlStop(size_t GmlIdx)
       for(auto &e : k->tune_ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
       if(k->mod) (void)hipModuleUnload(k->mod);
       if(k->mod2) (void)hipModuleUnload(k->mod2);
+      if(k->mod3) (void)hipModuleUnload(k->mod3);
    }
    for(int r = 0; r < GmlMaxRed; r++)
       for(auto &e : g->red_evts[r]) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
@@ -319,6 +328,7 @@ void GmlStop(size_t GmlIdx)
    for(int a = 0; a < GmlMaxEleTyp; a++)
       for(int b = 0; b < GmlMaxEleTyp; b++)
          for(auto &sp : g->up[a][b].stage) { FreeStagePlan(g, sp); sp = nullptr; }
+   for(auto &dp : g->dplan) { FreeDownPlan(g, dp); dp = nullptr; }
    FreeAlgebra(g);
    for(int i = 1; i <= GmlMaxDat; i++)
       FreeDataSlot(g, i);
@@ -823,6 +833,7 @@ int GmlxInvalidateHost(size_t GmlIdx, int DatIdx)
    GML_CTX(g, GmlIdx);
    if(DatIdx < 1 || DatIdx > GmlMaxDat || !g->dat[DatIdx].use) return 0;
    g->dat[DatIdx].host_valid = false;
+   BumpVersion(&g->dat[DatIdx]);
    return 1;
 }
 

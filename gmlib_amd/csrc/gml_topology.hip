@@ -705,6 +705,7 @@ int GmlSetNeighbours(size_t GmlIdx, int typ)
       }
    }
    d->host_valid = false;
+   BumpVersion(d);
    return idx;
 }
 

@@ -58,19 +58,37 @@ int    GmlxSetStaging(size_t GmlIdx, int on);
    built for kernels without uplinks) has its own source, NULL if none. */
 const char *GmlxGetKernelSource(size_t GmlIdx, int KrnIdx);
 const char *GmlxGetKernelPairSource(size_t GmlIdx, int KrnIdx);
+/* the element-tile form's source (see GmlxGetKernelForm), NULL if none */
+const char *GmlxGetKernelTileSource(size_t GmlIdx, int KrnIdx);
 
 /* Group size and XCD remap flag the launch-shape tuner settled on for a
    kernel (0 in both while it is still tuning). */
 int    GmlxGetKernelShape(size_t GmlIdx, int KrnIdx, int *GrpSiz, int *Remap);
 
-/* Rows per thread of a kernel's launches: 1 or 2 (its paired form), 0 while
-   the tuner is still choosing; -1 for an invalid index. */
+/* Rows per thread of a kernel's launches: 1, 2 (its paired form) or, in the
+   element-tile form, the tile's rows per work-item (512 / group size); 0
+   while the tuner is still choosing; -1 for an invalid index. */
 int    GmlxGetKernelRows(size_t GmlIdx, int KrnIdx);
+
+/* Form of a kernel's launches: 1 one row per work-item, 2 the paired form,
+   3 the element-tile form (an element loop reading vertex data through its
+   own table: per work-group a tile of 512 elements whose distinct vertices
+   are loaded once into LDS; built when the body never names the table row,
+   used when every tile of the table reaches at most 256 vertices); 0 while
+   the tuner is still choosing; -1 for an invalid index. */
+int    GmlxGetKernelForm(size_t GmlIdx, int KrnIdx);
+
+/* Element tile plan of element kind typ (built if needed; see
+   GmlxGetKernelForm), 6 values: {tile rows, tiles, sum of the tiles' vertex
+   counts, largest tile, index words per element, plan bytes}.  Returns 1,
+   or 0 if the kind has no plan. */
+int    GmlxGetTilePlanInfo(size_t GmlIdx, int typ, long long *info);
 
 /* Pin the launch shape of a kernel (disables its tuner): group size 64, 128
    or 256 (generated kernels are compiled for at most 256 work-items), XCD
-   remap, rows per thread (2 needs the paired form).  Returns 1, or 0 if the
-   shape is not available for the kernel. */
+   remap, rows per thread (2 needs the paired form; -1 selects the
+   element-tile form).  Returns 1, or 0 if the shape is not available for
+   the kernel. */
 int    GmlxSetKernelShape(size_t GmlIdx, int KrnIdx, int GrpSiz, int Remap, int Rows);
 
 /* --- device residency ---------------------------------------------------- */
@@ -173,7 +191,11 @@ int    GmlxGetSweepPlanInfo(size_t GmlIdx, long long *info);
    256 (SpMV, CG kernels); Remap 1 gives each XCD a contiguous block range */
 int    GmlxSetBuiltinShape(size_t GmlIdx, int which, int GrpSiz, int Remap);
 /* items per thread of a built-in kernel that takes several: the quality
-   kernel (GmlxBuiltinQuality) takes 1, 2 (default) or 4 tets per thread */
+   kernel (GmlxBuiltinQuality) takes 1, 2 or 4 tets per thread, or 0 (the
+   default, work-groups of 128): tiles of 512 tets whose distinct vertices
+   are loaded once into LDS (the tet table's tile plan, built by the second
+   call on an unchanged table; until then, and without a plan, 2 tets per
+   thread) */
 int    GmlxSetBuiltinItems(size_t GmlIdx, int which, int PerThread);
 /* summed device time (s) and launch count of a built-in kernel on a handle
    (GmlxBuiltinSpMV: the block SpMV of GmlMultMatVec; GmlxBuiltinCgSpMV and

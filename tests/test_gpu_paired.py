@@ -89,12 +89,16 @@ def test_paired_heat_kernels_and_tuner():
         finally:
             s.close()
     assert np.array_equal(outs[0], outs[1])
-    # the tuner tries both forms and keeps one
+    # the tuner tries every form (one row, paired, element tiles: 15 shapes
+    # and the first again) and keeps one
     crd, ref, tet = G.cube_mesh(10, 0.2, 1)
     with G.Gml() as g:
         vi, ti = g.set_mesh(crd, ref, tet)
         q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
         k = g.compile(body("tet_meanratio"), "q", G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
-        for _ in range(12):
+        for _ in range(15):
             g.launch(k)
-        assert g.L.GmlxGetKernelRows(g.h, k) in (1, 2)
+        assert g.L.GmlxGetKernelForm(g.h, k) == 0
+        g.launch(k)
+        assert g.L.GmlxGetKernelForm(g.h, k) in (1, 2, 3)
+        assert g.L.GmlxGetKernelRows(g.h, k) in (1, 2, 4, 8)
