@@ -120,7 +120,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="all",
                     choices=["all", "smoothing", "small", "cg", "batch", "serve", "extract", "heat", "bandwidth"])
-    ap.add_argument("--configs", default="1,3,4,5",
+    ap.add_argument("--configs", default="1,3,3b,4,5",
                     help="all workload: the other configurations measured after the headline one")
     ap.add_argument("--qps", type=float, default=40.0, help="serve workload: arrival rate (jobs/s)")
     ap.add_argument("--jobs", type=int, default=400, help="serve workload: jobs in the timed stream")
@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per cube side")
+    ap.add_argument("--cg-n", type=int, default=128, help="cg workload: cells per side of the cube whose edge graph "
+                    "gives the matrix")
+    ap.add_argument("--cg-solve", type=int, default=1, help="cg workload: also run the two CG solves (0: SpMV only)")
     ap.add_argument("--sweeps", type=int, default=10)
     ap.add_argument("--omega", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -164,11 +167,14 @@ def main():
     out = run_smoothing(args)
     sub = {}
     names = {"1": ("config1_quality_20cube", run_small), "3": ("config3_spmv_cg_128cube", run_cg),
+             "3b": ("spmv_256cube", run_cg),
              "4": ("config4_batch_512meshes", run_batch), "5": ("config5_served_stream", run_serve)}
     for c in [x for x in args.configs.split(",") if x]:
         name, fn = names[c]
         a = argparse.Namespace(**vars(args))
         a.cpu_seconds = min(args.cpu_seconds, 3.0)
+        if c == "3b":                   # the SpMV of the 256^3 cube's edge graph (100.7M tets), no solves
+            a.cg_n, a.cg_solve = 256, 0
         t0 = time.time()
         log("---- %s ----" % name)
         try:
@@ -536,15 +542,19 @@ def run_small(args):
 def run_cg(args):
     import numpy as np
     import gmlib_amd as G
-    import gmlib_amd.workloads as W
     from tests import oracle_lib as O
 
-    n = args.n if args.n != 256 else 128
+    n = args.cg_n
     t0 = time.time()
     crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
-    e = W.cube_edges(tet)
-    rp, col, val = W.edge_laplacian3(len(crd), e, 2024)
-    del crd, ref, tet, e
+    L0 = G.lib()
+    rp = np.empty(len(crd) + 1, np.int32)
+    nb = L0.GmlxEdgeLaplacian3(len(crd), len(tet), G.ptr(tet), 2024, 0.01, G.ptr(rp), None, None, 0)
+    col = np.empty(nb, np.int32)
+    val = np.empty(nb * 9, np.float32)
+    if L0.GmlxEdgeLaplacian3(len(crd), len(tet), G.ptr(tet), 2024, 0.01, G.ptr(rp), G.ptr(col), G.ptr(val), nb) != nb:
+        raise RuntimeError("GmlxEdgeLaplacian3 failed")
+    del crd, ref, tet
     nv, nb = len(rp) - 1, len(col)
     rhs = np.random.default_rng(7).normal(size=nv * 3).astype(np.float32)
     log("edge Laplacian %d^3: %d block rows, %d 3x3 blocks (%.1f per row), built in %.1fs"
@@ -617,7 +627,7 @@ def run_cg(args):
     cg_upd_bytes = 144 * nv
     bn = float(np.linalg.norm(rhs.astype(np.float64)))
     solves = {}
-    for name, tol, refine in (("fp32", 1e-5, 0), ("fp32_fp64_refined", 1e-10, 1)):
+    for name, tol, refine in (("fp32", 1e-5, 0), ("fp32_fp64_refined", 1e-10, 1)) if args.cg_solve else ():
         rr, it = C.c_double(), C.c_int()
         ca, cb = C.c_longlong(), C.c_longlong()
         s0 = L.GmlxGetBuiltinRunTime(h, G.GmlxBuiltinCgSpMV, C.byref(ca)); n0 = ca.value
@@ -647,7 +657,7 @@ def run_cg(args):
 
     # plain streaming rate of this GPU: GmlScaleVec (read + write) over a 2 GiB vector
     # (skipped in counter-calibration runs, whose GmlScaleVec launches must all be the small one)
-    big = L.GmlNewVector(h, 1 << 27, 4, None, G.GmlFlt) if not args.copy_calib else 0
+    big = L.GmlNewVector(h, 1 << 27, 4, None, G.GmlFlt) if not args.copy_calib and args.cg_solve else 0
     probe = None
     if big > 0:
         one = C.c_double(1.0)
@@ -679,8 +689,8 @@ def run_cg(args):
         log("cpu baseline: %s" % cpu)
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_cg128.json")
-    if n == 128 and os.path.exists(tf):
+    tf = os.path.join(ROOT, "profiles", "traffic_cg%d.json" % n)
+    if os.path.exists(tf):
         traffic = json.load(open(tf)).get("spmv")
     out = {
         "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",

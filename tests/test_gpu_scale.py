@@ -5,7 +5,8 @@
   sweep and the operator-form sweep, then tet quality and its minimum;
 - the 3x3 edge Laplacian of the 128^3 cube (2,146,689 block rows, 31.8M
   blocks): SpMV, FP32 CG and FP64-refined CG, residuals recomputed by the
-  oracle's FP64 product;
+  oracle's FP64 product; the SpMV of the 256^3 cube's edge Laplacian
+  (16,974,593 block rows, 253M blocks);
 - 512 cube meshes with cells per side log-uniform in [8, 96] (342M tets)
   in one batch: per-tet quality, per-vertex ball minimum and per-mesh
   minimum / mean;
@@ -234,3 +235,33 @@ def test_served_mix_10k_to_20m_tets_vs_oracle():
         emean = abs(r[1] - math.fsum(q) / len(q)) / abs(math.fsum(q) / len(q))
         log("mesh %d (%d tets), job %d: min rel err %.2e, mean rel err %.2e" % (i, len(tet), t, emin, emean))
         assert emin <= tol_min and emean <= 1e-5
+
+
+def test_spmv_256cube_vs_oracle():
+    # the edge Laplacian of the 256^3 cube (the 100.7M-tet mesh): 16,974,593
+    # block rows, 253,036,801 3x3 blocks; y = A x against the oracle's FP64
+    # product over every row
+    n = 256
+    crd, ref, tet = G.cube_mesh(n, 0.2, 2024, True)
+    V, T = len(crd), len(tet)
+    L = G.lib()
+    rowptr = np.empty(V + 1, np.int32)
+    nb = L.GmlxEdgeLaplacian3(V, T, G.ptr(tet), 2024, 0.01, G.ptr(rowptr), None, None, 0)
+    assert nb == V + 2 * 118031104
+    col = np.empty(nb, np.int32)
+    val = np.empty(nb * 9, np.float32)
+    assert L.GmlxEdgeLaplacian3(V, T, G.ptr(tet), 2024, 0.01, G.ptr(rowptr), G.ptr(col), G.ptr(val), nb) == nb
+    del crd, ref, tet
+    x = np.random.default_rng(11).normal(size=V * 3).astype(np.float32)
+    with G.Gml() as g:
+        m = L.GmlNewMatrix(g.h, V, nb, 3, G.ptr(val), G.ptr(col), G.ptr(rowptr), G.GmlFlt)
+        vx = L.GmlNewVector(g.h, V, 3, G.ptr(x), G.GmlFlt)
+        vy = L.GmlNewVector(g.h, V, 3, None, G.GmlFlt)
+        assert min(m, vx, vy) > 0
+        assert L.GmlMultMatVec(g.h, m, vy, vx) == 1
+        y = np.empty(V * 3, np.float32)
+        assert L.GmlxGetVector(g.h, vy, G.ptr(y))
+    y64 = O.bsr_spmv64(rowptr, col, val, x.astype(np.float64), 3)
+    err = float(np.abs(y - y64).max() / np.abs(y64).max())
+    log("SpMV 256^3: max |y - y64| / max |y64| = %.2e" % err)
+    assert err <= 1e-5
