@@ -71,6 +71,7 @@ struct BinFile {
    FILE *f = nullptr;
    bool swap = false;
    int ver = 0, dim = 0;
+   long long size = 0;         // file size in bytes
    ~BinFile() { if(f) fclose(f); }
 
    bool Read(void *p, size_t siz, size_t cnt)
@@ -101,11 +102,22 @@ struct BinFile {
 // keyword positions of a binary file: code -> (data position, line count)
 struct BinKwd { int kwd; long long pos, n; };
 
+// smallest size in bytes of one line of mesh kind typ in a binary file
+long long MinLine(const BinFile &b, int typ)
+{
+   const long long isz = b.ver >= 4 ? 8 : 4, rsz = b.ver >= 2 ? 8 : 4;
+   return typ == GmlVertices ? b.dim * rsz + isz : (EleNmbNod[typ] + 1) * isz;
+}
+
+// The file is not trusted: a keyword's next position must lie past what was
+// read of it and inside the file (so the scan always advances), its line
+// count must be >= 0 and its lines must fit in the rest of the file.
 bool ScanBinary(BinFile &b, std::vector<BinKwd> *kw)
 {
    for(;;)
    {
       long long kwd = 0, nxt = 0, n = 0;
+      const long long start = (long long)ftello(b.f);
       if(!b.Int(&kwd, false)) return false;
       if(kwd == GmfEnd) return true;
       if(!b.Int(&nxt, b.ver >= 3)) return false;
@@ -113,14 +125,18 @@ bool ScanBinary(BinFile &b, std::vector<BinKwd> *kw)
       {
          long long d = 0;
          if(!b.Int(&d, false)) return false;
+         if(d < 1 || d > 3) return false;
          b.dim = (int)d;
       }
       else if(Gmf2Gml((int)kwd) >= 0)
       {
          if(!b.Int(&n, b.ver >= 4)) return false;
-         kw->push_back({(int)kwd, (long long)ftello(b.f), n});
+         const long long pos = (long long)ftello(b.f);
+         if(n < 0 || n > (b.size - pos) / MinLine(b, Gmf2Gml((int)kwd))) return false;
+         kw->push_back({(int)kwd, pos, n});
       }
       if(!nxt) return true;
+      if(nxt <= start || nxt < (long long)ftello(b.f) || nxt > b.size) return false;
       if(fseeko(b.f, (off_t)nxt, SEEK_SET)) return false;
    }
 }
@@ -209,7 +225,13 @@ bool ReadAscii(const char *nam, const std::vector<int> &want, std::vector<Block>
       for(int i = 0; i < GmlMaxEleTyp; i++)
          if(tok == GmfMshNam[i]) typ = i;
       if(tok == "MeshVersionFormatted") { t.Tok(); tok = t.Tok(); continue; }
-      if(tok == "Dimension") { dim = atoi(t.Tok().c_str()); tok = t.Tok(); continue; }
+      if(tok == "Dimension")
+      {
+         dim = atoi(t.Tok().c_str());
+         if(dim < 1 || dim > 3) return false;
+         tok = t.Tok();
+         continue;
+      }
       int slot = -1;
       for(size_t k = 0; k < want.size(); k++)
          if(want[k] == typ) slot = (int)k;
@@ -222,6 +244,8 @@ bool ReadAscii(const char *nam, const std::vector<int> &want, std::vector<Block>
       }
       Block &b = (*out)[slot];
       b.n = atoll(t.Tok().c_str());
+      // every line takes at least two characters per number
+      if(b.n < 0 || b.n > (long long)(t.buf.size() / 2)) return false;
       b.ref.resize(b.n);
       if(typ == GmlVertices)
       {
@@ -252,10 +276,18 @@ bool ReadAscii(const char *nam, const std::vector<int> &want, std::vector<Block>
    return true;
 }
 
-// new mesh table of kind typ holding a block (host mirror filled, uploaded)
-int Install(size_t GmlIdx, GmlCtx *g, int typ, const Block &b)
+// new mesh table of kind typ holding a block (host mirror filled, uploaded);
+// element node indices must lie in the vertex table (nver vertices)
+int Install(size_t GmlIdx, GmlCtx *g, int typ, const Block &b, long long nver)
 {
    if(b.n < 1 || b.n >= INT_MAX) return 0;
+   if(typ != GmlVertices)
+      for(int v : b.nod)
+         if(v < 0 || v >= nver)
+         {
+            printf("%s: a node index is outside the %lld vertices\n", GmfMshNam[typ], nver);
+            return 0;
+         }
    const int idx = GmlNewMeshData(GmlIdx, typ, (int)b.n);
    if(!idx) return 0;
    Data *d = &g->dat[idx], *r = &g->dat[g->RefIdx[typ]];
@@ -305,9 +337,10 @@ extern "C" {
 // codes, at most 10) of a .mesh / .meshb file into new mesh tables.
 // Returns the number of requested keywords the engine knows (present in the
 // file or not), 0 if the file cannot be read.
+static int ImportMesh(size_t GmlIdx, const char *MshNam, const std::vector<int> &want);
+
 int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
 {
-   GML_CTX(g, GmlIdx);
    std::vector<int> want;
    va_list ap;
    va_start(ap, MshNam);
@@ -318,6 +351,21 @@ int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
    }
    va_end(ap);
    if(!MshNam) return 0;
+   // nothing thrown by the reader (an allocation failure) crosses the C ABI
+   try
+   {
+      return ImportMesh(GmlIdx, MshNam, want);
+   }
+   catch(...)
+   {
+      printf("%s: cannot be read\n", MshNam);
+      return 0;
+   }
+}
+
+static int ImportMesh(size_t GmlIdx, const char *MshNam, const std::vector<int> &want)
+{
+   GML_CTX(g, GmlIdx);
    const std::string nam = MshNam;
    std::vector<Block> blk(want.size());
    std::vector<bool> found(want.size(), false);
@@ -325,7 +373,7 @@ int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
    {
       if(!ReadAscii(MshNam, want, &blk, &found))
       {
-         printf("Could not open file %s\n", MshNam);
+         printf("Could not read file %s\n", MshNam);
          return 0;
       }
    }
@@ -334,7 +382,8 @@ int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
       BinFile b;
       int32_t code = 0, ver = 0;
       std::vector<BinKwd> kw;
-      if(!(b.f = fopen(MshNam, "rb")) || fread(&code, 4, 1, b.f) != 1)
+      if(!(b.f = fopen(MshNam, "rb")) || fseeko(b.f, 0, SEEK_END) || (b.size = (long long)ftello(b.f)) < 8
+      || fseeko(b.f, 0, SEEK_SET) || fread(&code, 4, 1, b.f) != 1)
       {
          printf("Could not open file %s\n", MshNam);
          return 0;
@@ -354,7 +403,7 @@ int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
       b.dim = 3;
       if(!ScanBinary(b, &kw))
       {
-         printf("%s: truncated file\n", MshNam);
+         printf("%s: truncated or inconsistent file\n", MshNam);
          return 0;
       }
       for(size_t k = 0; k < want.size(); k++)
@@ -374,7 +423,7 @@ int GmlImportMesh(size_t GmlIdx, char *MshNam, ...)
    for(int pass = 0; pass < 2; pass++)
       for(size_t k = 0; k < want.size(); k++)
          if(found[k] && (want[k] == GmlVertices) == (pass == 0) && !g->TypIdx[want[k]])
-            Install(GmlIdx, g, want[k], blk[k]);
+            Install(GmlIdx, g, want[k], blk[k], g->NmbEle[GmlVertices]);
    return (int)want.size();
 }
 

@@ -284,3 +284,56 @@ def test_extract_edges_of_reference_surface_mesh(tmp_path):
     assert len(keys) == ne
     assert np.array_equal(np.sort(want[:, 0].astype(np.int64) * V + want[:, 1]), keys)
     assert (cnt == 2).all() and 2 * ne == 3 * len(tri)       # a closed surface
+
+
+def _bin(ver, words):
+    """a binary file from (format, value) pairs after the 1 / version header"""
+    out = bytearray(struct.pack("<ii", 1, ver))
+    for f, v in words:
+        out += struct.pack("<" + f, v)
+    return bytes(out)
+
+
+@pytest.mark.timeout(30)
+def test_import_rejects_malformed_binary_files(tmp_path):
+    # the reader does not trust the file: negative or oversized counts, a
+    # next-keyword position that does not advance (it would loop), past the
+    # end of the file, or an impossible dimension give 0 and no table
+    V = GMF["Vertices"]
+    cases = {
+        "negative_count": [("i", GMF["Dimension"]), ("i", 20), ("i", 3), ("i", V), ("i", 32), ("i", -5)],
+        "count_beyond_file": [("i", GMF["Dimension"]), ("i", 20), ("i", 3), ("i", V), ("i", 32), ("i", 1 << 30)],
+        "position_loops": [("i", GMF["Dimension"]), ("i", 8), ("i", 3)],
+        "position_past_end": [("i", GMF["Dimension"]), ("i", 1 << 20), ("i", 3)],
+        "bad_dimension": [("i", GMF["Dimension"]), ("i", 20), ("i", 1000000)],
+    }
+    for name, words in cases.items():
+        path = str(tmp_path / (name + ".meshb"))
+        open(path, "wb").write(_bin(1, words))
+        with G.Gml(host_only=True) as g:
+            assert imp(g, path, GMF["Vertices"]) == 0, name
+            assert not g.L.GmlGetMeshInfo(g.h, G.GmlVertices, None, None), name
+
+
+def test_import_rejects_out_of_range_nodes(tmp_path):
+    # a tet referencing vertex 9 of a file with 4 vertices: the tets are not
+    # installed (a generated gather would read outside the vertex table)
+    verts = np.array([[0, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1]], float)
+    tets = np.array([[1, 2, 3, 9, 0]])
+    for ver in (1, 4):
+        path = str(tmp_path / ("bad%d.meshb" % ver))
+        write_meshb(path, ver, 3, [("Vertices", verts), ("Tetrahedra", tets)])
+        with G.Gml(host_only=True) as g:
+            imp(g, path, GMF["Vertices"], GMF["Tetrahedra"])
+            assert g.L.GmlGetMeshInfo(g.h, G.GmlVertices, None, None)
+            assert not g.L.GmlGetMeshInfo(g.h, G.GmlTetrahedra, None, None)
+    path = str(tmp_path / "bad.mesh")
+    write_mesh_ascii(path, 3, [("Vertices", verts), ("Tetrahedra", tets)])
+    with G.Gml(host_only=True) as g:
+        imp(g, path, GMF["Vertices"], GMF["Tetrahedra"])
+        assert not g.L.GmlGetMeshInfo(g.h, G.GmlTetrahedra, None, None)
+    # an ASCII count far beyond the file
+    path = str(tmp_path / "huge.mesh")
+    open(path, "w").write("MeshVersionFormatted 2\nDimension\n3\nVertices\n99999999999\n0 0 0 1\nEnd\n")
+    with G.Gml(host_only=True) as g:
+        assert imp(g, path, GMF["Vertices"]) == 0
