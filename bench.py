@@ -190,6 +190,16 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+FORMS = {1: "one row", 2: "paired", 3: "element tiles"}
+
+
+def tuned(L, h, k):
+    """whether a generated kernel's launch-shape tuner has settled"""
+    a, b = C.c_int(), C.c_int()
+    L.GmlxGetKernelShape(h, k, C.byref(a), C.byref(b))
+    return a.value != 0
+
+
 def host_baseline(crd0, ref, tet, deg, ball, gpu_gather_src, omega, sweep_bytes, n):
     """CPU baseline of the smoothing sweep: the engine's own generated
     scatter and gather kernels (OpenCL-C bodies, prelude and GMlib
@@ -1295,6 +1305,10 @@ def run_heat(args):
     log("heat: %d^3 cube, %d tets, %d faces, set-up (mesh, faces, links, compile, dt) %.1fs"
         % (n, T, F, time.time() - t0))
     s.iterate(max(args.warmup, 1))
+    for _ in range(100):                 # until every kernel's launch-shape tuner has settled
+        if all(tuned(s.g.L, s.g.h, k) for k in (s.k_face, s.k_grad, s.k_fgrad, s.k_bal, s.k_step)):
+            break
+        s.iterate(1)
     kern = {"heat_face_value": s.k_face, "heat_cell_gradient": s.k_grad, "heat_face_gradient": s.k_fgrad,
             "heat_balance": s.k_bal, "heat_step": s.k_step}
     k0 = {k: s.g.kernel_time(v) for k, v in kern.items()}
@@ -1331,7 +1345,9 @@ def run_heat(args):
         "config": {"workload": "cell-centered heat equation (HeatCellCentered), %d^3-cell Kuhn cube (%d tets, "
                                "%d faces, %d vertices), one explicit iteration + L2 residual per step" % (n, T, F, V),
                    "tets": T, "faces": F, "vertices": V, "bytes_per_step": alg,
-                   "kernel_us": {k: round(v * 1e6, 2) for k, v in kt.items()}},
+                   "kernel_us": {k: round(v * 1e6, 2) for k, v in kt.items()},
+                   "kernel_frac": {k: round(kb[k] / v / 1e9 / HBM_PEAK_GBS, 4) for k, v in kt.items()},
+                   "kernel_form": {k: FORMS.get(s.g.L.GmlxGetKernelForm(s.g.h, v), "?") for k, v in kern.items()}},
         "iterations_per_s": round(1 / dt, 1),
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
@@ -1372,7 +1388,7 @@ def bandwidth_models(V, T):
     return alg
 
 
-def bandwidth_pass(G, crd, ref, tet, steps, warmup, tag=""):
+def bandwidth_pass(args, G, crd, ref, tet, steps, warmup, tag=""):
     """the four GpuBandwidth kernels on one handle: per-kernel average
     seconds (HIP events on the library's stream), GmlEvaluateNumbering;
     tag is appended to the kernel names (profiles tell the passes apart)"""
@@ -1391,10 +1407,12 @@ def bandwidth_pass(G, crd, ref, tet, steps, warmup, tag=""):
                                          [(vi, R, 0), (tdat, W, 0)]),
               "vertices_ball": g.compile(body("bw_vertices_ball"), "vertices_ball" + tag, G.GmlVertices,
                                          [(vdat, W, 0), (tdat, R, 0)])}
-        t = {}
+        t, forms = {}, {}
         for name, k in ks.items():           # the example's order: direct, indirect, ragged
-            # past the launch tuner's shapes (at most 11 launches)
-            for _ in range(max(warmup, 12)):
+            # past the launch tuner's trials of every shape, then the warm-up
+            for i in range(200):
+                if i >= warmup and tuned(g.L, g.h, k):
+                    break
                 if g.L.GmlLaunchKernel(g.h, k) != 1:
                     raise RuntimeError("launch failed")
             g.L.GmlxSync(g.h)
@@ -1403,7 +1421,8 @@ def bandwidth_pass(G, crd, ref, tet, steps, warmup, tag=""):
                 g.L.GmlLaunchKernel(g.h, k)
             g.L.GmlxSync(g.h)
             t[name] = (g.kernel_time(k) - t0) / steps
-        return t, score
+            forms[name] = FORMS.get(g.L.GmlxGetKernelForm(g.h, k), "?")
+        return t, score, forms
     finally:
         g.close()
 
@@ -1427,20 +1446,21 @@ def run_bandwidth(args):
     log("bandwidth: %d^3 cube (V=%d, T=%d), random numbering in %.1fs" % (n, V, T, time.time() - t0))
     alg = bandwidth_models(V, T)
     res = {}
-    tr, sr = bandwidth_pass(G, crdr, refr, tetr, args.steps, args.warmup, "_random")
-    res["random"] = (tr, sr)
+    tr, sr, fr = bandwidth_pass(args, G, crdr, refr, tetr, args.steps, args.warmup, "_random")
+    res["random"] = (tr, sr, fr)
     crdh, refh, teth = crdr.copy(), refr.copy(), tetr.copy()
     t0 = time.time()
     if G.lib().GmlxHilbertRenumber(V, G.ptr(crdh), G.ptr(refh), T, 4, G.ptr(teth), None, None, None, 0) != 1:
         raise RuntimeError("GmlxHilbertRenumber failed")
     log("bandwidth: GPU Hilbert renumbering %.2fs" % (time.time() - t0))
     del crdr, refr, tetr
-    th, sh = bandwidth_pass(G, crdh, refh, teth, args.steps, args.warmup)
-    res["hilbert"] = (th, sh)
+    th, sh, fh = bandwidth_pass(args, G, crdh, refh, teth, args.steps, args.warmup)
+    res["hilbert"] = (th, sh, fh)
 
-    def summary(t, score):
+    def summary(t, score, forms):
         k = {name: {"avg_us": round(v * 1e6, 2), "alg_GBs": round(alg[name] / v / 1e9, 1),
-                    "frac": round(alg[name] / v / 1e9 / HBM_PEAK_GBS, 4)} for name, v in t.items()}
+                    "frac": round(alg[name] / v / 1e9 / HBM_PEAK_GBS, 4), "form": forms[name]}
+             for name, v in t.items()}
         direct = t["direct_vertices"] + t["direct_tets"]
         refm = {"direct_GBs": round((32 * V + 32 * T) / direct / 1e9, 1),
                 "indirect_GBs": round(96 * T / t["indirect_tets"] / 1e9, 1),
