@@ -1114,6 +1114,13 @@ def run_batch(args):
     tf = os.path.join(ROOT, "profiles", "traffic_batch512.json")
     if args.nmesh == 512 and os.path.exists(tf):
         traffic = json.load(open(tf)).get("batch_quality")
+    # the kernel's own compulsory bytes: its tile plan (vertex lists, tile
+    # tets as 4 x uint16, ball slabs as uint16, offsets), the coordinates of
+    # every tile's vertices (halo included), quality and ball minimum out,
+    # the per-tile partials -- fewer than the model's 36 T + 20 V
+    bi = (C.c_longlong * 12)()
+    L.GmlxGetBatchInfo(h, b, bi)
+    own = bi[6] + 16 * bi[8] + 4 * T + 4 * V + 16 * bi[3]
     out = {
         "metric": "HBM GB/s (% peak) per kernel; served mesh-jobs/s + p99 vs plain path",
         "value": round(alg / dt / 1e9, 2),
@@ -1137,7 +1144,9 @@ def run_batch(args):
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_batch_quality<%d> (graph: + k_batch_finish)" % pad.value,
-                     "bytes_per_launch": alg, "avg_us": round(kavg * 1e6, 2)},
+                     "bytes_per_launch": alg, "avg_us": round(kavg * 1e6, 2),
+                     "own_bytes_per_launch": own, "own_frac": round(own / kavg / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic_frac": round(traffic / kavg / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "cpu_baseline": None,
     }
     if args.cpu_seconds > 0:
