@@ -200,6 +200,20 @@ def tuned(L, h, k):
     return a.value != 0
 
 
+def opencl_devices():
+    """Device types the OpenCL platforms of this machine report (clinfo, in a
+    child process): a CPU device would run the reference-style kernels
+    directly; None if clinfo is absent."""
+    import re
+    import subprocess
+    try:
+        r = subprocess.run(["clinfo"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=60)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    types = re.findall(r"Device Type:\s*(\S+)", r.stdout)
+    return {"cpu": sum("CPU" in t for t in types), "gpu": sum("GPU" in t for t in types)}
+
+
 def host_baseline(crd0, ref, tet, deg, ball, gpu_gather_src, omega, sweep_bytes, n):
     """CPU baseline of the smoothing sweep: the engine's own generated
     scatter and gather kernels (OpenCL-C bodies, prelude and GMlib
@@ -229,7 +243,7 @@ def host_baseline(crd0, ref, tet, deg, ball, gpu_gather_src, omega, sweep_bytes,
         times.append(time.perf_counter() - t1)
     med = float(np.median(times))
     rec = {"value": round(sweep_bytes / med / 1e9, 2), "unit": "GB/s", "cores": hk.threads, "kind": "port",
-           "sweep_ms_median": round(med * 1e3, 2),
+           "sweep_ms_median": round(med * 1e3, 2), "opencl_devices": opencl_devices(),
            "sample": "the generated smooth_scatter + smooth_gather kernels (project OpenCL-C bodies with the "
                      "engine's GMlib prologue/epilogue, one-row form) compiled for x86-64 by clang and run with "
                      "OpenMP over %d host threads, no CPU OpenCL runtime present; the same %d^3 mesh and initial "
