@@ -469,6 +469,58 @@ k_item_dedup(long long ne, int len, const int *__restrict__ ele, LocalItems li, 
       }
 }
 
+// The same block-local pass for one-word keys, without rounds: an item
+// claims a slot or finds its key there with one 64-bit compare-and-swap on
+// the slot's key (EMPTY marks a free slot: a one-word key of valid nodes
+// stays below 2^62), then keeps the smaller position with an atomic min.
+// Every lane probes on its own, with no barrier in the loop; the table
+// keeps the same load bound, so every probe sequence reaches a free slot or
+// its key.  An item with a node out of range raises *bad and is not
+// inserted.
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+template <int HS>
+__global__ void __launch_bounds__(kDedupThreads)
+k_item_dedup1(long long ne, int len, const int *__restrict__ ele, LocalItems li, KeyForm kf, long long nv,
+              long long pos0, unsigned long long *__restrict__ olo, int *__restrict__ opos,
+              unsigned long long *__restrict__ ocnt, int *__restrict__ bad)
+{
+   __shared__ unsigned long long hkey[HS];
+   __shared__ int hpos[HS];
+   __shared__ unsigned int nuniq, nout;
+   __shared__ unsigned long long base;
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads) { hkey[i] = kEmptyKey; hpos[i] = INT_MAX; }
+   if(threadIdx.x == 0) nuniq = nout = 0;
+   __syncthreads();
+   const long long e = (long long)blockIdx.x * kDedupThreads + threadIdx.x;
+   if(e < ne)
+      for(int j = 0; j < li.n; j++)
+      {
+         unsigned long long lo = 0, hi = 0;
+         if(!item_key(ele, e, len, li, j, kf, nv, &lo, &hi)) { *bad = 1; continue; }
+         const int pos = (int)(pos0 + e * li.n + j);
+         unsigned int sl = dedup_slot(lo, 0) & (HS - 1);
+         for(;;)
+         {
+            const unsigned long long prev = atomicCAS(&hkey[sl], kEmptyKey, lo);
+            if(prev == kEmptyKey) { atomicAdd(&nuniq, 1u); break; }
+            if(prev == lo) break;
+            sl = (sl + 1) & (HS - 1);
+         }
+         atomicMin(&hpos[sl], pos);
+      }
+   __syncthreads();
+   if(threadIdx.x == 0) base = atomicAdd(ocnt, (unsigned long long)nuniq);
+   __syncthreads();
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads)
+      if(hkey[i] != kEmptyKey)
+      {
+         const unsigned long long k = base + atomicAdd(&nout, 1u);
+         olo[k] = hkey[i];
+         opos[k] = hpos[i];
+      }
+}
+
 // for each run of equal keys in sorted order (after the block-local pass a
 // key occurs at most once per work-group): its head gets flag 1 and the
 // run's smallest position
@@ -506,6 +558,149 @@ __global__ void k_item_write(long long m, const int *__restrict__ u, const int *
       hi >>= kf.b;
    }
    for(int j = 0; j < w; j++) tab[k * w + j] = j < kf.deg ? a[j] : 0;
+   ref[k] = (p < nold && oref) ? oref[p] : 0;
+}
+
+// Bucketed first appearances, the default after the block-local pass: each
+// block-distinct item goes to a bucket named by its lowest node, a vertex
+// (with a Hilbert-ordered numbering a work-group's buckets lie close
+// together), split four ways by its second node, and one counting atomic
+// per item gives its slot there; after a scan of the bucket sizes the items
+// are copied into bucket order, each flags its position if no item of its
+// bucket has an equal key at a smaller position (the item's first
+// appearance), and a compaction of all positions lists the flagged ones in
+// ascending order: the new table's order.  A bucket of more than kMaxBucket
+// items (one vertex under very many items) sends the call down the sorted
+// path instead, which has no such bound.
+constexpr int kMaxBucket = 256;
+
+// node i of a key (0 = the lowest node: the most significant b bits)
+__device__ __forceinline__ int key_node(unsigned long long lo, unsigned long long hi, const KeyForm &kf, int i)
+{
+   const int sh = kf.b * (kf.deg - 1 - i);
+   const unsigned long long v = sh >= 64 ? hi >> (sh - 64) : (lo >> sh) | (sh ? hi << (64 - sh) : 0ull);
+   return (int)(v & ((1ull << kf.b) - 1));
+}
+
+// the bucket of a key: its lowest node, split kSubBuckets ways by the low
+// bits of its second node (equal keys share both)
+constexpr int kSubBuckets = 4;
+__device__ __forceinline__ long long key_bucket(unsigned long long lo, unsigned long long hi, const KeyForm &kf)
+{
+   return (long long)key_node(lo, hi, kf, 0) * kSubBuckets + (key_node(lo, hi, kf, 1) & (kSubBuckets - 1));
+}
+
+__global__ void k_bucket_count(long long n, const unsigned long long *__restrict__ klo,
+                               const unsigned long long *__restrict__ khi, KeyForm kf, int *__restrict__ cnt,
+                               int *__restrict__ slot)
+{
+   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(i >= n) return;
+   slot[i] = atomicAdd(&cnt[key_bucket(klo[i], khi ? khi[i] : 0ull, kf)], 1);
+}
+
+__global__ void k_bucket_fill(long long n, const unsigned long long *__restrict__ klo,
+                              const unsigned long long *__restrict__ khi, const int *__restrict__ pos,
+                              const int *__restrict__ slot, const int *__restrict__ off, KeyForm kf,
+                              unsigned long long *__restrict__ blo, unsigned long long *__restrict__ bhi,
+                              int *__restrict__ bpos)
+{
+   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(i >= n) return;
+   const unsigned long long lo = klo[i], hi = khi ? khi[i] : 0ull;
+   const long long d = (long long)off[key_bucket(lo, hi, kf)] + slot[i];
+   blo[d] = lo;
+   if(bhi) bhi[d] = hi;
+   bpos[d] = pos[i];
+}
+
+// one thread per item in bucket order: flag[p] = 1 for the item at position
+// p if no item of its bucket has an equal key at a smaller position.  The
+// lanes of a wave scan the same or neighbouring buckets, so the loop's loads
+// are shared through the cache.
+__global__ void k_bucket_first(long long n, KeyForm kf, const int *__restrict__ cnt, const int *__restrict__ off,
+                               const unsigned long long *__restrict__ blo, const unsigned long long *__restrict__ bhi,
+                               const int *__restrict__ bpos, unsigned char *__restrict__ flag)
+{
+   const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(d >= n) return;
+   const unsigned long long lo = blo[d], hi = bhi ? bhi[d] : 0ull;
+   const long long a = key_bucket(lo, hi, kf), b = off[a], e = b + cnt[a];
+   const int p = bpos[d];
+   for(long long k = b; k < e; k++)
+      if(blo[k] == lo && (!bhi || bhi[k] == hi) && bpos[k] < p) return;
+   flag[p] = 1;
+}
+
+// Stream compaction of the flagged positions, in order: kFlagChunk
+// positions per work-group, kFlagChunk / 256 = 16 flag bytes per thread read
+// as one 16-byte load.  A count pass, a scan of the per-chunk counts, then
+// an emit pass with a work-group scan.
+constexpr int kFlagChunk = 4096;
+
+__device__ __forceinline__ int flag_bits(const uint4 &v)
+{
+   return __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);   // each flag byte is 0 or 1
+}
+
+__global__ void __launch_bounds__(256) k_flag_count(const uint4 *__restrict__ flag, int *__restrict__ ccnt)
+{
+   typedef hipcub::BlockReduce<int, 256> Red;
+   __shared__ typename Red::TempStorage tmp;
+   const int c = flag_bits(flag[(size_t)blockIdx.x * 256 + threadIdx.x]);
+   const int t = Red(tmp).Sum(c);
+   if(threadIdx.x == 0) ccnt[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_flag_emit(const uint4 *__restrict__ flag, const int *__restrict__ coff,
+                                                   int *__restrict__ sel)
+{
+   typedef hipcub::BlockScan<int, 256> Scan;
+   __shared__ typename Scan::TempStorage tmp;
+   const uint4 v = flag[(size_t)blockIdx.x * 256 + threadIdx.x];
+   int at;
+   Scan(tmp).ExclusiveSum(flag_bits(v), at);
+   at += coff[blockIdx.x];
+   const int p0 = blockIdx.x * kFlagChunk + threadIdx.x * 16;
+   const unsigned w[4] = {v.x, v.y, v.z, v.w};
+   for(int i = 0; i < 16; i++)
+      if((w[i >> 2] >> (8 * (i & 3))) & 0xff) sel[at++] = p0 + i;
+}
+
+// the sources of an extraction and the first item position of each
+constexpr int kMaxSources = 8;
+struct SourceSet {
+   int n;
+   long long at[kMaxSources];
+   Source s[kMaxSources];
+};
+
+// row k of the new table from the item at position sel[k]: its element's
+// nodes, sorted (decoded from the key as k_item_write does), its ref (the
+// existing table's ref for a position in it, else 0)
+__global__ void k_item_write_pos(long long m, const int *__restrict__ sel, SourceSet ss, KeyForm kf, long long nv,
+                                 long long nold, const int *__restrict__ oref, int w, int *__restrict__ tab,
+                                 int *__restrict__ ref)
+{
+   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(k >= m) return;
+   const long long p = sel[k];
+   int q = 0;
+   while(q + 1 < ss.n && p >= ss.at[q + 1]) q++;
+   const Source &sr = ss.s[q];
+   const long long e = (p - ss.at[q]) / sr.li.n;
+   const int j = (int)(p - ss.at[q] - e * sr.li.n);
+   unsigned long long lo, hi;
+   (void)item_key(sr.ele, e, sr.len, sr.li, j, kf, nv, &lo, &hi);
+   int a[4] = {0, 0, 0, 0};
+   const unsigned long long mask = (1ull << kf.b) - 1;
+   for(int i = kf.deg - 1; i >= 0; i--)
+   {
+      a[i] = (int)(lo & mask);
+      lo = (lo >> kf.b) | (hi << (64 - kf.b));
+      hi >>= kf.b;
+   }
+   for(int i = 0; i < w; i++) tab[k * w + i] = i < kf.deg ? a[i] : 0;
    ref[k] = (p < nold && oref) ? oref[p] : 0;
 }
 
@@ -624,12 +819,25 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    hipStream_t s = g->stream;
    const long long nv = g->NmbEle[GmlVertices];
    const KeyForm kf = MakeKeyForm(deg, nv);
+   // the sources with the position of their first item (bucketed path)
+   SourceSet ss = {};
+   ss.n = (int)std::min(src.size(), (size_t)kMaxSources);   // at most 7: the table and six element kinds
+   long long sat = 0;
+   for(int i = 0; i < ss.n; i++)
+   {
+      ss.at[i] = sat;
+      ss.s[i] = src[i];
+      sat += src[i].ne * src[i].li.n;
+   }
+   const long long nbk = nv * kSubBuckets;   // buckets: an item's lowest node, split by its second
    auto clock = []() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
    double tim[5] = {clock(), 0, 0, 0, 0};
    ItemSort it(g);
    unsigned long long *cnt = nullptr, hc[2] = {0, 0};
    int *pos = nullptr, *first = nullptr, *hsel = nullptr, *usel = nullptr, *p2 = nullptr, *u2 = nullptr, *nsel = nullptr;
-   unsigned char *head = nullptr;
+   int *bcnt = nullptr, *boff = nullptr, *slot = nullptr, *sel = nullptr, *ccnt = nullptr, *coff = nullptr;
+   unsigned char *head = nullptr, *flag = nullptr;
+   void *xtmp = nullptr;
    int rc = -1, hbad = 0, m = 0;
    // 1. block-local first appearances of every source's items
    bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
@@ -642,10 +850,16 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       const dim3 grid((unsigned)((sr.ne + kDedupThreads - 1) / kDedupThreads));
       if(!sr.ne) continue;
       const bool small = sr.li.n <= 6;
-      auto kern = kf.two ? (small ? k_item_dedup<true, 2048> : k_item_dedup<true, 4096>)
-                         : (small ? k_item_dedup<false, 2048> : k_item_dedup<false, 4096>);
-      hipLaunchKernelGGL(kern, grid, dim3(kDedupThreads), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo,
-                         kf.two ? it.khi : nullptr, pos, cnt, (int *)(cnt + 1));
+      if(kf.two)
+         hipLaunchKernelGGL((small ? k_item_dedup<true, 2048> : k_item_dedup<true, 4096>), grid, dim3(kDedupThreads),
+                            0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, it.khi, pos, cnt, (int *)(cnt + 1));
+      else if(kf.deg * kf.b >= 64)   // a full 64-bit key: every value may be a key
+         hipLaunchKernelGGL((small ? k_item_dedup<false, 2048> : k_item_dedup<false, 4096>), grid,
+                            dim3(kDedupThreads), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, nullptr, pos,
+                            cnt, (int *)(cnt + 1));
+      else
+         hipLaunchKernelGGL((small ? k_item_dedup1<2048> : k_item_dedup1<4096>), grid, dim3(kDedupThreads), 0, s,
+                            sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, pos, cnt, (int *)(cnt + 1));
       at += sr.ne * sr.li.n;
    }
    ok = ok && ok_(hipGetLastError()) && ok_(hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, s))
@@ -657,30 +871,84 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       fprintf(stderr, "gml: extraction: an element references a vertex outside 0..%lld\n", nv - 1);
       ok = false;
    }
-   // 2. sort the block-distinct items by key; each run's smallest position is
-   // the item's first appearance; those sorted by position are the table
    const long long n2 = (long long)hc[0];
-   if(ok)
+   // 2. bucketed path: the block-distinct items by lowest node, each bucket
+   // scanned for its keys' smallest positions, those positions selected in
+   // order; the table rows are rebuilt from the elements at them
+   bool bucketed = false;
+   size_t tb = 0;
+   const long long nchunk = (total + kFlagChunk - 1) / kFlagChunk;
+   if(ok && nbk < INT_MAX)
+   {
+      // one temporary for the bucket reduce and the two scans
+      size_t t1 = 0, t2 = 0, t3 = 0;
+      int hmax = 0;
+      ok = DAlloc(g, &bcnt, nbk * 4) && DAlloc(g, &boff, nbk * 4) && DAlloc(g, &slot, n2 * 4)
+        && DAlloc(g, &flag, nchunk * kFlagChunk) && DAlloc(g, &sel, n2 * 4) && DAlloc(g, &ccnt, (nchunk + 1) * 4)
+        && DAlloc(g, &coff, (nchunk + 1) * 4)
+        && ok_(hipcub::DeviceReduce::Max(nullptr, t1, bcnt, nsel, (int)nbk, s))
+        && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, bcnt, boff, (int)nbk, s))
+        && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, ccnt, coff, (int)(nchunk + 1), s));
+      tb = std::max(t1, std::max(t2, t3));
+      ok = ok && DAlloc(g, &xtmp, tb) && ok_(hipMemsetAsync(bcnt, 0, nbk * 4, s));
+      if(ok)
+      {
+         hipLaunchKernelGGL(k_bucket_count, blocks(n2), dim3(kB), 0, s, n2, it.klo, kf.two ? it.khi : nullptr, kf,
+                            bcnt, slot);
+         size_t t = tb;
+         ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceReduce::Max(xtmp, t, bcnt, nsel, (int)nbk, s))
+           && ok_(hipMemcpyAsync(&hmax, nsel, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+      }
+      bucketed = ok && hmax <= kMaxBucket;
+   }
+   if(bucketed)
+   {
+      size_t t = tb;
+      ok = ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, bcnt, boff, (int)nbk, s))
+        && ok_(hipMemsetAsync(flag, 0, nchunk * kFlagChunk, s)) && ok_(hipMemsetAsync(ccnt + nchunk, 0, 4, s));
+      // the sorted-key arrays are unused on this path: they take the items
+      // in bucket order
+      if(ok)
+      {
+         hipLaunchKernelGGL(k_bucket_fill, blocks(n2), dim3(kB), 0, s, n2, it.klo, kf.two ? it.khi : nullptr, pos,
+                            slot, boff, kf, it.slo, kf.two ? it.shi : nullptr, it.ord);
+         hipLaunchKernelGGL(k_bucket_first, blocks(n2), dim3(kB), 0, s, n2, kf, bcnt, boff, it.slo,
+                            kf.two ? it.shi : nullptr, it.ord, flag);
+         hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, ccnt);
+         t = tb;
+         ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, ccnt, coff, (int)(nchunk + 1), s));
+      }
+      if(ok)
+      {
+         hipLaunchKernelGGL(k_flag_emit, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, coff, sel);
+         ok = ok_(hipGetLastError()) && ok_(hipMemcpyAsync(&m, coff + nchunk, 4, hipMemcpyDeviceToHost, s))
+           && ok_(hipStreamSynchronize(s));
+      }
+   }
+   // 2'. sorted path: sort the block-distinct items by key; each run's
+   // smallest position is the item's first appearance; those sorted by
+   // position are the table
+   else if(ok)
    {
       it.n = n2;
       ok = SortPrepared(s, &it) && DAlloc(g, &head, n2) && DAlloc(g, &first, n2 * 4) && DAlloc(g, &hsel, n2 * 4)
         && DAlloc(g, &usel, n2 * 4) && DAlloc(g, &p2, n2 * 4) && DAlloc(g, &u2, n2 * 4);
-   }
-   if(ok)
-   {
-      hipLaunchKernelGGL(k_run_first, blocks(n2), dim3(kB), 0, s, n2, it.slo, it.shi, it.ord, pos, head, first);
-      size_t t = it.tbytes;
-      ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, first, head, hsel, nsel, (int64_t)n2, s));
-      t = it.tbytes;
-      ok = ok && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, it.ord, head, usel, nsel, (int64_t)n2, s))
-        && ok_(hipMemcpyAsync(&m, nsel, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+      if(ok)
+      {
+         hipLaunchKernelGGL(k_run_first, blocks(n2), dim3(kB), 0, s, n2, it.slo, it.shi, it.ord, pos, head, first);
+         size_t t = it.tbytes;
+         ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, first, head, hsel, nsel, (int64_t)n2, s));
+         t = it.tbytes;
+         ok = ok && ok_(hipcub::DeviceSelect::Flagged(it.tmp, t, it.ord, head, usel, nsel, (int64_t)n2, s))
+           && ok_(hipMemcpyAsync(&m, nsel, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+      }
+      if(ok)
+      {
+         size_t t = it.tbytes;
+         ok = ok_(hipcub::DeviceRadixSort::SortPairs(it.tmp, t, hsel, p2, usel, u2, m, 0, bits_for(total), s));
+      }
    }
    tim[3] = clock();
-   if(ok)
-   {
-      size_t t = it.tbytes;
-      ok = ok_(hipcub::DeviceRadixSort::SortPairs(it.tmp, t, hsel, p2, usel, u2, m, 0, bits_for(total), s));
-   }
    // 3. the new table (created before the old one is dropped: the old refs are read)
    if(ok)
    {
@@ -689,9 +957,13 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       else
       {
          const Data *d = &g->dat[nidx];
-         hipLaunchKernelGGL(k_item_write, blocks(m), dim3(kB), 0, s, (long long)m, u2, p2, kf, it.klo, it.khi, nold,
-                            oref ? (const int *)g->dat[oref].dev : nullptr, d->ItmLen, (int *)d->dev,
-                            (int *)g->dat[g->RefIdx[typ]].dev);
+         const int *orf = oref ? (const int *)g->dat[oref].dev : nullptr;
+         if(bucketed)
+            hipLaunchKernelGGL(k_item_write_pos, blocks(m), dim3(kB), 0, s, (long long)m, sel, ss, kf, nv, nold, orf,
+                               d->ItmLen, (int *)d->dev, (int *)g->dat[g->RefIdx[typ]].dev);
+         else
+            hipLaunchKernelGGL(k_item_write, blocks(m), dim3(kB), 0, s, (long long)m, u2, p2, kf, it.klo, it.khi,
+                               nold, orf, d->ItmLen, (int *)d->dev, (int *)g->dat[g->RefIdx[typ]].dev);
          ok = ok_(hipGetLastError()) && ok_(hipStreamSynchronize(s));
          if(old)
          {
@@ -704,13 +976,15 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    if(!ok) (void)hipGetLastError();
    (void)hipStreamSynchronize(s);
    for(void *p : {(void *)cnt, (void *)nsel, (void *)pos, (void *)first, (void *)hsel, (void *)usel, (void *)p2,
-                  (void *)u2, (void *)head})
+                  (void *)u2, (void *)head, (void *)bcnt, (void *)boff, (void *)slot, (void *)sel, (void *)flag, xtmp,
+                  (void *)ccnt, (void *)coff})
       ItemFree(g, p);
    if(rc < 0 && !hbad) fprintf(stderr, "gml: GPU extraction of %lld items failed\n", total);
    tim[4] = clock();
-   DbgPrint(g, "Extraction of %lld items (%lld after the block pass, %d distinct): allocation %.1f ms, "
-            "block pass %.1f ms, sorts and selects %.1f ms, table and frees %.1f ms\n", total, n2, m,
-            1e3 * (tim[1] - tim[0]), 1e3 * (tim[2] - tim[1]), 1e3 * (tim[3] - tim[2]), 1e3 * (tim[4] - tim[3]));
+   DbgPrint(g, "Extraction of %lld items (%lld after the block pass, %d distinct, %s path): allocation %.1f ms, "
+            "block pass %.1f ms, first appearances %.1f ms, table and frees %.1f ms\n", total, n2, m,
+            bucketed ? "bucketed" : "sorted", 1e3 * (tim[1] - tim[0]), 1e3 * (tim[2] - tim[1]),
+            1e3 * (tim[3] - tim[2]), 1e3 * (tim[4] - tim[3]));
    return rc;
 }
 

@@ -130,9 +130,12 @@ def test_gpu_extract_tets_against_restatement(n):
     assert np.array_equal(f[G.GmlTriangles][0], want) and np.array_equal(f[G.GmlTriangles][1], wref)
 
 
-@pytest.mark.parametrize("spread", [1, 40000])
+@pytest.mark.parametrize("spread", [1, 800, 40000])
 def test_gpu_extract_mixed_kinds_same_as_host(spread):
-    # spread 40000: 2.56M vertices (22-bit ids) -- two-word keys for triangles and quads
+    # spread 800: 51,200 vertices (16-bit ids) -- quad keys fill all 64 bits of
+    # one word (the round-based block pass), edges and triangles take the
+    # compare-and-swap pass; spread 40000: 2.56M vertices (22-bit ids) --
+    # two-word keys for triangles and quads
     nv, tabs = mixed_mesh(3, spread)
     quads = np.array([tabs[G.GmlHexahedra][0, [4, 5, 6, 7]][::-1], [0, spread, 2 * spread, 3 * spread]], np.int32)
     extra = {G.GmlQuadrilaterals: (quads, np.array([21, 22], np.int32))}
@@ -252,3 +255,63 @@ def test_gpu_extract_reference_sample_mesh(tmp_path):
     assert ne == len(want) and np.array_equal(got[0], want) and np.array_equal(got[1], wref)
     assert np.array_equal(got[0][:len(old)], np.sort(old, axis=1))          # the file's edges, nodes sorted
     assert (got[1][:len(old)] == oref).all() and (got[1][len(old):] == 0).all()
+
+
+def extract_logged(g, what, capfd):
+    """run one extraction with the debug log on: (count, path taken)"""
+    capfd.readouterr()
+    g.L.GmlDebugOn(g.h)
+    n = g.L.GmlExtractEdges(g.h) if what == "edges" else g.L.GmlExtractFaces(g.h)
+    g.L.GmlDebugOff(g.h)
+    out = capfd.readouterr().out
+    path = "bucketed" if "bucketed path" in out else "sorted" if "sorted path" in out else None
+    return n, path
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_gpu_extract_cube_tables_against_oracle(capfd, shuffle):
+    # a 40^3 cube (384,000 tets; 2.3M edge items, 1.5M face items), in its
+    # Hilbert numbering and with vertices and tets shuffled (buckets then
+    # spread over the whole vertex range): tables and order bit-identical to
+    # the oracle's restatement of the reference hash
+    crd, ref, tet = G.cube_mesh(40, 0.2, 17)
+    if shuffle:
+        rng = np.random.default_rng(3)
+        pv = rng.permutation(len(crd))
+        inv = np.empty_like(pv)
+        inv[pv] = np.arange(len(crd))
+        crd, ref = np.ascontiguousarray(crd[pv]), np.ascontiguousarray(ref[pv])
+        tet = np.ascontiguousarray(inv[tet[rng.permutation(len(tet))]].astype(np.int32))
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe == "bucketed" and pf == "bucketed"
+    assert ne == len(want_e[0]) and nf == len(want_f[0])
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+def test_gpu_extract_large_bucket_takes_sorted_path(capfd):
+    # a fan of 3000 tets (0, i, i+1, i+2): vertex 0 is the lowest node of
+    # thousands of distinct edges and faces, more than a bucket takes, so the
+    # call goes down the sorted path; its tables must still equal the oracle's
+    nt = 3000
+    tet = np.array([[0, i, i + 1, i + 2] for i in range(1, nt + 1)], np.int32)
+    nv = nt + 3
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        set_vertices(g, nv)
+        set_block(g, G.GmlTetrahedra, tet, np.zeros(nt, np.int32))
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe == "sorted" and pf == "sorted"
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
