@@ -599,37 +599,51 @@ __global__ void k_bucket_count(long long n, const unsigned long long *__restrict
    slot[i] = atomicAdd(&cnt[key_bucket(klo[i], khi ? khi[i] : 0ull, kf)], 1);
 }
 
+// an item in bucket order: its key and position in one record, so that the
+// scattered copy into bucket order is one store per item
+struct BucketItem1 { unsigned long long lo; int pos, pad; };
+struct BucketItem2 { unsigned long long lo, hi; int pos, pad; };
+__device__ __forceinline__ unsigned long long item_hi(const BucketItem1 &) { return 0ull; }
+__device__ __forceinline__ unsigned long long item_hi(const BucketItem2 &r) { return r.hi; }
+__device__ __forceinline__ void set_hi(BucketItem1 &, unsigned long long) {}
+__device__ __forceinline__ void set_hi(BucketItem2 &r, unsigned long long hi) { r.hi = hi; }
+__device__ __forceinline__ void pad_item(BucketItem1 &r) { r.pad = 0; }
+__device__ __forceinline__ void pad_item(BucketItem2 &r) { r.pad = 0; }
+
+template <class R>
 __global__ void k_bucket_fill(long long n, const unsigned long long *__restrict__ klo,
                               const unsigned long long *__restrict__ khi, const int *__restrict__ pos,
                               const int *__restrict__ slot, const int *__restrict__ off, KeyForm kf,
-                              unsigned long long *__restrict__ blo, unsigned long long *__restrict__ bhi,
-                              int *__restrict__ bpos)
+                              R *__restrict__ rec)
 {
    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(i >= n) return;
-   const unsigned long long lo = klo[i], hi = khi ? khi[i] : 0ull;
-   const long long d = (long long)off[key_bucket(lo, hi, kf)] + slot[i];
-   blo[d] = lo;
-   if(bhi) bhi[d] = hi;
-   bpos[d] = pos[i];
+   R r;
+   r.lo = klo[i];
+   set_hi(r, khi ? khi[i] : 0ull);
+   r.pos = pos[i];
+   pad_item(r);
+   rec[(long long)off[key_bucket(r.lo, item_hi(r), kf)] + slot[i]] = r;
 }
 
 // one thread per item in bucket order: flag[p] = 1 for the item at position
 // p if no item of its bucket has an equal key at a smaller position.  The
 // lanes of a wave scan the same or neighbouring buckets, so the loop's loads
 // are shared through the cache.
+template <class R>
 __global__ void k_bucket_first(long long n, KeyForm kf, const int *__restrict__ cnt, const int *__restrict__ off,
-                               const unsigned long long *__restrict__ blo, const unsigned long long *__restrict__ bhi,
-                               const int *__restrict__ bpos, unsigned char *__restrict__ flag)
+                               const R *__restrict__ rec, unsigned char *__restrict__ flag)
 {
    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
    if(d >= n) return;
-   const unsigned long long lo = blo[d], hi = bhi ? bhi[d] : 0ull;
-   const long long a = key_bucket(lo, hi, kf), b = off[a], e = b + cnt[a];
-   const int p = bpos[d];
+   const R r = rec[d];
+   const long long a = key_bucket(r.lo, item_hi(r), kf), b = off[a], e = b + cnt[a];
    for(long long k = b; k < e; k++)
-      if(blo[k] == lo && (!bhi || bhi[k] == hi) && bpos[k] < p) return;
-   flag[p] = 1;
+   {
+      const R q = rec[k];
+      if(q.lo == r.lo && item_hi(q) == item_hi(r) && q.pos < r.pos) return;
+   }
+   flag[r.pos] = 1;
 }
 
 // Stream compaction of the flagged positions, in order: kFlagChunk
@@ -837,7 +851,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    int *pos = nullptr, *first = nullptr, *hsel = nullptr, *usel = nullptr, *p2 = nullptr, *u2 = nullptr, *nsel = nullptr;
    int *bcnt = nullptr, *boff = nullptr, *slot = nullptr, *sel = nullptr, *ccnt = nullptr, *coff = nullptr;
    unsigned char *head = nullptr, *flag = nullptr;
-   void *xtmp = nullptr;
+   void *xtmp = nullptr, *brec = nullptr;
    int rc = -1, hbad = 0, m = 0;
    // 1. block-local first appearances of every source's items
    bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
@@ -886,6 +900,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       ok = DAlloc(g, &bcnt, nbk * 4) && DAlloc(g, &boff, nbk * 4) && DAlloc(g, &slot, n2 * 4)
         && DAlloc(g, &flag, nchunk * kFlagChunk) && DAlloc(g, &sel, n2 * 4) && DAlloc(g, &ccnt, (nchunk + 1) * 4)
         && DAlloc(g, &coff, (nchunk + 1) * 4)
+        && DAlloc(g, &brec, n2 * (kf.two ? sizeof(BucketItem2) : sizeof(BucketItem1)))
         && ok_(hipcub::DeviceReduce::Max(nullptr, t1, bcnt, nsel, (int)nbk, s))
         && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, bcnt, boff, (int)nbk, s))
         && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, ccnt, coff, (int)(nchunk + 1), s));
@@ -906,14 +921,22 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       size_t t = tb;
       ok = ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, bcnt, boff, (int)nbk, s))
         && ok_(hipMemsetAsync(flag, 0, nchunk * kFlagChunk, s)) && ok_(hipMemsetAsync(ccnt + nchunk, 0, 4, s));
-      // the sorted-key arrays are unused on this path: they take the items
-      // in bucket order
       if(ok)
       {
-         hipLaunchKernelGGL(k_bucket_fill, blocks(n2), dim3(kB), 0, s, n2, it.klo, kf.two ? it.khi : nullptr, pos,
-                            slot, boff, kf, it.slo, kf.two ? it.shi : nullptr, it.ord);
-         hipLaunchKernelGGL(k_bucket_first, blocks(n2), dim3(kB), 0, s, n2, kf, bcnt, boff, it.slo,
-                            kf.two ? it.shi : nullptr, it.ord, flag);
+         if(kf.two)
+         {
+            hipLaunchKernelGGL(k_bucket_fill<BucketItem2>, blocks(n2), dim3(kB), 0, s, n2, it.klo, it.khi, pos, slot,
+                               boff, kf, (BucketItem2 *)brec);
+            hipLaunchKernelGGL(k_bucket_first<BucketItem2>, blocks(n2), dim3(kB), 0, s, n2, kf, bcnt, boff,
+                               (const BucketItem2 *)brec, flag);
+         }
+         else
+         {
+            hipLaunchKernelGGL(k_bucket_fill<BucketItem1>, blocks(n2), dim3(kB), 0, s, n2, it.klo, nullptr, pos, slot,
+                               boff, kf, (BucketItem1 *)brec);
+            hipLaunchKernelGGL(k_bucket_first<BucketItem1>, blocks(n2), dim3(kB), 0, s, n2, kf, bcnt, boff,
+                               (const BucketItem1 *)brec, flag);
+         }
          hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, ccnt);
          t = tb;
          ok = ok_(hipGetLastError()) && ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, ccnt, coff, (int)(nchunk + 1), s));
@@ -977,7 +1000,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    (void)hipStreamSynchronize(s);
    for(void *p : {(void *)cnt, (void *)nsel, (void *)pos, (void *)first, (void *)hsel, (void *)usel, (void *)p2,
                   (void *)u2, (void *)head, (void *)bcnt, (void *)boff, (void *)slot, (void *)sel, (void *)flag, xtmp,
-                  (void *)ccnt, (void *)coff})
+                  (void *)ccnt, (void *)coff, brec})
       ItemFree(g, p);
    if(rc < 0 && !hbad) fprintf(stderr, "gml: GPU extraction of %lld items failed\n", total);
    tim[4] = clock();
