@@ -408,8 +408,10 @@ template <bool TWO, int HS>
 __global__ void __launch_bounds__(kDedupThreads)
 k_item_dedup(long long ne, int len, const int *__restrict__ ele, LocalItems li, KeyForm kf, long long nv,
              long long pos0, unsigned long long *__restrict__ olo, unsigned long long *__restrict__ ohi,
-             int *__restrict__ opos, unsigned long long *__restrict__ ocnt, int *__restrict__ bad)
+             int *__restrict__ opos, unsigned long long *__restrict__ ocnt, int *__restrict__ bad,
+             const unsigned char *__restrict__ only)
 {
+   if(only && !only[blockIdx.x]) return;   // a block the relative-key pass took
    __shared__ unsigned long long hlo[HS];
    __shared__ unsigned long long hhi[TWO ? HS : 1];
    __shared__ int hpos[HS], hown[HS];
@@ -517,6 +519,96 @@ k_item_dedup1(long long ne, int len, const int *__restrict__ ele, LocalItems li,
       {
          const unsigned long long k = base + atomicAdd(&nout, 1u);
          olo[k] = hkey[i];
+         opos[k] = hpos[i];
+      }
+}
+
+// The compare-and-swap pass for two-word keys: a work-group whose elements'
+// nodes span a range r with deg x bits(r) <= 63 packs each item's nodes
+// relative to the group's lowest node into one word, deduplicates those as
+// k_item_dedup1 does, and writes the items back in the absolute (lo, hi)
+// form.  A group whose range is wider (or that holds a node out of range)
+// marks itself in defer[] and leaves its elements to the round-based pass,
+// launched next with only = defer.  With a Hilbert-ordered numbering almost
+// every group fits.
+template <int HS>
+__global__ void __launch_bounds__(kDedupThreads)
+k_item_dedup_rel(long long ne, int len, const int *__restrict__ ele, LocalItems li, KeyForm kf, long long nv,
+                 long long pos0, unsigned long long *__restrict__ olo, unsigned long long *__restrict__ ohi,
+                 int *__restrict__ opos, unsigned long long *__restrict__ ocnt, unsigned char *__restrict__ defer)
+{
+   __shared__ unsigned long long hkey[HS];
+   __shared__ int hpos[HS];
+   __shared__ int nlo, nhi;
+   __shared__ unsigned int nuniq, nout;
+   __shared__ unsigned long long base;
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads) { hkey[i] = kEmptyKey; hpos[i] = INT_MAX; }
+   if(threadIdx.x == 0) { nuniq = nout = 0; nlo = INT_MAX; nhi = INT_MIN; }
+   __syncthreads();
+   const long long e = (long long)blockIdx.x * kDedupThreads + threadIdx.x;
+   const bool act = e < ne;
+   int mn = INT_MAX, mx = INT_MIN;
+   if(act)   // the nodes the items use (a table row may carry padding)
+      for(int j = 0; j < li.n; j++)
+         for(int k = 0; k < kf.deg; k++)
+         {
+            const int v = ele[e * len + li.nod[j][k]];
+            mn = min(mn, v);
+            mx = max(mx, v);
+         }
+   for(int o = 32; o > 0; o >>= 1)
+   {
+      mn = min(mn, __shfl_xor(mn, o));
+      mx = max(mx, __shfl_xor(mx, o));
+   }
+   if((threadIdx.x & 63) == 0) { atomicMin(&nlo, mn); atomicMax(&nhi, mx); }
+   __syncthreads();
+   const int lo0 = nlo, hi0 = nhi;
+   if(lo0 > hi0) return;                          // no element in this group
+   int rb = 1;
+   while(rb < 31 && (1ll << rb) < (long long)hi0 - lo0 + 1) rb++;
+   if(lo0 < 0 || hi0 >= nv || kf.deg * rb > 63)
+   {
+      if(threadIdx.x == 0) defer[blockIdx.x] = 1;
+      return;
+   }
+   if(act)
+      for(int j = 0; j < li.n; j++)
+      {
+         int a[4] = {0, 0, 0, 0};
+         for(int k = 0; k < kf.deg; k++) a[k] = ele[e * len + li.nod[j][k]] - lo0;
+         sort_nodes(a, kf.deg);
+         unsigned long long key = 0;
+         for(int k = 0; k < kf.deg; k++) key = (key << rb) | (unsigned)a[k];
+         const int pos = (int)(pos0 + e * li.n + j);
+         unsigned int sl = dedup_slot(key, 0) & (HS - 1);
+         for(;;)
+         {
+            const unsigned long long prev = atomicCAS(&hkey[sl], kEmptyKey, key);
+            if(prev == kEmptyKey) { atomicAdd(&nuniq, 1u); break; }
+            if(prev == key) break;
+            sl = (sl + 1) & (HS - 1);
+         }
+         atomicMin(&hpos[sl], pos);
+      }
+   __syncthreads();
+   if(threadIdx.x == 0) base = atomicAdd(ocnt, (unsigned long long)nuniq);
+   __syncthreads();
+   for(int i = threadIdx.x; i < HS; i += kDedupThreads)
+      if(hkey[i] != kEmptyKey)
+      {
+         // the absolute key: nodes + lo0 packed b bits each, most significant first
+         unsigned long long r = hkey[i], l = 0, h = 0;
+         int a[4] = {0, 0, 0, 0};
+         for(int k = kf.deg - 1; k >= 0; k--) { a[k] = (int)(r & ((1ull << rb) - 1)) + lo0; r >>= rb; }
+         for(int k = 0; k < kf.deg; k++)
+         {
+            h = (h << kf.b) | (l >> (64 - kf.b));
+            l = (l << kf.b) | (unsigned)a[k];
+         }
+         const unsigned long long k = base + atomicAdd(&nout, 1u);
+         olo[k] = l;
+         ohi[k] = h;
          opos[k] = hpos[i];
       }
 }
@@ -850,12 +942,14 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    unsigned long long *cnt = nullptr, hc[2] = {0, 0};
    int *pos = nullptr, *first = nullptr, *hsel = nullptr, *usel = nullptr, *p2 = nullptr, *u2 = nullptr, *nsel = nullptr;
    int *bcnt = nullptr, *boff = nullptr, *slot = nullptr, *sel = nullptr, *ccnt = nullptr, *coff = nullptr;
-   unsigned char *head = nullptr, *flag = nullptr;
+   unsigned char *head = nullptr, *flag = nullptr, *defer = nullptr;
    void *xtmp = nullptr, *brec = nullptr;
    int rc = -1, hbad = 0, m = 0;
    // 1. block-local first appearances of every source's items
+   long long maxgrid = 1;
+   for(const Source &sr : src) maxgrid = std::max(maxgrid, (sr.ne + kDedupThreads - 1) / kDedupThreads);
    bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
-          && AllocItems(g, total, kf, &it) && DAlloc(g, &pos, total * 4);
+          && AllocItems(g, total, kf, &it) && DAlloc(g, &pos, total * 4) && DAlloc(g, &defer, maxgrid);
    tim[1] = clock();
    long long at = 0;
    for(const Source &sr : src)
@@ -865,12 +959,21 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       if(!sr.ne) continue;
       const bool small = sr.li.n <= 6;
       if(kf.two)
+      {
+         // relative one-word keys where a group's node range allows, rounds
+         // for the groups that marked themselves
+         ok = ok_(hipMemsetAsync(defer, 0, grid.x, s));
+         if(!ok) break;
+         hipLaunchKernelGGL((small ? k_item_dedup_rel<2048> : k_item_dedup_rel<4096>), grid, dim3(kDedupThreads), 0,
+                            s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, it.khi, pos, cnt, defer);
          hipLaunchKernelGGL((small ? k_item_dedup<true, 2048> : k_item_dedup<true, 4096>), grid, dim3(kDedupThreads),
-                            0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, it.khi, pos, cnt, (int *)(cnt + 1));
+                            0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, it.khi, pos, cnt, (int *)(cnt + 1),
+                            (const unsigned char *)defer);
+      }
       else if(kf.deg * kf.b >= 64)   // a full 64-bit key: every value may be a key
          hipLaunchKernelGGL((small ? k_item_dedup<false, 2048> : k_item_dedup<false, 4096>), grid,
                             dim3(kDedupThreads), 0, s, sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, nullptr, pos,
-                            cnt, (int *)(cnt + 1));
+                            cnt, (int *)(cnt + 1), nullptr);
       else
          hipLaunchKernelGGL((small ? k_item_dedup1<2048> : k_item_dedup1<4096>), grid, dim3(kDedupThreads), 0, s,
                             sr.ne, sr.len, sr.ele, sr.li, kf, nv, at, it.klo, pos, cnt, (int *)(cnt + 1));
@@ -1000,7 +1103,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    (void)hipStreamSynchronize(s);
    for(void *p : {(void *)cnt, (void *)nsel, (void *)pos, (void *)first, (void *)hsel, (void *)usel, (void *)p2,
                   (void *)u2, (void *)head, (void *)bcnt, (void *)boff, (void *)slot, (void *)sel, (void *)flag, xtmp,
-                  (void *)ccnt, (void *)coff, brec})
+                  (void *)ccnt, (void *)coff, brec, (void *)defer})
       ItemFree(g, p);
    if(rc < 0 && !hbad) fprintf(stderr, "gml: GPU extraction of %lld items failed\n", total);
    tim[4] = clock();

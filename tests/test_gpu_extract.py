@@ -315,3 +315,22 @@ def test_gpu_extract_large_bucket_takes_sorted_path(capfd):
     assert pe == "sorted" and pf == "sorted"
     assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
     assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+def test_gpu_extract_two_word_keys_against_oracle(capfd):
+    # the 40^3 cube's node ids spread 64 apart (4.4M vertex slots, 23-bit ids):
+    # triangle keys take two words; a work-group's nodes still span a narrow
+    # range, so the block pass packs them relative to its lowest node (groups
+    # whose range is too wide take the round-based pass); tables bit-identical
+    # to the oracle
+    crd, ref, tet = G.cube_mesh(40, 0.2, 21)
+    tet = np.ascontiguousarray(tet * 64)
+    nv = len(crd) * 64
+    want = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        set_vertices(g, nv)
+        set_block(g, G.GmlTetrahedra, tet, np.zeros(len(tet), np.int32))
+        nf, pf = extract_logged(g, "faces", capfd)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pf == "bucketed" and nf == len(want[0])
+    assert np.array_equal(f[0], want[0]) and np.array_equal(f[1], want[1])
