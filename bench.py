@@ -585,24 +585,51 @@ def run_small(args):
                      "min_bitexact": bool(min_exact), "mean_rel_err_vs_fp64": mean_err}
         ok &= qerr <= 1e-5 and ball_exact and min_exact and mean_err <= 1e-5
     par["ok"] = bool(ok)
-    # CPU baseline: the oracle's FP32 quality, ball minimum and reductions
+    # CPU baseline: the same generated quality and ball-minimum kernels built
+    # for the host cores (oracle/host_kernels.py), the reductions by the
+    # oracle's C loops; the oracle's own quality and ball minimum beside it
+    from oracle import host_kernels as HK
     OL = O.lib()
-    t0 = time.perf_counter()
-    k = 0
-    while k < 5 or time.perf_counter() - t0 < min(args.cpu_seconds, 2.0):
+    src = HK.quality_sources(G, os.path.join(ROOT, "gmlib_amd", "kernels"))
+    hk = HK.HostKernels(HK.build(src, tag="quality"), src)
+    hq = HK.Quality(hk, crd, tet, deg, ball)
+
+    def host_step():
+        hq.step()
+        O.reduce(G.GmlMin, hq.qal)
+        O.reduce(G.GmlSum, hq.qal)
+
+    def oracle_step():
         q32 = O.tet_quality32(tet, crd)
         O.ball_min(deg, ball, q32)
         O.reduce(G.GmlMin, q32)
         O.reduce(G.GmlSum, q32)
-        k += 1
-    ct = (time.perf_counter() - t0) / k
+
+    def per_step(fn):
+        fn()
+        t0, k = time.perf_counter(), 0
+        while k < 5 or time.perf_counter() - t0 < min(args.cpu_seconds, 2.0):
+            fn()
+            k += 1
+        return (time.perf_counter() - t0) / k, k
+
+    ct, k = per_step(host_step)
+    q32 = O.tet_quality32(tet, crd)
+    host_equal = bool(np.array_equal(hq.qal, q32) and np.array_equal(hq.vq, O.ball_min(deg, ball, q32)))
+    ot, ko = per_step(oracle_step)
     out = {"workload": "tet quality + ball min + GmlReduceVector min / mean, 20^3-cell Kuhn cube (%d vertices, "
                        "%d tets)" % (V, T),
            "vertices": V, "tets": T, "bytes": {"quality": qb, "ball_min": bb, "reduce": rb},
            "paths": res, "parity": par,
            "cpu_baseline": {"value": round((qb + bb + 2 * rb) / ct / 1e9, 2), "unit": "GB/s", "step_us": round(ct * 1e6, 1),
-                            "cores": OL.orc_num_threads(), "kind": "port",
-                            "sample": "%d oracle steps (FP32 quality, ball min, min and sum), C + OpenMP" % k}}
+                            "cores": hk.threads, "kind": "port", "equal_to_oracle": host_equal,
+                            "sample": "%d steps: the generated tet_meanratio and ball_min kernels (project OpenCL-C "
+                                      "bodies with the engine's GMlib prologue/epilogue) compiled for x86-64 by clang "
+                                      "and run with OpenMP, min and sum by the oracle's C loops" % k,
+                            "oracle_c": {"value": round((qb + bb + 2 * rb) / ot / 1e9, 2), "step_us": round(ot * 1e6, 1),
+                                         "cores": OL.orc_num_threads(),
+                                         "sample": "%d oracle steps (FP32 quality, ball min, min and sum), "
+                                                   "C + OpenMP" % ko}}}
     log("config 1 (20^3): %s" % out)
     return out
 
@@ -1165,33 +1192,39 @@ def run_batch(args):
     }
     if args.cpu_seconds > 0:
         from tests import oracle_lib as O
-        OL = O.lib()
-        # vertex balls are set-up work on both sides (the GPU builds them at
-        # batch creation): built before the timed region
+        # the generated quality and ball-minimum kernels built for the host
+        # cores (oracle/host_kernels.py), the reductions by the oracle's C
+        # loops.  Vertex balls are set-up work on both sides (the GPU builds
+        # them at batch creation): built before the timed region
+        from oracle import host_kernels as HK
+        src = HK.quality_sources(G, os.path.join(ROOT, "gmlib_amd", "kernels"))
+        hk = HK.HostKernels(HK.build(src, tag="quality"), src)
         tb = time.perf_counter()
-        balls = []
+        hqs = []
         for i in range(M):
-            balls.append(O.vertex_ball(ms[i][2], len(ms[i][0]), 32)[:2])
+            deg, ball, md = O.vertex_ball(ms[i][2], len(ms[i][0]), 32)
+            if md > 32:
+                break
+            hqs.append(HK.Quality(hk, ms[i][0], ms[i][2], deg, ball))
             if time.perf_counter() - tb > args.cpu_seconds:
                 break
         t0 = time.perf_counter()
         done_t = done_v = 0
         i = 0
-        while i < len(balls) and (i < 8 or time.perf_counter() - t0 < args.cpu_seconds):
-            crd_m, ref_m, tet_m = ms[i]
-            qm = O.tet_quality32(tet_m, crd_m)
-            deg, ball = balls[i]
-            O.ball_min(deg, ball, qm)
-            O.reduce(G.GmlMin, qm)
-            O.reduce(G.GmlSum, qm)
-            done_t += len(tet_m)
-            done_v += len(crd_m)
+        while i < len(hqs) and (i < 8 or time.perf_counter() - t0 < args.cpu_seconds):
+            hq = hqs[i]
+            hq.step()
+            O.reduce(G.GmlMin, hq.qal)
+            O.reduce(G.GmlSum, hq.qal)
+            done_t += hq.T
+            done_v += hq.V
             i += 1
         ct = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round((36 * done_t + 20 * done_v) / ct / 1e9, 2), "unit": "GB/s",
-                               "cores": OL.orc_num_threads(), "kind": "port",
-                               "sample": "the first %d meshes of the batch (%d tets): oracle FP32 quality, ball min "
-                                         "(balls built beforehand), min and sum reductions, C + OpenMP, %.2f s"
+                               "cores": hk.threads, "kind": "port",
+                               "sample": "the first %d meshes of the batch (%d tets): the generated tet_meanratio and "
+                                         "ball_min kernels compiled for x86-64 by clang and run with OpenMP (balls "
+                                         "built beforehand), min and sum by the oracle's C loops, %.2f s"
                                          % (i, done_t, ct)}
         log("cpu baseline: %s" % out["cpu_baseline"])
     g.close()

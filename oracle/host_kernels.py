@@ -201,8 +201,26 @@ def smoothing_sources(G, kernel_dir, omega_block=16, n=4):
                 "smooth_gather": g.L.GmlxGetKernelSource(g.h, kg).decode()}
 
 
-def ball_width(src):
-    m = re.search(r"#define\s+BaryDegMax\s+(\d+)", src)
+def quality_sources(G, kernel_dir, n=4):
+    """The generated texts of the 20^3 quality step's kernels (the tet
+    mean-ratio body over the tet -> vertex downlink, the ball minimum body
+    over the vertex -> tet ball), written as smoothing_sources does."""
+    crd, ref, tet = G.cube_mesh(n, 0.2, 3, True)
+    body = lambda k: open(os.path.join(kernel_dir, k + ".cl")).read()
+    with G.Gml(host_only=True) as g:
+        G.lib().GmlxSetStaging(g.h, 0)
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+        kq = g.compile(body("tet_meanratio"), "tet_meanratio", G.GmlTetrahedra,
+                       [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        kb = g.compile(body("ball_min"), "ball_min", G.GmlVertices, [(q, G.GmlReadMode, 0), (vq, G.GmlWriteMode, 0)])
+        return {"tet_meanratio": g.L.GmlxGetKernelSource(g.h, kq).decode(),
+                "ball_min": g.L.GmlxGetKernelSource(g.h, kb).decode()}
+
+
+def ball_width(src, name="Bary"):
+    m = re.search(r"#define\s+%sDegMax\s+(\d+)" % name, src)
     return int(m.group(1)) if m else None
 
 
@@ -222,7 +240,6 @@ class Smoothing:
         self.bary = np.zeros((self.T, 4), np.float32)
         self.par = np.zeros(4, np.float32)
         self.par[0] = omega
-
         self.arrays = {"TetVerTab": self.tet, "VerCrdTab": self.crd, "BaryTab": self.bary, "GmlPar": self.par,
                        "VerTetBalTab": self.tab, "VerTetBalOff": self.off, "VerTetDegTab": self.deg,
                        "VerRefTab": self.ref}
@@ -230,3 +247,27 @@ class Smoothing:
     def sweep(self):
         self.hk.run("smooth_scatter", self.arrays, self.T)
         self.hk.run("smooth_gather", self.arrays, self.V)
+
+
+class Quality:
+    """The generated quality and ball-minimum kernels on the host over one
+    mesh: crd (V x 4 float32), tet, the dense ball (deg, V x w packed
+    entries).  step() fills qal (per tet) and vq (per vertex)."""
+
+    def __init__(self, hk, crd, tet, deg, ball):
+        self.hk = hk
+        self.V, self.T = len(crd), len(tet)
+        self.crd = np.ascontiguousarray(crd, np.float32)
+        self.tet = np.ascontiguousarray(tet, np.int32)
+        self.deg = np.ascontiguousarray(deg, np.int32)
+        self.tab, self.off = sell64(deg, ball)
+        self.qal = np.zeros(self.T, np.float32)
+        self.vq = np.zeros(self.V, np.float32)
+        self.par = np.zeros(4, np.float32)
+        self.arrays = {"TetVerTab": self.tet, "VerCrdTab": self.crd, "qalTab": self.qal, "vqTab": self.vq,
+                       "GmlPar": self.par, "VerTetBalTab": self.tab, "VerTetBalOff": self.off,
+                       "VerTetDegTab": self.deg}
+
+    def step(self):
+        self.hk.run("tet_meanratio", self.arrays, self.T)
+        self.hk.run("ball_min", self.arrays, self.V)

@@ -73,3 +73,16 @@ def test_staged_source_is_refused():
         s = g.L.GmlxGetKernelSource(g.h, k).decode()
     with pytest.raises(ValueError):
         HK.wrapper(s)
+
+
+def test_host_generated_quality_and_ball_min_equal_oracle(tmp_path):
+    src = HK.quality_sources(G, KDIR)
+    hk = HK.HostKernels(HK.build(src, out=str(tmp_path), tag="quality"), src)
+    assert HK.ball_width(src["ball_min"], "qal") == 32
+    crd, ref, tet = G.cube_mesh(12, 0.3, 5)
+    deg, ball, _ = O.vertex_ball(tet, len(crd), 32)
+    hq = HK.Quality(hk, crd, tet, deg, ball)
+    hq.step()
+    want = O.tet_quality32(tet, crd)
+    assert np.array_equal(hq.qal, want)
+    assert np.array_equal(hq.vq, O.ball_min(deg, ball, want))
