@@ -276,22 +276,18 @@ void FreeItemPool(GmlCtx *g, bool idle_only)
 
 namespace {
 
-// Keys of every item of the sources (in order) and their stable sort.
-// bad: device flag raised by a node out of range.  The temporary storage
-// also fits a select and a 32-bit key sort over n items.
-// Arrays of an item sort of n items; the temporary storage also fits a
-// select, a 32-bit key sort and a 32-bit pair sort over n items.
-bool AllocItems(GmlCtx *g, long long n, const KeyForm &kf, ItemSort *it)
+// the sort buffers of an item sort whose key arrays exist (slo, shi, ord,
+// idx and the temporary storage)
+bool AllocSortBuffers(GmlCtx *g, ItemSort *it)
 {
    hipStream_t s = g->stream;
-   it->n = n;
-   it->kf = kf;
+   const long long n = it->n;
+   const KeyForm &kf = it->kf;
    if(!n) return true;
    size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
    int *cnt = nullptr;
    unsigned char *flags = nullptr;
-   bool ok = DAlloc(g, &it->klo, n * 8) && DAlloc(g, &it->slo, n * 8)
-          && (!kf.two || (DAlloc(g, &it->khi, n * 8) && DAlloc(g, &it->shi, n * 8)))
+   bool ok = DAlloc(g, &it->slo, n * 8) && (!kf.two || DAlloc(g, &it->shi, n * 8))
           && DAlloc(g, &it->ord, n * 4) && DAlloc(g, &it->idx, n * 4)
           && ok_(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, it->klo, it->slo, it->idx, it->ord, (int)n, 0, 64, s))
           && ok_(hipcub::DeviceSelect::Flagged(nullptr, t2, it->ord, flags, it->idx, cnt, (int64_t)n, s))
@@ -303,6 +299,21 @@ bool AllocItems(GmlCtx *g, long long n, const KeyForm &kf, ItemSort *it)
       ok = DAlloc(g, &it->tmp, it->tbytes);
    }
    return ok;
+}
+
+// the key arrays of n items (klo, and khi in the two-word form)
+bool AllocKeys(GmlCtx *g, long long n, const KeyForm &kf, ItemSort *it)
+{
+   it->n = n;
+   it->kf = kf;
+   return !n || (DAlloc(g, &it->klo, n * 8) && (!kf.two || DAlloc(g, &it->khi, n * 8)));
+}
+
+// Arrays of an item sort of n items; the temporary storage also fits a
+// select, a 32-bit key sort and a 32-bit pair sort over n items.
+bool AllocItems(GmlCtx *g, long long n, const KeyForm &kf, ItemSort *it)
+{
+   return AllocKeys(g, n, kf, it) && AllocSortBuffers(g, it);
 }
 
 // stable sort of the it->n keys already in klo / khi
@@ -949,7 +960,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    long long maxgrid = 1;
    for(const Source &sr : src) maxgrid = std::max(maxgrid, (sr.ne + kDedupThreads - 1) / kDedupThreads);
    bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
-          && AllocItems(g, total, kf, &it) && DAlloc(g, &pos, total * 4) && DAlloc(g, &defer, maxgrid);
+          && AllocKeys(g, total, kf, &it) && DAlloc(g, &pos, total * 4) && DAlloc(g, &defer, maxgrid);
    tim[1] = clock();
    long long at = 0;
    for(const Source &sr : src)
@@ -1057,7 +1068,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    else if(ok)
    {
       it.n = n2;
-      ok = SortPrepared(s, &it) && DAlloc(g, &head, n2) && DAlloc(g, &first, n2 * 4) && DAlloc(g, &hsel, n2 * 4)
+      ok = AllocSortBuffers(g, &it) && SortPrepared(s, &it) && DAlloc(g, &head, n2) && DAlloc(g, &first, n2 * 4) && DAlloc(g, &hsel, n2 * 4)
         && DAlloc(g, &usel, n2 * 4) && DAlloc(g, &p2, n2 * 4) && DAlloc(g, &u2, n2 * 4);
       if(ok)
       {
