@@ -2,7 +2,7 @@
  * GmlExportSolution (sources/gmlib3.c:361-373, 4456-4740), for callers that
  * do not include libMeshb's own header (libmeshb8.h), whose values these
  * are.  Dimension, Vertices, Edges, Triangles and End are also the codes
- * found in the reference's sample meshes (sample_meshes/*.meshb).
+ * found in the reference's sample meshes (the .meshb files of sample_meshes).
  * Include this header or libmeshb8.h, not both. */
 
 #ifndef GML_GMF_H
