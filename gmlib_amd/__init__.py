@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgml.so")
+# the library to load: libgml.so beside this file, or GMLIB_AMD_LIB (a checking build)
+LIB_PATH = os.environ.get("GMLIB_AMD_LIB") or os.path.join(HERE, "libgml.so")
 
 # enums and flags (sources/gmlib3.h:46-70)
 GmlMaxDat, GmlMaxMat, GmlMaxVec, GmlMaxKrn = 100, 10, 10, 100

@@ -1,6 +1,11 @@
 """Build recipe of libgml.so (HIP for gfx950) and the CPU oracle.
 
 Usage: python -m gmlib_amd.build   (or __graft_entry__.build())
+       python -m gmlib_amd.build --asan   (host-AddressSanitizer build in
+       gmlib_amd/_build_asan/, for the CPU tests:
+       GMLIB_AMD_LIB=gmlib_amd/_build_asan/libgml.so
+       LD_PRELOAD=<clang>/lib/linux/libclang_rt.asan-x86_64.so
+       ASAN_OPTIONS=detect_leaks=0 python -m pytest tests -m "not gpu")
 Every HIP translation unit is compiled with hipcc for gfx950 only, with
 packed-FP32 instructions disabled, then linked with hipRTC into one shared
 library next to this file.
@@ -52,14 +57,23 @@ def _run(cmd):
     return out
 
 
-def build_lib(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+# host code with AddressSanitizer (device code unchanged): a checking build
+# for the CPU tests, run with GMLIB_AMD_LIB pointing at it and the
+# sanitizer's runtime preloaded
+ASAN = ["-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_LIB = os.path.join(HERE, "_build_asan", "libgml.so")
+
+
+def build_lib(verbose=False, asan=False):
+    bdir = os.path.dirname(ASAN_LIB) if asan else BUILD
+    lib = ASAN_LIB if asan else LIB
+    os.makedirs(bdir, exist_ok=True)
     objs = []
     procs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(BUILD, src + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", path, "-o", obj]
+        obj = os.path.join(bdir, src + ".o")
+        cmd = [HIPCC] + COMMON + DEVICE + (ASAN if asan else []) + ["-x", "hip", "-c", path, "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     failed = []
@@ -74,11 +88,11 @@ def build_lib(verbose=False):
         for cmd, out in failed:
             sys.stderr.write(" ".join(cmd) + "\n" + out + "\n")
         raise RuntimeError("HIP compilation failed")
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+    tmp = lib + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH] + (ASAN[1:3] if asan else []) + ["-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-lhiprtc", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_oracle():
@@ -86,6 +100,9 @@ def build_oracle():
 
 
 def main():
+    if "--asan" in sys.argv:
+        print(build_lib(verbose=True, asan=True))
+        return
     build_oracle()
     print(build_lib(verbose=True))
 
