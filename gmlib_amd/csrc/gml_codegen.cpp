@@ -413,6 +413,13 @@ static bool BodyWritesParams(const char *body)
 static const char *kRowIds[] = {"barrier", "get_global_id", "get_local_id", "get_group_id", "get_local_size",
                                 "get_global_size", "get_num_groups", "__syncthreads", "gml_gid"};
 
+static bool UsesRowIds(const char *KrnSrc)
+{
+   for(const char *id : kRowIds)
+      if(BodyUses(KrnSrc, id)) return true;
+   return false;
+}
+
 static int TileStagedArgs(const GmlCtx *g, const std::vector<Arg> &args, int MshTyp, const char *KrnSrc,
                           std::vector<int> *staged)
 {
@@ -504,9 +511,11 @@ static std::string EmitKernel(GmlCtx *g, const char *KrnSrc, const char *PrcNam,
            "};\n";
    if(g->UsrTlk)
    {
-      s += "// USER'S TOOLKIT\n";
+      // OpenCL-C functions carry no qualifier: compiled for host and device
+      // alike, so the kernel can call them
+      s += "// USER'S TOOLKIT\n#pragma clang force_cuda_host_device begin\n";
       s += UserCode(g->UsrTlk);
-      s += "\n";
+      s += "\n#pragma clang force_cuda_host_device end\n";
    }
    s += "// USER'S ARGUMENTS STRUCTURE\n";
    if(ParSrc)
@@ -949,6 +958,13 @@ static int NewKernel(GmlCtx *g, const std::string &src, const char *PrcNam, std:
 // are warm.
 void SetShapes(Kernel *k)
 {
+   if(k->rowids)
+   {
+      // OpenCL's work-item ids name the row: no XCD remap, one row per item
+      k->shapes = {{256, 0, 0}, {128, 0, 0}, {64, 0, 0}};
+      k->remap = 0;
+      return;
+   }
    k->shapes = {{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {256, 0, 0}, {128, 0, 0}, {64, 0, 0}};
    if(k->fn2 || !k->src2.empty())
       for(std::array<int, 3> sh : {std::array<int, 3>{256, 1, 1}, {128, 1, 1}, {256, 0, 1}, {128, 0, 1}})
@@ -1416,6 +1432,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
          DbgPrint(g, "Kernel %s: the element-tile form does not compile, the other forms are kept\n%s\n", PrcNam,
                   tlog.c_str());
    }
+   k->rowids = UsesRowIds(KrnSrc);
    SetShapes(k);
    for(auto &a : args)
    {
@@ -1458,6 +1475,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
       return 0;
    }
    Kernel *h = &g->krn[HghIdx];
+   h->rowids = g->krn[KrnIdx].rowids;
    SetShapes(h);
    h->args = g->krn[KrnIdx].args;
    h->wargs = g->krn[KrnIdx].wargs;
@@ -1577,6 +1595,7 @@ int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
    if(Rows == 2 && !k->fn2 && (g->host_only ? k->src2.empty() : true)) return 0;
    if(Rows == -1 && !k->fn3 && (g->host_only ? k->src3.empty() : true)) return 0;
    if(k->stage_src >= 0 && GrpSiz != 64 && GrpSiz != 128 && GrpSiz != 256) return 0;
+   if(Remap && k->rowids) return 0;          // work-item ids must name the row
    for(int i : {idx, k->HghIdx})
    {
       if(!i) continue;

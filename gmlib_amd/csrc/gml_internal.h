@@ -101,6 +101,9 @@ struct Kernel {
    // element -> vertex data, kDownTile rows per work-group)
    int           grp = 256, remap = 1, tuned = 0, tune_step = 0, form = 0;
    std::vector<std::array<int, 3>> shapes;   // the tuner's candidates {group, remap, form}
+   // the body names a work-item builtin (get_global_id ...): its launches
+   // keep the plain block order, so get_global_id(0) is the row
+   bool          rowids = false;
    // paired form (two rows per thread), when the kernel has one
    hipModule_t   mod2 = nullptr;
    hipFunction_t fn2 = nullptr;

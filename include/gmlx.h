@@ -88,7 +88,9 @@ int    GmlxGetTilePlanInfo(size_t GmlIdx, int typ, long long *info);
    or 256 (generated kernels are compiled for at most 256 work-items), XCD
    remap, rows per thread (2 needs the paired form; -1 selects the
    element-tile form).  Returns 1, or 0 if the shape is not available for
-   the kernel. */
+   the kernel.  A kernel whose body calls a work-item function
+   (get_global_id ...) runs in plain block order, one row per work-item, so
+   that get_global_id(0) is its row as in OpenCL: no remap, no other form. */
 int    GmlxSetKernelShape(size_t GmlIdx, int KrnIdx, int GrpSiz, int Remap, int Rows);
 
 /* --- device residency ---------------------------------------------------- */

@@ -39,6 +39,36 @@ static inline double rsqrt(double x) { return 1. / sqrt(x); }
 static inline float __int_as_float(int x) { float f; memcpy(&f, &x, 4); return f; }
 static inline int __float_as_int(float f) { int x; memcpy(&x, &f, 4); return x; }
 
+// HIP's atomics on host memory (relaxed, as the device's)
+template <class T> static inline T atomicAdd(T *p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+template <class T> static inline T atomicSub(T *p, T v) { return __atomic_fetch_sub(p, v, __ATOMIC_RELAXED); }
+template <class T> static inline T atomicAnd(T *p, T v) { return __atomic_fetch_and(p, v, __ATOMIC_RELAXED); }
+template <class T> static inline T atomicOr(T *p, T v) { return __atomic_fetch_or(p, v, __ATOMIC_RELAXED); }
+template <class T> static inline T atomicXor(T *p, T v) { return __atomic_fetch_xor(p, v, __ATOMIC_RELAXED); }
+template <class T> static inline T atomicExch(T *p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_RELAXED); }
+static inline float atomicExch(float *p, float v)
+{
+   int o = __atomic_exchange_n((int *)p, __float_as_int(v), __ATOMIC_RELAXED);
+   return __int_as_float(o);
+}
+template <class T> static inline T atomicCAS(T *p, T c, T v)
+{
+   __atomic_compare_exchange_n(p, &c, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+   return c;
+}
+template <class T> static inline T atomicMin(T *p, T v)
+{
+   T o = __atomic_load_n(p, __ATOMIC_RELAXED);
+   while(v < o && !__atomic_compare_exchange_n(p, &o, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+   return o;
+}
+template <class T> static inline T atomicMax(T *p, T v)
+{
+   T o = __atomic_load_n(p, __ATOMIC_RELAXED);
+   while(v > o && !__atomic_compare_exchange_n(p, &o, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+   return o;
+}
+
 // HIP's device overloads of min / max for scalars
 static inline int min(int a, int b) { return a < b ? a : b; }
 static inline int max(int a, int b) { return a > b ? a : b; }

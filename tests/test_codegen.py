@@ -373,3 +373,23 @@ def test_opencl_vector_literals_rewritten(mesh):
         tri = (C.c_int * 6)(vi, G.GmlReadMode, 0, q, G.GmlWriteMode, 0)
         assert g.L.GmlxCompileKernelv(g.h, b"{ float4 a = (float4)(1.f, 2.f, 3.f); qal = a.x; }", b"bad",
                                       G.GmlTetrahedra, 2, tri) == 0
+
+
+def test_atomics_toolkit_and_work_item_bodies_compile(mesh):
+    # OpenCL atomics on GmlManual data, a user toolkit of plain OpenCL-C
+    # functions (compiled for host and device), and a body naming work-item
+    # functions, whose launches keep the plain block order
+    crd, ref, tet = mesh
+    with G.Gml(host_only=True) as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        c = g.solution(G.GmlVertices, 1, G.GmlInt, "cnt")
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        k = g.compile("{ for(int j = 0; j < 4; j++) { atomic_inc(&cntTab[TetVer[j]]); atom_max(&cntTab[0], j); } }",
+                      "atomics", G.GmlTetrahedra, [(c, G.GmlReadMode | G.GmlWriteMode | G.GmlManual, 0)])
+        assert not g.L.GmlxGetKernelPairSource(g.h, k) and not g.L.GmlxGetKernelTileSource(g.h, k)
+        g.L.GmlIncludeUserToolkit(g.h, C.c_char_p(b"float twice(float x) { return 2.f * x; }\n"))
+        k2 = g.compile("{ qal = twice(VerCrd[0].x) + (float)get_global_id(0); }", "tk_ids", G.GmlTetrahedra,
+                       [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        assert "#pragma clang force_cuda_host_device begin" in src_of(g, k2)
+        assert not g.L.GmlxSetKernelShape(g.h, k2, 128, 1, 1)
+        assert g.L.GmlxSetKernelShape(g.h, k2, 128, 0, 1)

@@ -1,0 +1,97 @@
+"""GPU parity of API features a user body relies on beyond gathers
+(sources/gmlib3.c): GmlManual data (the body indexes the data table itself,
+:2683, 2771, 2917) with OpenCL atomic functions, a user toolkit
+(GmlIncludeUserToolkit, :4445-4453) whose OpenCL-C functions the body calls,
+compiler options (GmlSetCompilerOptions, :2993), and the work-item
+functions (get_global_id ...), which name the row as in the reference's
+generated counter.  Expected values come from numpy on the same inputs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import gmlib_amd as G
+
+pytestmark = pytest.mark.gpu
+
+
+def test_manual_data_with_atomics():
+    # per vertex: the number of tets holding it (atomic_inc), the largest and
+    # smallest of their indices (atomic_max / atomic_min) and the sum of
+    # their local node numbers (atomic_add), through the tets' own table
+    crd, ref, tet = G.cube_mesh(12, 0.2, 4)
+    V, T = len(crd), len(tet)
+    body = """{
+       for(int j = 0; j < 4; j++)
+       {
+          atomic_inc(&cntTab[TetVer[j]]);
+          atomic_max(&hiTab[TetVer[j]], TetIdx);
+          atomic_min(&loTab[TetVer[j]], TetIdx);
+          atomic_add(&locTab[TetVer[j]], j);
+       }
+    }"""
+    RWM = G.GmlReadMode | G.GmlWriteMode | G.GmlManual
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        d = {n: g.solution(G.GmlVertices, 1, G.GmlInt, n) for n in ("cnt", "hi", "lo", "loc")}
+        g.upload(d["cnt"], np.zeros(V, np.int32))
+        g.upload(d["hi"], np.full(V, -1, np.int32))
+        g.upload(d["lo"], np.full(V, T, np.int32))
+        g.upload(d["loc"], np.zeros(V, np.int32))
+        k = g.compile(body, "vertex_counts", G.GmlTetrahedra, [(d[n], RWM, 0) for n in ("cnt", "hi", "lo", "loc")])
+        g.launch(k)
+        got = {n: g.download_raw(d[n], np.int32, (V,)) for n in d}
+        assert g.L.GmlxGetKernelForm(g.h, k) == 1          # atomics: one row per work-item only
+    flat, idx = tet.ravel(), np.repeat(np.arange(T, dtype=np.int32), 4)
+    hi = np.full(V, -1, np.int32)
+    lo = np.full(V, T, np.int32)
+    np.maximum.at(hi, flat, idx)
+    np.minimum.at(lo, flat, idx)
+    loc = np.zeros(V, np.int64)
+    np.add.at(loc, flat, np.tile(np.arange(4), T))
+    assert np.array_equal(got["cnt"], np.bincount(flat, minlength=V))
+    assert np.array_equal(got["hi"], hi) and np.array_equal(got["lo"], lo)
+    assert np.array_equal(got["loc"], loc)
+
+
+def test_user_toolkit_and_compiler_options():
+    crd, ref, tet = G.cube_mesh(8, 0.2, 6)
+    T = len(tet)
+    toolkit = C.c_char_p(b"float twice(float x) { return 2.f * x; }\n"
+                         b"float4 mid(float4 a, float4 b) { return (a + b) * .5f; }\n")
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        g.L.GmlIncludeUserToolkit(g.h, toolkit)
+        g.L.GmlSetCompilerOptions(g.h, C.c_char_p(b"-DGMLX_SCALE=3 -cl-mad-enable"))
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        k = g.compile("{ qal = GMLX_SCALE * twice(mid(VerCrd[0], VerCrd[1]).x); }", "toolkit_body",
+                      G.GmlTetrahedra, [(vi, G.GmlReadMode, 0), (q, G.GmlWriteMode, 0)])
+        g.launch(k)
+        got = g.download_raw(q, np.float32, (T,))
+    c = crd.astype(np.float32)
+    m = (c[tet[:, 0], 0] + c[tet[:, 1], 0]) * np.float32(0.5)
+    want = np.float32(3) * (np.float32(2) * m)
+    assert np.array_equal(got, want)
+
+
+def test_work_item_functions_name_the_row():
+    crd, ref, tet = G.cube_mesh(10, 0.2, 8)
+    T = len(tet)
+    body = """{
+       qal = (float)get_global_id(0);
+       grp = (get_group_id(0) * get_local_size(0) + get_local_id(0) == get_global_id(0)
+              && get_global_size(0) == get_num_groups(0) * get_local_size(0)) ? 1 : 0;
+    }"""
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        ok = g.solution(G.GmlTetrahedra, 1, G.GmlInt, "grp")
+        k = g.compile(body, "row_ids", G.GmlTetrahedra, [(q, G.GmlWriteMode, 0), (ok, G.GmlWriteMode, 0)])
+        assert not g.L.GmlxSetKernelShape(g.h, k, 128, 1, 1)     # no remap for such a body
+        for _ in range(6):                                        # through the tuner's shapes
+            g.launch(k)
+            assert np.array_equal(g.download_raw(q, np.float32, (T,)), np.arange(T, dtype=np.float32))
+            assert g.download_raw(ok, np.int32, (T,)).all()
+        a, b = C.c_int(), C.c_int()
+        g.L.GmlxGetKernelShape(g.h, k, C.byref(a), C.byref(b))
+        assert a.value in (64, 128, 256) and b.value == 0
