@@ -95,3 +95,67 @@ def test_work_item_functions_name_the_row():
         a, b = C.c_int(), C.c_int()
         g.L.GmlxGetKernelShape(g.h, k, C.byref(a), C.byref(b))
         assert a.value in (64, 128, 256) and b.value == 0
+
+
+def test_transverse_reads_through_neighbours_and_user_links():
+    # a tet field read through the tet -> tet neighbour link (GmlSetNeighbours)
+    # and through a link the user fills (GmlNewLinkData); entry 0 means none
+    # and reads the null value (sources/gmlib3.c:2788-2790, 2838-2842)
+    crd, ref, tet = G.cube_mesh(6, 0.2, 9)
+    T = len(tet)
+    rng = np.random.default_rng(11)
+    val0 = rng.uniform(1.0, 2.0, T).astype(np.float32)
+    pair = ((np.arange(T) * 7919) % T).astype(np.int32)
+    pair[::5] = 0
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        ngb = g.L.GmlSetNeighbours(g.h, G.GmlTetrahedra)
+        assert ngb
+        v = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "val")
+        g.upload(v, val0)
+        s = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "nsum")
+        k = g.compile("{ nsum = val[0] + val[1] + val[2] + val[3]; }", "neighbour_sum", G.GmlTetrahedra,
+                      [(v, G.GmlReadMode, ngb), (s, G.GmlWriteMode, 0)])
+        g.launch(k)
+        nsum = g.download_raw(s, np.float32, (T,))
+        table = g.download_raw(ngb, np.int32, (T, 4))
+        lnk = g.L.GmlNewLinkData(g.h, G.GmlTetrahedra, G.GmlTetrahedra, 1, g.cstr("pair"))
+        assert lnk
+        g.upload(lnk, pair)
+        o = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "other")
+        k2 = g.compile("{ other = val; }", "pair_read", G.GmlTetrahedra, [(v, G.GmlReadMode, lnk), (o, G.GmlWriteMode, 0)])
+        g.launch(k2)
+        other = g.download_raw(o, np.float32, (T,))
+    want = np.zeros(T, np.float32)
+    for c in range(4):
+        want = want + np.where(table[:, c] != 0, val0[table[:, c]], np.float32(0))
+    assert (table != 0).any() and (table == 0).any()
+    assert np.array_equal(nsum, want)
+    assert np.array_equal(other, np.where(pair != 0, val0[pair], np.float32(0)))
+
+
+def test_parameters_round_trip_and_counters():
+    # a body reads a parameter and one row writes another; the host reads it
+    # back with GmlDownloadParameters; the run-time and byte counters move
+    crd, ref, tet = G.cube_mesh(6, 0.2, 10)
+    T = len(tet)
+    src = "typedef struct { float scale; int seen; int pad[2]; } GmlParSct;"
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        par = C.cast(g.new_parameters(16, src), C.POINTER(C.c_float))
+        par[0] = 2.5
+        C.cast(par, C.POINTER(C.c_int))[1] = 0
+        assert g.L.GmlUploadParameters(g.h) == 1
+        q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+        k = g.compile("{ qal = GmlPar->scale * (float)TetIdx; if(TetIdx == 3) GmlPar->seen = 42; }", "par_rw",
+                      G.GmlTetrahedra, [(q, G.GmlWriteMode, 0)])
+        t0 = g.L.GmlGetWallClock()
+        g.launch(k)
+        got = g.download_raw(q, np.float32, (T,))
+        assert g.L.GmlDownloadParameters(g.h) == 1
+        assert C.cast(par, C.POINTER(C.c_int))[1] == 42 and par[0] == 2.5
+        assert g.reduce(q, G.GmlMax) == float(np.float32(2.5) * np.float32(T - 1))
+        assert g.L.GmlGetKernelRunTime(g.h, k) > 0 and g.L.GmlGetReduceRunTime(g.h, G.GmlMax) > 0
+        assert g.L.GmlGetWallClock() >= t0
+        assert g.L.GmlGetMemoryUsage(g.h) > 0 and g.L.GmlGetMemoryTransfer(g.h) > 0
+    assert np.array_equal(got, np.float32(2.5) * np.arange(T, dtype=np.float32))
