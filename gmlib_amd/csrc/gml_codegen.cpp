@@ -338,6 +338,7 @@ static std::string ArgDecl(const Arg &a)
 static int StagedArgs(const std::vector<Arg> &args, std::vector<int> *staged)
 {
    int ball = -1;
+   size_t bytes = 0;
    staged->clear();
    for(size_t i = 0; i < args.size(); i++)
    {
@@ -347,8 +348,15 @@ static int StagedArgs(const std::vector<Arg> &args, std::vector<int> *staged)
       // a short ball (a face's two cells) gains nothing from the tile's
       // shared loads: its rows are gathered directly
       if(args[a.LnkIdx].MaxDeg <= 4) continue;
-      if(ball < 0) ball = a.LnkIdx;
-      if(a.LnkIdx == ball) staged->push_back((int)i);
+      if(ball >= 0 && a.LnkIdx != ball) continue;
+      // the smallest tile (64 rows) must fit the LDS whatever its sharing:
+      // at most 64 x the ball width entities, each item 16-byte aligned.
+      // Data that would not fit are gathered directly.
+      const size_t z = ((size_t)OclTypSiz[a.ItmTyp] + 15) & ~(size_t)15;
+      if((bytes + z) * 64 * args[a.LnkIdx].MaxDeg > 64 * 1024) continue;
+      bytes += z;
+      ball = a.LnkIdx;
+      staged->push_back((int)i);
    }
    return ball;
 }

@@ -159,3 +159,36 @@ def test_parameters_round_trip_and_counters():
         assert g.L.GmlGetWallClock() >= t0
         assert g.L.GmlGetMemoryUsage(g.h) > 0 and g.L.GmlGetMemoryTransfer(g.h) > 0
     assert np.array_equal(got, np.float32(2.5) * np.arange(T, dtype=np.float32))
+
+
+@pytest.mark.parametrize("typ,dtype,width", [(G.GmlByt, np.int8, 1), (G.GmlByt4, np.int8, 4), (G.GmlInt2, np.int32, 2),
+                                             (G.GmlFlt8, np.float32, 8), (G.GmlDbl4, np.float64, 4),
+                                             (G.GmlDbl16, np.float64, 16)])
+def test_uplink_reads_of_every_width_staged_and_direct(typ, dtype, width):
+    # tet data of 1 to 128 bytes read through the vertex -> tet ball, with the
+    # LDS staging on (several items per thread and pass, by item size) and
+    # off: every vertex gets its last ball entry's value (the smallest tet of
+    # its ball), compared exactly
+    crd, ref, tet = G.cube_mesh(9, 0.2, 12)
+    V, T = len(crd), len(tet)
+    rng = np.random.default_rng(width)
+    if dtype == np.int8:
+        x0 = rng.integers(-128, 128, size=(T, width)).astype(np.int8)
+    else:
+        x0 = rng.normal(size=(T, width)).astype(dtype)
+    body = "{ y = x[0]; for(int i = 1; i < xDeg; i++) y = x[i]; }"
+    outs = []
+    for staged in (1, 0):
+        with G.Gml() as g:
+            G.lib().GmlxSetStaging(g.h, staged)
+            vi, ti = g.set_mesh(crd, ref, tet)
+            x = g.solution(G.GmlTetrahedra, 1, typ, "x")
+            y = g.solution(G.GmlVertices, 1, typ, "y")
+            g.upload(x, x0)
+            k = g.compile(body, "last_of_ball", G.GmlVertices, [(x, G.GmlReadMode, 0), (y, G.GmlWriteMode, 0)])
+            for _ in range(4):
+                g.launch(k)
+            outs.append(g.download_raw(y, dtype, (V, width)))
+            deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, V, 32)
+    want = x0[ball[np.arange(V), deg - 1] >> 4]
+    assert np.array_equal(outs[0], want) and np.array_equal(outs[1], want)
