@@ -443,14 +443,17 @@ static int TileStagedArgs(const GmlCtx *g, const std::vector<Arg> &args, int Msh
    if(L < 0 || BodyUses(KrnSrc, args[L].nam)) return -1;
    const int w = args[L].NmbItm * args[L].ItmLen;
    if(w < 1 || w > 16) return -1;
+   size_t bytes = 0;
    for(size_t i = 0; i < args.size(); i++)
    {
       const Arg &a = args[i];
       if(a.LnkIdx != L) continue;
       if(!(a.FlgTab & GmlReadMode) || (a.FlgTab & GmlWriteMode) || a.NmbItm != 1 || a.CntIdx >= 0) return -1;
+      bytes += OclTypSiz[a.ItmTyp];
       staged->push_back((int)i);
    }
-   if(staged->empty()) return -1;
+   // a tile holds at most 256 vertices: their data must fit 64 KB of LDS
+   if(staged->empty() || bytes * 256 > 64 * 1024) return -1;
    for(const char *id : kRowIds)
       if(BodyUses(KrnSrc, id)) return -1;
    return L;

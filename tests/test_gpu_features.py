@@ -192,3 +192,36 @@ def test_uplink_reads_of_every_width_staged_and_direct(typ, dtype, width):
             deg, ball = g.uplink_table(G.GmlVertices, G.GmlTetrahedra, V, 32)
     want = x0[ball[np.arange(V), deg - 1] >> 4]
     assert np.array_equal(outs[0], want) and np.array_equal(outs[1], want)
+
+
+@pytest.mark.parametrize("fields", [[(G.GmlByt2, np.int8, 2)], [(G.GmlDbl16, np.float64, 16)],
+                                    [(G.GmlDbl16, np.float64, 16)] * 3])
+def test_downlink_reads_of_every_width_in_every_form(fields):
+    # vertex data of 2 to 128 bytes (three 128-byte fields: more than the
+    # element-tile form's LDS takes) read through the tet -> vertex downlink;
+    # every launch of the tuner's shapes gives the sum of the tet's corner 0
+    # and corner 3 values, exactly
+    crd, ref, tet = G.cube_mesh(9, 0.2, 13)
+    V, T = len(crd), len(tet)
+    rng = np.random.default_rng(len(fields))
+    xs = [(rng.integers(-60, 60, size=(V, w)).astype(dt) if dt == np.int8 else
+           rng.normal(size=(V, w)).astype(dt)) for _, dt, w in fields]
+    body = "{ " + " ".join("z%d = x%d[0] + x%d[3];" % (i, i, i) for i in range(len(fields))) + " }"
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        args, zs = [], []
+        for i, (typ, dt, w) in enumerate(fields):
+            x = g.solution(G.GmlVertices, 1, typ, "x%d" % i)
+            g.upload(x, xs[i])
+            z = g.solution(G.GmlTetrahedra, 1, typ, "z%d" % i)
+            args += [(x, G.GmlReadMode, 0)]
+            zs.append(z)
+        args += [(z, G.GmlWriteMode, 0) for z in zs]
+        k = g.compile(body, "corner_sum", G.GmlTetrahedra, args)
+        for _ in range(20):
+            g.launch(k)
+            for i, (typ, dt, w) in enumerate(fields):
+                got = g.download_raw(zs[i], dt, (T, w))
+                assert np.array_equal(got, (xs[i][tet[:, 0]] + xs[i][tet[:, 3]]).astype(dt))
+        tiled = bool(g.L.GmlxGetKernelTileSource(g.h, k))
+    assert tiled == (sum(f[0] != G.GmlByt2 and 128 or 2 for f in fields) * 256 <= 64 * 1024)
