@@ -225,3 +225,29 @@ def test_downlink_reads_of_every_width_in_every_form(fields):
                 assert np.array_equal(got, (xs[i][tet[:, 0]] + xs[i][tet[:, 3]]).astype(dt))
         tiled = bool(g.L.GmlxGetKernelTileSource(g.h, k))
     assert tiled == (sum(f[0] != G.GmlByt2 and 128 or 2 for f in fields) * 256 <= 64 * 1024)
+
+
+def test_downlink_and_neighbour_reads_in_one_kernel():
+    # a tet kernel reading its vertices' coordinates (downlink: the element-
+    # tile form stages them) and a tet field through the neighbour link
+    # (read directly): the same values in every form the tuner tries
+    crd, ref, tet = G.cube_mesh(8, 0.2, 14)
+    T = len(tet)
+    val0 = np.random.default_rng(2).uniform(1, 2, T).astype(np.float32)
+    with G.Gml() as g:
+        vi, ti = g.set_mesh(crd, ref, tet)
+        ngb = g.L.GmlSetNeighbours(g.h, G.GmlTetrahedra)
+        v = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "val")
+        g.upload(v, val0)
+        o = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "mix")
+        k = g.compile("{ mix = VerCrd[0].x + VerCrd[2].y + val[1]; }", "mixed_links", G.GmlTetrahedra,
+                      [(vi, G.GmlReadMode, 0), (v, G.GmlReadMode, ngb), (o, G.GmlWriteMode, 0)])
+        table = g.download_raw(ngb, np.int32, (T, 4))
+        c = crd.astype(np.float32)
+        want = (c[tet[:, 0], 0] + c[tet[:, 2], 1]) + np.where(table[:, 1] != 0, val0[table[:, 1]], np.float32(0))
+        forms = set()
+        for _ in range(20):
+            g.launch(k)
+            assert np.array_equal(g.download_raw(o, np.float32, (T,)), want)
+            forms.add(g.L.GmlxGetKernelForm(g.h, k))
+    assert forms - {0}
