@@ -138,51 +138,79 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ deg
    for(int i = 0; i < B; i++) y[(size_t)r * B + i] = acc[i];
 }
 
-// Elementwise vector kernels: 16 bytes per load and store (float4 /
-// double2), the tail of n done by the last thread.
+// Elementwise vector kernels: 16 bytes per load and store (4 floats / 2
+// doubles).  A work-group takes one contiguous block of kVecLoads x 256
+// vectors (thread t: vectors t, t + 256, ...), every load of the block in
+// flight before the stores, streamed past the caches (nontemporal); the
+// entries past the last whole vector are done by thread 0 of the last
+// work-group.  Each entry is read and written by the same thread, so the
+// output may be one of the inputs.
+typedef float  vf4 __attribute__((ext_vector_type(4)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
 template<class T> struct Vec16;
-template<> struct Vec16<float>  { typedef float4  t; static const int n = 4; };
-template<> struct Vec16<double> { typedef double2 t; static const int n = 2; };
+template<> struct Vec16<float>  { typedef vf4 t; static const int n = 4; };
+template<> struct Vec16<double> { typedef vd2 t; static const int n = 2; };
+static const int kVecLoads = 4;
 
 template<class T>
-__global__ void __launch_bounds__(256) k_add3(long long n, const T *__restrict__ u, const T *__restrict__ v,
-                                              const T *__restrict__ w, T *__restrict__ y)
+__global__ void __launch_bounds__(256) k_add3(long long n, const T *u, const T *v, const T *w, T *y)
 {
    typedef typename Vec16<T>::t V;
    const int W = Vec16<T>::n;
-   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * W;
-   if(i + W <= n)
+   const long long nv = n / W;
+   const long long i0 = (long long)blockIdx.x * (kVecLoads * 256) + threadIdx.x;
+   V a[kVecLoads], b[kVecLoads], c[kVecLoads];
+#pragma unroll
+   for(int k = 0; k < kVecLoads; k++)
    {
-      V a = *(const V *)&u[i], b = *(const V *)&v[i], c = *(const V *)&w[i];
-      a += b;
-      a += c;
-      *(V *)&y[i] = a;
+      const long long i = i0 + k * 256;
+      if(i < nv)
+      {
+         a[k] = __builtin_nontemporal_load((const V *)u + i);
+         b[k] = __builtin_nontemporal_load((const V *)v + i);
+         c[k] = __builtin_nontemporal_load((const V *)w + i);
+      }
    }
-   else
-      for(long long k = i; k < n; k++) y[k] = u[k] + v[k] + w[k];
+#pragma unroll
+   for(int k = 0; k < kVecLoads; k++)
+   {
+      const long long i = i0 + k * 256;
+      if(i < nv)
+      {
+         V r = a[k] + b[k];
+         r += c[k];
+         __builtin_nontemporal_store(r, (V *)y + i);
+      }
+   }
+   if(blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+      for(long long k = nv * W; k < n; k++) y[k] = u[k] + v[k] + w[k];
 }
 
 template<class T>
-__global__ void __launch_bounds__(256) k_scale(long long n, T *__restrict__ u, T s)
+__global__ void __launch_bounds__(256) k_scale(long long n, T *u, T s)
 {
    typedef typename Vec16<T>::t V;
    const int W = Vec16<T>::n;
-   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * W;
-   if(i + W <= n)
-   {
-      V a = *(const V *)&u[i];
-      a *= s;
-      *(V *)&u[i] = a;
-   }
-   else
-      for(long long k = i; k < n; k++) u[k] *= s;
+   const long long nv = n / W;
+   const long long i0 = (long long)blockIdx.x * (kVecLoads * 256) + threadIdx.x;
+   V a[kVecLoads];
+#pragma unroll
+   for(int k = 0; k < kVecLoads; k++)
+      if(i0 + k * 256 < nv) a[k] = __builtin_nontemporal_load((const V *)u + i0 + k * 256);
+#pragma unroll
+   for(int k = 0; k < kVecLoads; k++)
+      if(i0 + k * 256 < nv) __builtin_nontemporal_store(a[k] * s, (V *)u + i0 + k * 256);
+   if(blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+      for(long long k = nv * W; k < n; k++) u[k] *= s;
 }
 
-// grid of k_add3 / k_scale over n entries
+// grid of k_add3 / k_scale over n entries (at least one work-group, which
+// also does the entries past the last whole vector)
 template<class T> static unsigned int VecBlocks(long long n)
 {
-   const long long w = Vec16<T>::n * 256;
-   return (unsigned int)((n + w - 1) / w);
+   const long long w = (long long)Vec16<T>::n * 256 * kVecLoads;
+   const long long nb = (n + w - 1) / w;
+   return (unsigned int)(nb < 1 ? 1 : nb);
 }
 
 template<class T>
