@@ -27,9 +27,11 @@ Second workload (--workload cg): block-sparse SpMV and block-Jacobi CG on the
 3x3-block vector Laplacian of the edge graph of a 128^3-cell Kuhn cube
 (2,146,689 block rows, 31.8M blocks), FP32 values from a seeded SPD edge
 weighting.  A step is one GmlMultMatVec (y = A x), counted as 40 B per block
-(36 B values + 4 B column) + 28 B per row (x, y, row length); then a plain
-FP32 CG solve and an FP32 CG with FP64 residual refinement to 1e-10, each
-checked against the oracle's FP64 product b - A x on the host.
+(36 B values + 4 B column) + 28 B per row (x, y, row length); the kernel's
+own bytes (own_bytes_per_launch) count 2 B for a column it reads as a 16-bit
+offset from its row.  Then a plain FP32 CG solve and an FP32 CG with FP64
+residual refinement to 1e-10, each checked against the oracle's FP64 product
+b - A x on the host.
 
 Third workload (--workload batch): 512 Kuhn cube meshes with cells per side
 drawn log-uniform in [8, 96] (seeded; jitter +-0.3h), kept on the device as
@@ -664,6 +666,14 @@ def run_cg(args):
     vs = L.GmlNewVector(h, nv, 3, None, G.GmlDbl)
     if min(m, vx, vy, vs) <= 0:
         raise RuntimeError("matrix/vector creation failed")
+    # the product's own bytes: a block whose column the kernel reads as a
+    # 16-bit offset from its row (GmlxGetMatrixInfo) costs 38 B instead of 40
+    minfo = (C.c_longlong * 6)()
+    if L.GmlxGetMatrixInfo(h, m, minfo) != 1:
+        raise RuntimeError("GmlxGetMatrixInfo failed")
+    spmv_own = spmv_bytes - 2 * minfo[5]
+    log("matrix: %d SELL-64 slabs, %d of them with 16-bit columns; %d of %d blocks (%.1f%%) read through 16-bit "
+        "columns" % (minfo[3], minfo[4], minfo[5], nb, 100. * minfo[5] / nb))
     if args.copy_calib:
         one = C.c_double(1.0)
         for _ in range(args.copy_calib):
@@ -807,7 +817,9 @@ def run_cg(args):
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kgbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_spmv<3,float,float> (GmlMultMatVec)",
-                     "bytes_per_launch": spmv_bytes, "avg_us": round(kavg * 1e6, 2)},
+                     "bytes_per_launch": spmv_bytes, "avg_us": round(kavg * 1e6, 2),
+                     "own_bytes_per_launch": spmv_own, "own_frac": round(spmv_own / kavg / 1e9 / HBM_PEAK_GBS, 4),
+                     "blocks_16bit_columns": minfo[5]},
         "cg": solves,
         "stream_probe_GBs": probe,
         "cpu_baseline": cpu,

@@ -104,6 +104,7 @@ _SIGS = {
     "GmlxGetVector": (_I, [_S, _I, _P]),
     "GmlxFreeVector": (_I, [_S, _I]),
     "GmlxFreeMatrix": (_I, [_S, _I]),
+    "GmlxGetMatrixInfo": (_I, [_S, _I, C.POINTER(C.c_longlong)]),
     "GmlxDotVec": (_I, [_S, _I, _I, _DP]),
     "GmlxReduceVectorNoAlloc": (_I, [_S, _I, _I, _DP]),
     "GmlxSolveCG": (_I, [_S, _I, _I, _I, _D, _I, _I, _DP, _IP]),

@@ -34,6 +34,12 @@ struct Matrix {
    int *deg = nullptr, *col = nullptr;
    unsigned int *off = nullptr;
    void *val = nullptr;
+   // near slots: in every slab the first kn slots of each row hold columns
+   // within +-32767 of the row, also stored as 16-bit offsets from it
+   // (slab c: kn = off16[c + 1] - off16[c] slots at col16[(off16[c] + k) * 64 + lane])
+   unsigned int *off16 = nullptr;
+   short *col16 = nullptr;
+   long long nslab16 = 0, n16 = 0;   // slabs with 16-bit columns, blocks read through them
    double MemAcc = 0., FltOpp = 0.;
    CgWork *cg = nullptr;        // CG workspace, kept from the first solve on
 };
@@ -54,6 +60,7 @@ void FreeAlgebra(GmlCtx *g)
          FreeCgWork(g, g->mat[i]->cg);
          DevFree(g, g->mat[i]->deg); DevFree(g, g->mat[i]->col);
          DevFree(g, g->mat[i]->off); DevFree(g, g->mat[i]->val);
+         DevFree(g, g->mat[i]->off16); DevFree(g, g->mat[i]->col16);
          delete g->mat[i];
          g->mat[i] = nullptr;
       }
@@ -73,40 +80,60 @@ void FreeAlgebra(GmlCtx *g)
 // issued before either product, so a wave keeps ~2(B*B+B+1) loads in flight;
 // the products are added in slot order, as a one-at-a-time loop would.
 template<int B, class T, class X>
-__device__ __forceinline__ void row_product(int r, int l, int d, size_t base, const int *__restrict__ col,
+__device__ __forceinline__ void slot_pair(size_t s0, int j0, int j1, int l, const T *__restrict__ val,
+                                          const X *__restrict__ x, X *acc)
+{
+   const size_t s1 = s0 + 1;
+   T v0[B * B], v1[B * B];
+#pragma unroll
+   for(int e = 0; e < B * B; e++)
+   {
+      v0[e] = __builtin_nontemporal_load(&val[(s0 * (B * B) + e) * 64 + l]);
+      v1[e] = __builtin_nontemporal_load(&val[(s1 * (B * B) + e) * 64 + l]);
+   }
+   X x0[B], x1[B];
+#pragma unroll
+   for(int m = 0; m < B; m++) { x0[m] = x[(size_t)j0 * B + m]; x1[m] = x[(size_t)j1 * B + m]; }
+#pragma unroll
+   for(int i = 0; i < B; i++)
+   {
+      X t = 0;
+#pragma unroll
+      for(int m = 0; m < B; m++) t += (X)v0[i * B + m] * x0[m];
+      acc[i] += t;
+   }
+#pragma unroll
+   for(int i = 0; i < B; i++)
+   {
+      X t = 0;
+#pragma unroll
+      for(int m = 0; m < B; m++) t += (X)v1[i * B + m] * x1[m];
+      acc[i] += t;
+   }
+}
+
+// Row r (lane l of its slab, d slots from slab slot base).  The slab's first
+// kn slots (kn <= every row's degree in the slab, so the same count in every
+// lane) take their columns from the 16-bit offsets at slab slot base16; the
+// rest from the 32-bit columns.  At 256^3 (the edge Laplacian of the 100.7M-tet
+// cube) 4% of the columns lie beyond +-32767 of their row.
+template<int B, class T, class X>
+__device__ __forceinline__ void row_product(int r, int l, int d, size_t base, int kn, size_t base16,
+                                            const int *__restrict__ col, const short *__restrict__ col16,
                                             const T *__restrict__ val, const X *__restrict__ x, X *acc)
 {
    int k = 0;
+   for(; k + 1 < kn; k += 2)
+   {
+      const int j0 = r + (int)__builtin_nontemporal_load(&col16[(base16 + k) * 64 + l]);
+      const int j1 = r + (int)__builtin_nontemporal_load(&col16[(base16 + k + 1) * 64 + l]);
+      slot_pair<B, T, X>(base + k, j0, j1, l, val, x, acc);
+   }
    for(; k + 1 < d; k += 2)
    {
-      const size_t s0 = base + k, s1 = s0 + 1;
-      const int j0 = __builtin_nontemporal_load(&col[s0 * 64 + l]), j1 = __builtin_nontemporal_load(&col[s1 * 64 + l]);
-      T v0[B * B], v1[B * B];
-#pragma unroll
-      for(int e = 0; e < B * B; e++)
-      {
-         v0[e] = __builtin_nontemporal_load(&val[(s0 * (B * B) + e) * 64 + l]);
-         v1[e] = __builtin_nontemporal_load(&val[(s1 * (B * B) + e) * 64 + l]);
-      }
-      X x0[B], x1[B];
-#pragma unroll
-      for(int m = 0; m < B; m++) { x0[m] = x[(size_t)j0 * B + m]; x1[m] = x[(size_t)j1 * B + m]; }
-#pragma unroll
-      for(int i = 0; i < B; i++)
-      {
-         X t = 0;
-#pragma unroll
-         for(int m = 0; m < B; m++) t += (X)v0[i * B + m] * x0[m];
-         acc[i] += t;
-      }
-#pragma unroll
-      for(int i = 0; i < B; i++)
-      {
-         X t = 0;
-#pragma unroll
-         for(int m = 0; m < B; m++) t += (X)v1[i * B + m] * x1[m];
-         acc[i] += t;
-      }
+      const size_t s0 = base + k;
+      const int j0 = __builtin_nontemporal_load(&col[s0 * 64 + l]), j1 = __builtin_nontemporal_load(&col[(s0 + 1) * 64 + l]);
+      slot_pair<B, T, X>(s0, j0, j1, l, val, x, acc);
    }
    if(k < d)
    {
@@ -125,7 +152,8 @@ __device__ __forceinline__ void row_product(int r, int l, int d, size_t base, co
 
 template<int B, class T, class X>
 __global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ deg, const unsigned int *__restrict__ off,
-                                              const int *__restrict__ col, const T *__restrict__ val,
+                                              const int *__restrict__ col, const unsigned int *__restrict__ off16,
+                                              const short *__restrict__ col16, const T *__restrict__ val,
                                               const X *__restrict__ x, X *__restrict__ y, int remap)
 {
    const int r = (int)(xcd_block(remap) * blockDim.x + threadIdx.x);
@@ -133,7 +161,9 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int *__restrict__ deg
    X acc[B];
 #pragma unroll
    for(int i = 0; i < B; i++) acc[i] = 0;
-   row_product<B, T, X>(r, r & 63, deg[r], (size_t)off[r >> 6], col, val, x, acc);
+   const unsigned int b16 = off16[r >> 6];
+   row_product<B, T, X>(r, r & 63, deg[r], (size_t)off[r >> 6], (int)(off16[(r >> 6) + 1] - b16), (size_t)b16, col, col16,
+                        val, x, acc);
 #pragma unroll
    for(int i = 0; i < B; i++) y[(size_t)r * B + i] = acc[i];
 }
@@ -248,19 +278,20 @@ static int SpMV(const GmlCtx *g, const Matrix *m, const X *x, X *y)
    const Shape sh = BuiltinShape(g, GmlxBuiltinSpMV);
    const dim3 nb(Blocks(m->NmbLin, sh.grp)), gs(sh.grp);
    const int *deg = m->deg, *col = m->col;
-   const unsigned int *off = m->off;
+   const unsigned int *off = m->off, *o16 = m->off16;
+   const short *c16 = m->col16;
    const T *val = (const T *)m->val;
    int n = m->NmbLin, rm = sh.remap;
    switch(m->b)
    {
-      case 1: hipLaunchKernelGGL((k_spmv<1, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 2: hipLaunchKernelGGL((k_spmv<2, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 3: hipLaunchKernelGGL((k_spmv<3, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 4: hipLaunchKernelGGL((k_spmv<4, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 5: hipLaunchKernelGGL((k_spmv<5, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 6: hipLaunchKernelGGL((k_spmv<6, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 7: hipLaunchKernelGGL((k_spmv<7, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
-      case 8: hipLaunchKernelGGL((k_spmv<8, T, X>), nb, gs, 0, s, n, deg, off, col, val, x, y, rm); break;
+      case 1: hipLaunchKernelGGL((k_spmv<1, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 2: hipLaunchKernelGGL((k_spmv<2, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 3: hipLaunchKernelGGL((k_spmv<3, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 4: hipLaunchKernelGGL((k_spmv<4, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 5: hipLaunchKernelGGL((k_spmv<5, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 6: hipLaunchKernelGGL((k_spmv<6, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 7: hipLaunchKernelGGL((k_spmv<7, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
+      case 8: hipLaunchKernelGGL((k_spmv<8, T, X>), nb, gs, 0, s, n, deg, off, col, o16, c16, val, x, y, rm); break;
       default: return -1;
    }
    return hipGetLastError() == hipSuccess ? 1 : -6;
@@ -370,7 +401,8 @@ __global__ void __launch_bounds__(kFinBlock) k_cg_finish(CgScal *S, CgPart P, un
 // q = A p and this work-group's part of p.q
 template<int B>
 __global__ void __launch_bounds__(256) k_cg_spmv(int n, const int *__restrict__ deg, const unsigned int *__restrict__ off,
-                                                 const int *__restrict__ col, const float *__restrict__ val,
+                                                 const int *__restrict__ col, const unsigned int *__restrict__ off16,
+                                                 const short *__restrict__ col16, const float *__restrict__ val,
                                                  const float *__restrict__ p, float *__restrict__ q, CgPart P, int remap)
 {
    const int r = (int)(xcd_block(remap) * blockDim.x + threadIdx.x);
@@ -380,7 +412,9 @@ __global__ void __launch_bounds__(256) k_cg_spmv(int n, const int *__restrict__ 
       float acc[B];
 #pragma unroll
       for(int i = 0; i < B; i++) acc[i] = 0.f;
-      row_product<B, float, float>(r, r & 63, deg[r], (size_t)off[r >> 6], col, val, p, acc);
+      const unsigned int b16 = off16[r >> 6];
+      row_product<B, float, float>(r, r & 63, deg[r], (size_t)off[r >> 6], (int)(off16[(r >> 6) + 1] - b16), (size_t)b16,
+                                   col, col16, val, p, acc);
 #pragma unroll
       for(int i = 0; i < B; i++)
       {
@@ -672,43 +706,78 @@ int GmlNewMatrix(size_t GmlIdx, int NmbLin, int NmbBlk, int BlkSiz, void *val, i
       off[c + 1] = off[c] + w;
    }
    long long nslab = off[nchunk];
+   // each row's blocks in slot order: those whose column lies within +-32767
+   // of the row first, then the others, each group in the caller's order;
+   // a slab's first kn slots (kn = its rows' smallest count of near blocks)
+   // also get their columns as 16-bit offsets from the row
+   auto near = [](int c, int r) { return c - r <= 32767 && r - c <= 32767; };
+   std::vector<unsigned int> off16((size_t)nchunk + 1, 0);
+   for(long long c = 0; c < nchunk; c++)
+   {
+      int kn = INT32_MAX;
+      for(long long r = c * 64; r < std::min<long long>(NmbLin, c * 64 + 64); r++)
+      {
+         int nn = 0;
+         for(int k = lin[r]; k < lin[r + 1]; k++) nn += near(col[k], (int)r);
+         kn = std::min(kn, nn);
+      }
+      off16[c + 1] = off16[c] + (unsigned int)kn;
+   }
+   const long long nslab16 = off16[nchunk];
    std::vector<int> hc((size_t)nslab * 64 + 64, 0);
+   std::vector<short> h16((size_t)nslab16 * 64 + 64, 0);
    std::vector<char> hv(((size_t)nslab * bb * 64 + 64) * fs, 0);
+   std::vector<int> order;
    for(int r = 0; r < NmbLin; r++)
    {
-      size_t base = off[r >> 6], l = r & 63;
+      const size_t base = off[r >> 6], base16 = off16[r >> 6], l = r & 63;
+      const int kn = (int)(off16[(r >> 6) + 1] - off16[r >> 6]);
+      order.clear();
+      for(int k = lin[r]; k < lin[r + 1]; k++) if(near(col[k], r)) order.push_back(k);
+      for(int k = lin[r]; k < lin[r + 1]; k++) if(!near(col[k], r)) order.push_back(k);
       for(int k = 0; k < deg[r]; k++)
       {
-         size_t s = base + k;
-         hc[s * 64 + l] = col[lin[r] + k];
+         const size_t s = base + k;
+         const int src = order[k];
+         hc[s * 64 + l] = col[src];
+         if(k < kn) h16[(base16 + k) * 64 + l] = (short)(col[src] - r);
          for(int e = 0; e < bb; e++)
-            memcpy(&hv[((s * bb + e) * 64 + l) * fs], (char *)val + ((size_t)(lin[r] + k) * bb + e) * fs, fs);
+            memcpy(&hv[((s * bb + e) * 64 + l) * fs], (char *)val + ((size_t)src * bb + e) * fs, fs);
       }
    }
    Matrix *m = new Matrix();
    m->NmbLin = NmbLin; m->NmbBlk = NmbBlk; m->b = b; m->flt = FltTyp; m->nslab = nslab;
+   m->nslab16 = nslab16;
+   for(int r = 0; r < NmbLin; r++) m->n16 += off16[(r >> 6) + 1] - off16[r >> 6];
    m->deg = (int *)DevAlloc(g, (size_t)NmbLin * 4);
    m->off = (unsigned int *)DevAlloc(g, off.size() * 4);
    m->col = (int *)DevAlloc(g, hc.size() * 4);
    m->val = DevAlloc(g, hv.size());
-   if(!m->deg || !m->off || !m->col || !m->val
+   m->off16 = (unsigned int *)DevAlloc(g, off16.size() * 4);
+   m->col16 = (short *)DevAlloc(g, h16.size() * 2);
+   if(!m->deg || !m->off || !m->col || !m->val || !m->off16 || !m->col16
    || !DevCopy(g, m->deg, deg.data(), (size_t)NmbLin * 4, 1)
    || !DevCopy(g, m->off, off.data(), off.size() * 4, 1)
    || !DevCopy(g, m->col, hc.data(), hc.size() * 4, 1)
-   || !DevCopy(g, m->val, hv.data(), hv.size(), 1))
+   || !DevCopy(g, m->val, hv.data(), hv.size(), 1)
+   || !DevCopy(g, m->off16, off16.data(), off16.size() * 4, 1)
+   || !DevCopy(g, m->col16, h16.data(), h16.size() * 2, 1))
    {
       DevFree(g, m->deg); DevFree(g, m->off); DevFree(g, m->col); DevFree(g, m->val);
+      DevFree(g, m->off16); DevFree(g, m->col16);
       delete m;
       return 0;
    }
-   g->MemSiz += (size_t)NmbLin * 4 + off.size() * 4 + hc.size() * 4 + hv.size();
-   g->MovSiz += (size_t)NmbLin * 4 + off.size() * 4 + hc.size() * 4 + hv.size();
+   const size_t held = (size_t)NmbLin * 4 + off.size() * 4 + hc.size() * 4 + hv.size() + off16.size() * 4 + h16.size() * 2;
+   g->MemSiz += held;
+   g->MovSiz += held;
    // bytes and flops of one product, as the reference models them
    m->MemAcc = ((double)NmbLin * b + (double)NmbBlk * b * (b + 1)) * fs;
    m->FltOpp = (double)NmbBlk * b * b * 2;
    g->mat[idx] = m;
-   DbgPrint(g, "Matrix %d: %d block rows, %d blocks of %dx%d, %lld SELL-64 slabs (%.1f%% occupancy)\n",
-            idx, NmbLin, NmbBlk, b, b, nslab, nslab ? 100. * NmbBlk / (nslab * 64.) : 0.);
+   DbgPrint(g, "Matrix %d: %d block rows, %d blocks of %dx%d, %lld SELL-64 slabs (%.1f%% occupancy), "
+            "%lld with 16-bit columns\n", idx, NmbLin, NmbBlk, b, b, nslab, nslab ? 100. * NmbBlk / (nslab * 64.) : 0.,
+            nslab16);
    return idx;
 }
 
@@ -904,6 +973,16 @@ int GmlxFreeVector(size_t GmlIdx, int VecIdx)
    return 1;
 }
 
+int GmlxGetMatrixInfo(size_t GmlIdx, int MatIdx, long long *info)
+{
+   GML_CTX(g, GmlIdx);
+   const Matrix *m = Mat(g, MatIdx);
+   if(!m || !info) return 0;
+   info[0] = m->NmbLin; info[1] = m->NmbBlk; info[2] = m->b;
+   info[3] = m->nslab; info[4] = m->nslab16; info[5] = m->n16;
+   return 1;
+}
+
 int GmlxFreeMatrix(size_t GmlIdx, int MatIdx)
 {
    GML_CTX(g, GmlIdx);
@@ -911,6 +990,7 @@ int GmlxFreeMatrix(size_t GmlIdx, int MatIdx)
    if(!m) return 0;
    FreeCgWork(g, m->cg);
    DevFree(g, m->deg); DevFree(g, m->col); DevFree(g, m->off); DevFree(g, m->val);
+   DevFree(g, m->off16); DevFree(g, m->col16);
    delete m;
    g->mat[MatIdx] = nullptr;
    return 1;
@@ -993,8 +1073,8 @@ int GmlxSolveCG(size_t GmlIdx, int MatIdx, int RhsIdx, int SolIdx, double tol, i
          for(int c = 0; c < chunk && it < cap; c++, it++)
          {
             int t1 = TimedLaunch(g, GmlxBuiltinCgSpMV, [&]() {
-               GML_B(k_cg_spmv, dim3(nbs), dim3(sh.grp), 0, s, nrow, A->deg, A->off, A->col, (const float *)A->val,
-                     w.p, w.q, w.P, sh.remap);
+               GML_B(k_cg_spmv, dim3(nbs), dim3(sh.grp), 0, s, nrow, A->deg, A->off, A->col, A->off16, A->col16,
+                     (const float *)A->val, w.p, w.q, w.P, sh.remap);
                hipLaunchKernelGGL(k_cg_finish<CgFinAlpha>, dim3(1), dim3(kFinBlock), 0, s, w.S, w.P, nbs);
             });
             int t2 = TimedLaunch(g, GmlxBuiltinCgUpdate, [&]() {

@@ -137,6 +137,15 @@ int    GmlxGetVector(size_t GmlIdx, int VecIdx, void *val);
 int    GmlxFreeVector(size_t GmlIdx, int VecIdx);
 int    GmlxFreeMatrix(size_t GmlIdx, int MatIdx);
 
+/* Layout of a block matrix made by GmlNewMatrix, 6 values: {block rows,
+   blocks, block size, SELL-64 slabs, slabs with 16-bit columns, blocks read
+   through a 16-bit column}.  Each row's blocks whose column lies within
+   +-32767 of the row come first (the caller's order otherwise kept); a
+   64-row slab's first kn slots, kn its rows' smallest count of such blocks,
+   also hold their columns as 16-bit offsets, which the products read instead
+   of the 32-bit columns.  Returns 1, or 0 for an invalid index. */
+int    GmlxGetMatrixInfo(size_t GmlIdx, int MatIdx, long long *info);
+
 /* Dot product of two FP32 vectors with FP64 accumulation. */
 int    GmlxDotVec(size_t GmlIdx, int VecA, int VecB, double *res);
 

@@ -155,3 +155,81 @@ def test_block_cg(refine, tol):
         # FP32 CG stops on its recurrence residual; the true residual of an
         # FP32 iterate bottoms out near cond(A) * 2^-24
         assert r <= 1e-4 and fe <= 1e-5
+
+
+def permuted(rp, col, val, perm):
+    """The block matrix P A P^T: new row i is old row perm[i], its columns
+    renumbered by the inverse permutation (block order within a row kept)."""
+    n = len(rp) - 1
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    deg = np.diff(rp)[perm]
+    nrp = np.zeros(n + 1, np.int32)
+    nrp[1:] = np.cumsum(deg)
+    idx = np.repeat(rp[perm] - nrp[:-1], deg) + np.arange(nrp[-1])
+    return nrp, inv[col[idx]].astype(np.int32), val[idx]
+
+
+@pytest.mark.parametrize("b,flt", [(3, G.GmlFlt), (2, G.GmlDbl)])
+def test_spmv_far_columns_and_ragged_rows(b, flt):
+    """Rows whose columns lie beyond +-32767 of the row take the 32-bit column
+    path, the near ones the 16-bit offsets; degrees differ inside a slab
+    (so the slab's 16-bit slot count is its smallest near count), some rows
+    are empty, near and far columns alternate in the caller's order."""
+    n = 70001
+    rng = np.random.default_rng(11 + b)
+    deg = rng.integers(0, 9, size=n)
+    deg[::97] = 0
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(deg)
+    rows = np.repeat(np.arange(n), deg)
+    near = rng.random(len(rows)) < 0.7
+    # far columns about half the matrix away (|col - row| >= 34000)
+    col = np.where(near, np.clip(rows + rng.integers(-32767, 32768, len(rows)), 0, n - 1),
+                   (rows + n // 2 + rng.integers(-1000, 1000, len(rows))) % n).astype(np.int32)
+    dt = np.float32 if flt == G.GmlFlt else np.float64
+    val = rng.normal(size=(len(col), b, b)).astype(dt)
+    x = rng.normal(size=n * b).astype(dt)
+    with G.Gml() as g:
+        m = g.L.GmlNewMatrix(g.h, n, len(col), b, G.ptr(val), G.ptr(col), G.ptr(rp), flt)
+        assert m > 0
+        vx = new_vec(g, n, b, x, flt)
+        vy = new_vec(g, n, b, None, flt)
+        assert g.L.GmlMultMatVec(g.h, m, vy, vx) == 1
+        y = get_vec(g, vy, n, b, dt)
+    far = np.abs(col.astype(np.int64) - rows) > 32767
+    assert far.mean() > 0.2
+    # FP64 products: the oracle's (FP32 block values) for FP32 data, scipy's
+    # BSR product for FP64 data
+    if flt == G.GmlFlt:
+        y64 = O.bsr_spmv64(rp, col, val, x.astype(np.float64), b)
+    else:
+        y64 = sp.bsr_matrix((val, col, rp), shape=(n * b, n * b)) @ x
+    err = np.abs(y - y64).max() / np.abs(y64).max()
+    print("b=%d %s: %.1f%% far columns, max err / max|y| vs FP64 oracle %.3e"
+          % (b, dt.__name__, 100 * far.mean(), err))
+    assert err <= (1e-5 if flt == G.GmlFlt else 1e-12)
+    assert np.all(y[np.repeat(deg == 0, b)] == 0)
+
+
+def test_block_cg_far_columns():
+    """Refined CG on a randomly renumbered edge Laplacian (68,921 block rows):
+    a quarter of the couplings lie beyond +-32767 rows, so both column paths
+    of the CG product run; the FP64 residual is recomputed on the host."""
+    nv, rp, col, val = laplacian(40, 5)
+    perm = np.random.default_rng(3).permutation(nv)
+    rp, col, val = permuted(rp, col, val, perm)
+    rows = np.repeat(np.arange(nv), np.diff(rp))
+    assert (np.abs(col.astype(np.int64) - rows) > 32767).mean() > 0.15
+    rhs = np.random.default_rng(8).normal(size=nv * 3).astype(np.float32)
+    with G.Gml() as g:
+        m = g.L.GmlNewMatrix(g.h, nv, len(col), 3, G.ptr(val), G.ptr(col), G.ptr(rp), G.GmlFlt)
+        vb = new_vec(g, nv, 3, rhs)
+        vx = new_vec(g, nv, 3, None, G.GmlDbl)
+        rel, it = C.c_double(), C.c_int()
+        assert g.L.GmlxSolveCG(g.h, m, vb, vx, 1e-10, 20000, 1, C.byref(rel), C.byref(it)) == 1
+        x = get_vec(g, vx, nv, 3, np.float64)
+    b64 = rhs.astype(np.float64)
+    r = np.linalg.norm(b64 - O.bsr_spmv64(rp, col, val, x, 3)) / np.linalg.norm(b64)
+    print("permuted CG: %d iterations, device relres %.3e, host FP64 relres %.3e" % (it.value, rel.value, r))
+    assert r <= 1e-10
