@@ -20,6 +20,8 @@
 
 #include "gml_internal.h"
 
+#include <mutex>
+
 #include <hip/hiprtc.h>
 #include <cstdarg>
 #include <cstdlib>
@@ -997,8 +999,7 @@ void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
          float ms = 0.f;
          if(hipEventElapsedTime(&ms, ev[i].beg, ev[i].end) == hipSuccess)
             *acc += ms * 1e-3;
-         (void)hipEventDestroy(ev[i].beg);
-         (void)hipEventDestroy(ev[i].end);
+         FreeRec(ev[i]);
       }
       else
          ev[keep++] = ev[i];
@@ -1007,11 +1008,47 @@ void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
    ev.resize(keep);
 }
 
-static bool NewRec(LaunchRec *r)
+// Timing events are pooled per device: creating and destroying two events
+// for every launch cost several microseconds of host time per launch, more
+// than a small kernel runs.  A pool keeps at most kEvPoolMax events.
+static const int kEvPoolDev = 64;
+static const size_t kEvPoolMax = 8192;
+static std::mutex g_ev_mtx;
+static std::vector<hipEvent_t> g_ev_pool[kEvPoolDev];
+
+static bool TakeEvent(int dev, hipEvent_t *e)
 {
-   if(hipEventCreate(&r->beg) != hipSuccess) return false;
-   if(hipEventCreate(&r->end) != hipSuccess) { (void)hipEventDestroy(r->beg); return false; }
+   {
+      std::lock_guard<std::mutex> lk(g_ev_mtx);
+      if(!g_ev_pool[dev].empty()) { *e = g_ev_pool[dev].back(); g_ev_pool[dev].pop_back(); return true; }
+   }
+   return hipEventCreate(e) == hipSuccess;
+}
+
+static void GiveEvent(int dev, hipEvent_t e)
+{
+   if(!e) return;
+   {
+      std::lock_guard<std::mutex> lk(g_ev_mtx);
+      if(dev >= 0 && dev < kEvPoolDev && g_ev_pool[dev].size() < kEvPoolMax) { g_ev_pool[dev].push_back(e); return; }
+   }
+   (void)hipEventDestroy(e);
+}
+
+bool NewRec(LaunchRec *r)
+{
+   int dev = 0;
+   if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kEvPoolDev) { (void)hipGetLastError(); return false; }
+   r->dev = dev;
+   if(!TakeEvent(dev, &r->beg)) { (void)hipGetLastError(); return false; }
+   if(!TakeEvent(dev, &r->end)) { (void)hipGetLastError(); GiveEvent(dev, r->beg); return false; }
    return true;
+}
+
+void FreeRec(const LaunchRec &r)
+{
+   GiveEvent(r.dev, r.beg);
+   GiveEvent(r.dev, r.end);
 }
 
 int RunKernel(GmlCtx *g, Kernel *k)
@@ -1144,8 +1181,8 @@ int RunKernel(GmlCtx *g, Kernel *k)
    {
       (void)hipGetLastError();
       fprintf(stderr, "gml: launching kernel %s failed: %s\n", k->name.c_str(), hipGetErrorString(e));
-      if(have) { (void)hipEventDestroy(rec.beg); (void)hipEventDestroy(rec.end); }
-      if(tun) { (void)hipEventDestroy(tr.beg); (void)hipEventDestroy(tr.end); }
+      if(have) FreeRec(rec);
+      if(tun) FreeRec(tr);
       return -6;
    }
    if(tun) (void)hipEventRecord(tr.end, g->stream);
@@ -1176,7 +1213,7 @@ int RunKernel(GmlCtx *g, Kernel *k)
             { best = ms; bi = i; }
          }
          (void)hipGetLastError();
-         for(auto &t : k->tune_ev) { (void)hipEventDestroy(t.beg); (void)hipEventDestroy(t.end); }
+         for(auto &t : k->tune_ev) FreeRec(t);
          k->tune_ev.clear();
          const int *sh = ShapeAt(k, bi);
          k->grp = k->stage_src >= 0 ? k->stage_grp : sh[0];
@@ -1611,7 +1648,7 @@ int GmlxSetKernelShape(size_t GmlIdx, int idx, int GrpSiz, int Remap, int Rows)
    {
       if(!i) continue;
       Kernel *t = &g->krn[i];
-      for(auto &e : t->tune_ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+      for(auto &e : t->tune_ev) FreeRec(e);
       t->tune_ev.clear();
       t->grp = GrpSiz;
       if(t->stage_src >= 0) t->stage_grp = GrpSiz;

@@ -84,7 +84,9 @@ struct Uplink {
    StagePlan *stage[3] = {nullptr, nullptr, nullptr};   // work-group size 64, 128, 256
 };
 
-struct LaunchRec { hipEvent_t beg, end; };
+// a pair of timing events around one launch, from the process's event pool
+// of the device they were created on (NewRec / FreeRec)
+struct LaunchRec { hipEvent_t beg = nullptr, end = nullptr; int dev = 0; };
 
 struct Kernel {
    bool          use = false;
@@ -283,6 +285,11 @@ int  NoAllocScratchInit(int dev);   // device current; once per device (GmlInit)
 void FreeAlgebra(GmlCtx *g);
 
 void HarvestEvents(std::vector<LaunchRec> &ev, double *acc);
+// take an event pair from the pool of the current device (created when the
+// pool is empty); false if none could be created
+bool NewRec(LaunchRec *r);
+// return an event pair to its device's pool
+void FreeRec(const LaunchRec &r);
 
 // launch shape of a built-in kernel on a handle
 inline Shape BuiltinShape(const GmlCtx *g, int which) { return g->shape[which]; }
@@ -315,13 +322,12 @@ int TimedLaunch(GmlCtx *g, int which, F launch)
 {
    KernStat *st = Stat(g, which);
    LaunchRec r;
-   bool ev = hipEventCreate(&r.beg) == hipSuccess;
-   if(ev && hipEventCreate(&r.end) != hipSuccess) { (void)hipEventDestroy(r.beg); ev = false; }
+   bool ev = NewRec(&r);
    if(ev) (void)hipEventRecord(r.beg, g->stream);
    launch();
    if(hipGetLastError() != hipSuccess)
    {
-      if(ev) { (void)hipEventDestroy(r.beg); (void)hipEventDestroy(r.end); }
+      if(ev) FreeRec(r);
       return -6;
    }
    if(ev)

@@ -41,7 +41,7 @@ void FreeBuiltinStats(GmlCtx *g)
       if(g_stats[i].first == g)
       {
          for(int w = 0; w < kNmbBuiltin; w++)
-            for(auto &e : g_stats[i].second[w].ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+            for(auto &e : g_stats[i].second[w].ev) FreeRec(e);
          delete[] g_stats[i].second;
          g_stats.erase(g_stats.begin() + i);
          return;

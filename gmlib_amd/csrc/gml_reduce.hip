@@ -148,8 +148,11 @@ __global__ void __launch_bounds__(kRedBlock) k_reduce(const float *__restrict__ 
 // order, eight independent loads in flight per thread
 static const int kFinBlock = 1024;
 
+// host, if not null, is pinned host memory that also receives the result (a
+// system-scope store: visible to the host once the stream has synchronised),
+// which saves the device-to-host copy of one double, 3.6 us per call
 template<int OP>
-__global__ void __launch_bounds__(kFinBlock) k_reduce_finish(RedScratch *__restrict__ s, int np)
+__global__ void __launch_bounds__(kFinBlock) k_reduce_finish(RedScratch *__restrict__ s, int np, double *host)
 {
    __shared__ double lds[kFinBlock / 64];
    double a = red_init<OP>();
@@ -166,7 +169,11 @@ __global__ void __launch_bounds__(kFinBlock) k_reduce_finish(RedScratch *__restr
       for(int u = 0; u < 8; u++) a = red_comb<OP>(a, v[u]);
    }
    a = block_reduce<OP>(a, lds);
-   if(threadIdx.x == 0) s->result = a;
+   if(threadIdx.x == 0)
+   {
+      s->result = a;
+      if(host) __hip_atomic_store(host, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   }
 }
 
 // work-groups of a reduction over n floats (one partial each)
@@ -176,12 +183,12 @@ static int RedGrid(long long n)
    return (int)(want < 1 ? 1 : (want > kRedMaxGrid ? kRedMaxGrid : want));
 }
 
-static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s)
+static int ReduceLaunch(hipStream_t st, const float *x, long long n, int op, RedScratch *s, double *host = nullptr)
 {
    // about four float4 per thread, up to 16384 work-groups
    const int grid = RedGrid(n);
 #define GML_RED(O) do { hipLaunchKernelGGL(k_reduce<O>, dim3(grid), dim3(kRedBlock), 0, st, x, n, s); \
-                         hipLaunchKernelGGL(k_reduce_finish<O>, dim3(1), dim3(kFinBlock), 0, st, s, grid); } while(0)
+                         hipLaunchKernelGGL(k_reduce_finish<O>, dim3(1), dim3(kFinBlock), 0, st, s, grid, host); } while(0)
    switch(op)
    {
       case GmlMin:  GML_RED(GmlMin);  break;
@@ -242,19 +249,22 @@ int ReduceDevice(GmlCtx *g, const float *x, long long n, int op, double *res, bo
    if(g->host_only) return -6;
    RedScratch *s = (RedScratch *)g->red_scratch;
    LaunchRec r;
-   bool ev = timed && hipEventCreate(&r.beg) == hipSuccess && hipEventCreate(&r.end) == hipSuccess;
+   bool ev = timed && NewRec(&r);
    if(ev) (void)hipEventRecord(r.beg, g->stream);
-   int rc = ReduceLaunch(g->stream, x, n, op, s);
-   if(rc != 1) return rc;
+   int rc = ReduceLaunch(g->stream, x, n, op, s, g->red_host);
+   if(rc != 1)
+   {
+      if(ev) FreeRec(r);
+      return rc;
+   }
    if(ev)
    {
       (void)hipEventRecord(r.end, g->stream);
       g->red_evts[op].push_back(r);
    }
-   if(hipMemcpyAsync(g->red_host, &s->result, sizeof(double), hipMemcpyDeviceToHost, g->stream) != hipSuccess
-   || hipStreamSynchronize(g->stream) != hipSuccess)
+   if(hipStreamSynchronize(g->stream) != hipSuccess)
       return -6;
-   *res = *g->red_host;
+   *res = *(volatile double *)g->red_host;
    if(g->red_evts[op].size() > 256) HarvestEvents(g->red_evts[op], &g->red_time[op]);
    return 1;
 }

@@ -311,14 +311,14 @@ void GmlStop(size_t GmlIdx)
    for(int i = 1; i <= GmlMaxKrn; i++)
    {
       Kernel *k = &g->krn[i];
-      for(auto &e : k->evts) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
-      for(auto &e : k->tune_ev) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+      for(auto &e : k->evts) FreeRec(e);
+      for(auto &e : k->tune_ev) FreeRec(e);
       if(k->mod) (void)hipModuleUnload(k->mod);
       if(k->mod2) (void)hipModuleUnload(k->mod2);
       if(k->mod3) (void)hipModuleUnload(k->mod3);
    }
    for(int r = 0; r < GmlMaxRed; r++)
-      for(auto &e : g->red_evts[r]) { (void)hipEventDestroy(e.beg); (void)hipEventDestroy(e.end); }
+      for(auto &e : g->red_evts[r]) FreeRec(e);
    FreeBuiltinStats(g);
    FreeSweepPlan(g, g->plan);
    FreeSweepPlan(g, g->oplan);
