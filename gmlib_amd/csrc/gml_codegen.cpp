@@ -994,15 +994,18 @@ void HarvestEvents(std::vector<LaunchRec> &ev, double *acc)
    size_t keep = 0;
    for(size_t i = 0; i < ev.size(); i++)
    {
-      if(hipEventQuery(ev[i].end) == hipSuccess)
+      const hipError_t q = hipEventQuery(ev[i].end);
+      if(q == hipSuccess)
       {
          float ms = 0.f;
          if(hipEventElapsedTime(&ms, ev[i].beg, ev[i].end) == hipSuccess)
             *acc += ms * 1e-3;
          FreeRec(ev[i]);
       }
-      else
+      else if(q == hipErrorNotReady)
          ev[keep++] = ev[i];
+      // any other answer: the pair is unusable (its launch is not timed)
+      // and is neither kept nor returned to the pool
    }
    (void)hipGetLastError();
    ev.resize(keep);
