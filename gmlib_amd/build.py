@@ -32,6 +32,7 @@ SOURCES = [
     "gml_algebra.hip",
     "gml_mesh_kernels.hip",
     "gml_plan.cpp",
+    "gml_plan_gpu.hip",
     "gml_batch.hip",
     "gml_laplacian.cpp",
     "gml_serve.cpp",

@@ -347,6 +347,11 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op);
 void FreeSweepPlan(GmlCtx *g, SweepPlan *p);
 StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows);
 void FreeStagePlan(GmlCtx *g, StagePlan *p);
+// GPU builders of the element-tile and staging plans (gml_plan_gpu.hip)
+int GpuTilePlan(GmlCtx *g, const int *ele, long long n, int w, int wq, long long nv, int TR, unsigned int **tofs,
+                int **tlist, unsigned int **lid, long long *sum, int *maxn);
+int GpuStagePlan(GmlCtx *g, const int *ent, const unsigned int *off, const int *deg, long long nrows, int grp,
+                 size_t nslot, unsigned int **tofs, int **tlist, unsigned short *lidx, long long *sum, int *maxn);
 // the current element -> vertex tile plan of element kind typ, rebuilt when
 // the element table changed; nullptr when the kind cannot be staged
 DownPlan *GetDownPlan(GmlCtx *g, int typ);

@@ -224,73 +224,28 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
 StagePlan *NewStagePlan(GmlCtx *g, int Src, int Dst, int grp, long long nrows)
 {
    Uplink *u = &g->up[Src][Dst];
-   if(!u->BalIdx || grp < 64 || grp > 256 || (grp & 63) || nrows < 0 || nrows > g->NmbEle[Src]) return nullptr;
-   const long long nsrc = g->NmbEle[Src], nchunk = (nsrc + 63) / 64;
-   std::vector<int> deg((size_t)nsrc);
-   std::vector<unsigned int> off((size_t)nchunk + 1);
-   std::vector<int> ent((size_t)u->nslab * 64 + 64);
-   if(!DevCopy(g, deg.data(), g->dat[u->DegIdx].dev, (size_t)nsrc * 4, 2)
-   || !DevCopy(g, off.data(), g->dat[u->OffIdx].dev, off.size() * 4, 2)
-   || !DevCopy(g, ent.data(), g->dat[u->BalIdx].dev, ent.size() * 4, 2))
-      return nullptr;
-   const long long ntile = (nrows + grp - 1) / grp;
-   std::vector<std::vector<int>> tl((size_t)ntile);
-   std::vector<unsigned short> lidx((size_t)u->nslab * 64 + 64, 0);
-   std::atomic<bool> fits(true);
-   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-   std::vector<std::thread> th;
-   for(unsigned w = 0; w < nth; w++)
-      th.emplace_back([&, w]() {
-         std::vector<int> ts;
-         for(long long t = w; t < ntile; t += nth)
-         {
-            const long long r0 = t * grp, r1 = std::min(nrows, r0 + grp);
-            ts.clear();
-            for(long long r = r0; r < r1; r++)
-               for(int k = 0; k < deg[r]; k++)
-                  ts.push_back(ent[((size_t)off[r >> 6] + k) * 64 + (r & 63)] >> 4);
-            std::sort(ts.begin(), ts.end());
-            ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
-            if(ts.size() > 65535) { fits = false; continue; }
-            for(long long r = r0; r < r1; r++)
-               for(int k = 0; k < deg[r]; k++)
-               {
-                  const size_t sl = ((size_t)off[r >> 6] + k) * 64 + (r & 63);
-                  lidx[sl] = (unsigned short)(std::lower_bound(ts.begin(), ts.end(), ent[sl] >> 4) - ts.begin());
-               }
-            tl[t] = ts;
-         }
-      });
-   for(auto &x : th) x.join();
-   if(!fits) return nullptr;
+   if(!u->BalIdx || grp < 64 || grp > 256 || (grp & 63) || nrows < 1 || nrows > g->NmbEle[Src]) return nullptr;
    StagePlan *p = new StagePlan();
    p->grp = grp;
    p->nrows = nrows;
-   p->ntile = ntile;
-   std::vector<unsigned int> tofs((size_t)ntile + 1, 0);
-   for(long long t = 0; t < ntile; t++)
+   p->ntile = (nrows + grp - 1) / grp;
+   const size_t nslot = (size_t)u->nslab * 64 + 64;
+   p->lidx = (unsigned short *)DevAlloc(g, nslot * 2);
+   long long sum = 0;
+   int maxn = 0;
+   if(!p->lidx
+   || GpuStagePlan(g, (const int *)g->dat[u->BalIdx].dev, (const unsigned int *)g->dat[u->OffIdx].dev,
+                   (const int *)g->dat[u->DegIdx].dev, nrows, grp, nslot, &p->tofs, &p->tlist, p->lidx, &sum, &maxn) != 1)
    {
-      tofs[t + 1] = tofs[t] + (unsigned int)tl[t].size();
-      p->max_nt = std::max(p->max_nt, (int)tl[t].size());
-   }
-   std::vector<int> tlist((size_t)tofs[ntile] + 1);
-   for(long long t = 0; t < ntile; t++) std::copy(tl[t].begin(), tl[t].end(), tlist.begin() + tofs[t]);
-   p->tofs = (unsigned int *)DevAlloc(g, tofs.size() * 4);
-   p->tlist = (int *)DevAlloc(g, tlist.size() * 4);
-   p->lidx = (unsigned short *)DevAlloc(g, lidx.size() * 2);
-   if(!p->tofs || !p->tlist || !p->lidx
-   || !DevCopy(g, p->tofs, tofs.data(), tofs.size() * 4, 1)
-   || !DevCopy(g, p->tlist, tlist.data(), tlist.size() * 4, 1)
-   || !DevCopy(g, p->lidx, lidx.data(), lidx.size() * 2, 1))
-   {
-      FreeStagePlan(g, p);
+      DevFree(g, p->lidx);
+      delete p;
       return nullptr;
    }
-   p->bytes = tofs.size() * 4 + tlist.size() * 4 + lidx.size() * 2;
+   p->max_nt = maxn;
+   p->bytes = (size_t)(p->ntile + 1) * 4 + (size_t)sum * 4 + 4 + nslot * 2;
    g->MemSiz += p->bytes;
-   DbgPrint(g, "Staging plan %d->%d: %lld tiles of %d rows, %u tile entities (%.2f per entity), max %d per tile\n",
-            Src, Dst, ntile, grp, tofs[ntile], g->NmbEle[Dst] ? (double)tofs[ntile] / (double)g->NmbEle[Dst] : 0.,
-            p->max_nt);
+   DbgPrint(g, "Staging plan %d->%d: %lld tiles of %d rows, %lld tile entities (%.2f per entity), max %d per tile\n",
+            Src, Dst, p->ntile, grp, sum, g->NmbEle[Dst] ? (double)sum / (double)g->NmbEle[Dst] : 0., p->max_nt);
    return p;
 }
 
@@ -327,39 +282,6 @@ namespace gml {
 // ---------------------------------------------------------------------------
 // per tile of TR rows: the distinct ids and the rows' byte indices; false
 // if a tile exceeds 256 ids or holds an id outside [0, nv)
-static bool TilePlan(const std::vector<int> &ele, long long n, int w, int wq, long long nv, int TR,
-                     std::vector<std::vector<int>> *tl, std::vector<unsigned int> *lid)
-{
-   const long long ntile = (n + TR - 1) / TR;
-   tl->assign((size_t)ntile, std::vector<int>());
-   lid->assign((size_t)n * wq, 0u);
-   std::atomic<bool> ok(true);
-   const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-   std::vector<std::thread> th;
-   for(unsigned t = 0; t < nth; t++)
-      th.emplace_back([&, t]() {
-         std::vector<int> s;
-         for(long long k = t; k < ntile && ok; k += nth)
-         {
-            const long long a = k * TR, b = std::min(n, a + TR);
-            s.assign(ele.begin() + a * w, ele.begin() + b * w);
-            for(int v : s) if(v < 0 || v >= nv) { ok = false; break; }
-            std::sort(s.begin(), s.end());
-            s.erase(std::unique(s.begin(), s.end()), s.end());
-            if(s.size() > 256) { ok = false; break; }
-            for(long long r = a; r < b; r++)
-               for(int j = 0; j < w; j++)
-               {
-                  const unsigned int l = (unsigned int)(std::lower_bound(s.begin(), s.end(), ele[r * w + j]) - s.begin());
-                  (*lid)[(size_t)r * wq + (j >> 2)] |= l << (8 * (j & 3));
-               }
-            (*tl)[k] = s;
-         }
-      });
-   for(auto &x : th) x.join();
-   return ok;
-}
-
 static DownPlan *BuildDownPlan(GmlCtx *g, int typ)
 {
    const int idx = g->TypIdx[typ];
@@ -373,44 +295,26 @@ static DownPlan *BuildDownPlan(GmlCtx *g, int typ)
    p->src = d->dev;
    const long long nv = g->NmbEle[GmlVertices], n = p->nrows, w = p->w;
    if(w < 1 || w > 16 || n < 1 || !nv) { p->none = true; return p; }
-   std::vector<int> ele((size_t)n * w);
-   if(!DevCopy(g, ele.data(), d->dev, ele.size() * 4, 2)) { p->none = true; return p; }
-   for(int v : ele)
-      if(v < 0 || v >= nv) { p->none = true; return p; }
-   std::vector<std::vector<int>> tl;
-   std::vector<unsigned int> lid;
-   int TR = kDownTile;
-   while(TR >= 64 && !TilePlan(ele, n, (int)w, p->wq, nv, TR, &tl, &lid)) TR /= 2;
-   if(TR < 64) { p->none = true; return p; }
+   // the largest tile of at most 512 rows whose tiles each reach at most 256
+   // vertices (GpuTilePlan, one work-group per tile)
+   int TR = kDownTile, rc = 0;
+   long long sum = 0;
+   int maxn = 0;
+   while(TR * w > 8192 && TR > 64) TR /= 2;
+   for(; TR >= 64; TR /= 2)
+   {
+      rc = GpuTilePlan(g, (const int *)d->dev, n, (int)w, p->wq, nv, TR, &p->tofs, &p->tlist, &p->lid, &sum, &maxn);
+      if(rc != 0) break;
+   }
+   if(rc != 1) { p->none = true; return p; }
    p->rows = TR;
-   const long long ntile = (long long)tl.size();
-   std::vector<unsigned int> tofs((size_t)ntile + 1, 0);
-   for(long long k = 0; k < ntile; k++)
-   {
-      tofs[k + 1] = tofs[k] + (unsigned int)tl[k].size();
-      p->max_nv = std::max(p->max_nv, (int)tl[k].size());
-   }
-   std::vector<int> tlist((size_t)tofs[ntile] + 1, 0);
-   for(long long k = 0; k < ntile; k++) std::copy(tl[k].begin(), tl[k].end(), tlist.begin() + tofs[k]);
-   p->ntile = ntile;
-   p->sum_nv = tofs[ntile];
-   p->tofs = (unsigned int *)DevAlloc(g, tofs.size() * 4);
-   p->tlist = (int *)DevAlloc(g, tlist.size() * 4);
-   p->lid = (unsigned int *)DevAlloc(g, lid.size() * 4);
-   if(!p->tofs || !p->tlist || !p->lid
-   || !DevCopy(g, p->tofs, tofs.data(), tofs.size() * 4, 1)
-   || !DevCopy(g, p->tlist, tlist.data(), tlist.size() * 4, 1)
-   || !DevCopy(g, p->lid, lid.data(), lid.size() * 4, 1))
-   {
-      DevFree(g, p->tofs); DevFree(g, p->tlist); DevFree(g, p->lid);
-      p->tofs = nullptr; p->tlist = nullptr; p->lid = nullptr;
-      p->none = true;
-      return p;
-   }
-   p->bytes = tofs.size() * 4 + tlist.size() * 4 + lid.size() * 4;
+   p->ntile = (n + TR - 1) / TR;
+   p->sum_nv = sum;
+   p->max_nv = maxn;
+   p->bytes = (size_t)(p->ntile + 1) * 4 + (size_t)sum * 4 + 4 + (size_t)n * p->wq * 4;
    g->MemSiz += p->bytes;
    DbgPrint(g, "Element tile plan of %s: %lld tiles of %d rows, %lld tile vertices (%.2f per vertex), max %d per tile\n",
-            BalTypStr[typ], ntile, TR, p->sum_nv, (double)p->sum_nv / (double)nv, p->max_nv);
+            BalTypStr[typ], p->ntile, TR, p->sum_nv, (double)p->sum_nv / (double)nv, p->max_nv);
    return p;
 }
 
