@@ -352,6 +352,9 @@ int GpuTilePlan(GmlCtx *g, const int *ele, long long n, int w, int wq, long long
                 int **tlist, unsigned int **lid, long long *sum, int *maxn);
 int GpuStagePlan(GmlCtx *g, const int *ent, const unsigned int *off, const int *deg, long long nrows, int grp,
                  size_t nslot, unsigned int **tofs, int **tlist, unsigned short *lidx, long long *sum, int *maxn);
+int GpuSweepOpPlan(GmlCtx *g, const int *ent, const unsigned int *off, const int *deg, const int *tet, long long nv,
+                   int TV, unsigned int **vofs, int **vlist, unsigned int **noff, unsigned short **nbr, long long *nslab,
+                   long long *sum_halo, int *max_halo, long long *sum_ring);
 // the current element -> vertex tile plan of element kind typ, rebuilt when
 // the element table changed; nullptr when the kind cannot be staged
 DownPlan *GetDownPlan(GmlCtx *g, int typ);

@@ -40,6 +40,32 @@ SweepPlan *NewSweepPlan(GmlCtx *g, int TV, int op)
    const int ti = g->TypIdx[GmlTetrahedra];
    if(!u->BalIdx || !ti || TV < 64 || (TV & 63) || TV > 1024) return nullptr;
    Data *td = &g->dat[ti];
+   if(op)
+   {
+      // the operator form on the GPU for tiles of up to 256 vertices; the
+      // host below takes larger tiles and the tiles the GPU builder declines
+      SweepPlan *p = new SweepPlan();
+      long long sum_halo = 0;
+      int max_halo = 0;
+      const int rc = GpuSweepOpPlan(g, (const int *)g->dat[u->BalIdx].dev, (const unsigned int *)g->dat[u->OffIdx].dev,
+                                    (const int *)g->dat[u->DegIdx].dev, (const int *)td->dev, nv, TV, &p->vofs, &p->vlist,
+                                    &p->noff, &p->nbr, &p->nslab, &sum_halo, &max_halo, &p->sum_ring);
+      if(rc == 1)
+      {
+         p->TV = TV;
+         p->ntile = (nv + TV - 1) / TV;
+         p->max_nv = max_halo + TV;
+         p->sum_nv = sum_halo + nv;
+         p->bytes = (size_t)(p->ntile + 1) * 4 + (size_t)sum_halo * 4 + 4 + (size_t)((nv + 63) / 64 + 1) * 4
+                  + (size_t)p->nslab * 128 + 128;
+         g->MemSiz += p->bytes;
+         DbgPrint(g, "Operator sweep plan (GPU): %lld tiles of %d vertices, %lld halo vertices, %lld ring slabs, %zu MB\n",
+                  p->ntile, TV, sum_halo, p->nslab, p->bytes >> 20);
+         return p;
+      }
+      delete p;
+      if(rc < 0) return nullptr;
+   }
    if(!td->host_valid && !Download(g, ti)) return nullptr;
    const int *tet = (const int *)td->host;
    const long long nchunk = (nv + 63) / 64;

@@ -235,6 +235,190 @@ __global__ void __launch_bounds__(kPB) k_splan_fill(const int *__restrict__ ent,
    }
 }
 
+// --- operator-form sweep plan ------------------------------------------------
+// Per tile of TV vertices (TV <= 256): the halo (the vertices of the tile's
+// ball tets outside the tile, ascending) and, for every tile vertex, its
+// 1-ring as 16-bit entries (tile-local index << 4 | count), ascending local
+// index, counts above 15 split, the vertex itself only with the count beyond
+// its degree.  Dynamic LDS: the tile's tets, later its unique halo (kOpT
+// ints), then the halo candidates, later each thread's ring scratch (kOpH
+// ints).  A tile that needs more (a degree above kOpDeg, more tets or halo
+// candidates) sets flag 1 and the host builds the plan; a local index above
+// 4095 or a ring longer than 65535 entries sets flag 2 (no plan).
+constexpr int kOpT = 8192, kOpH = 16384, kOpDeg = 32;
+constexpr size_t kOpLds = (size_t)(kOpT + kOpH) * 4;
+
+// sum over i < m of f(i), exclusive prefix for each thread's chunk of
+// ceil(m / kPB) consecutive i; *total receives the whole sum
+template<class F>
+__device__ int chunk_scan(int m, F f, int *total)
+{
+   const int per = (m + kPB - 1) / kPB;
+   const int i0 = min(m, (int)threadIdx.x * per), i1 = min(m, i0 + per);
+   int c = 0;
+   for(int i = i0; i < i1; i++) c += f(i);
+   return block_scan(c, total);
+}
+
+// the tile's unique halo into U (ascending), its count returned (-1: the
+// tile takes the host path)
+__device__ int op_halo(const int *ent, const unsigned int *off, const int *deg, const int4 *tet, long long v0,
+                       long long v1, int *T, int *H, int *flags)
+{
+   __shared__ int rofs[kPB];
+   const int nr = (int)(v1 - v0);
+   const int d = (int)threadIdx.x < nr ? deg[v0 + threadIdx.x] : 0;
+   __shared__ int big;
+   if(threadIdx.x == 0) big = 0;
+   __syncthreads();
+   if(d > kOpDeg) big = 1;
+   int m = 0;
+   rofs[threadIdx.x] = block_scan(d, &m);
+   __syncthreads();
+   if(big || m > kOpT) { if(threadIdx.x == 0) atomicOr(flags, 1); return -1; }
+   if((int)threadIdx.x < nr)
+   {
+      const long long v = v0 + threadIdx.x;
+      const size_t base = (size_t)off[v >> 6] * 64 + (v & 63);
+      for(int k = 0; k < d; k++) T[rofs[threadIdx.x] + k] = ent[base + (size_t)k * 64] >> 4;
+   }
+   int np = next_pow2(max(m, 1));
+   for(int i = m + threadIdx.x; i < np; i += kPB) T[i] = INT_MAX;
+   __syncthreads();
+   bitonic(T, np);
+   // the outside vertices of each distinct tet, placed by a scan
+   auto outside = [&](int i) -> int {
+      if(i > 0 && T[i] == T[i - 1]) return 0;
+      const int4 q = tet[T[i]];
+      return (q.x < v0 || q.x >= v1) + (q.y < v0 || q.y >= v1) + (q.z < v0 || q.z >= v1) + (q.w < v0 || q.w >= v1);
+   };
+   int nc = 0;
+   int o = chunk_scan(m, outside, &nc);
+   if(nc > kOpH) { if(threadIdx.x == 0) atomicOr(flags, 1); return -1; }
+   {
+      const int per = (m + kPB - 1) / kPB;
+      const int i0 = min(m, (int)threadIdx.x * per), i1 = min(m, i0 + per);
+      for(int i = i0; i < i1; i++)
+      {
+         if(i > 0 && T[i] == T[i - 1]) continue;
+         const int4 q = tet[T[i]];
+         const int x[4] = {q.x, q.y, q.z, q.w};
+         for(int j = 0; j < 4; j++)
+            if(x[j] < v0 || x[j] >= v1) H[o++] = x[j];
+      }
+   }
+   np = next_pow2(max(nc, 1));
+   for(int i = nc + threadIdx.x; i < np; i += kPB) H[i] = INT_MAX;
+   __syncthreads();
+   bitonic(H, np);
+   // unique halo, compacted over the tets (no longer needed)
+   int nh = 0;
+   o = chunk_scan(nc, [&](int i) -> int { return i == 0 || H[i] != H[i - 1]; }, &nh);
+   if(nh > kOpT) { if(threadIdx.x == 0) atomicOr(flags, 1); return -1; }
+   {
+      const int per = (nc + kPB - 1) / kPB;
+      const int i0 = min(nc, (int)threadIdx.x * per), i1 = min(nc, i0 + per);
+      for(int i = i0; i < i1; i++)
+         if(i == 0 || H[i] != H[i - 1]) T[o++] = H[i];
+   }
+   __syncthreads();
+   return nh;
+}
+
+// the ring of tile vertex v0 + threadIdx.x: its entries into out[k * 64]
+// (fill) or only counted; returns the entry count (-1: an index over 4095)
+__device__ int op_ring(const int *ent, const unsigned int *off, const int *deg, const int4 *tet, long long v0,
+                       long long v1, int TV, const int *U, int nh, unsigned short *R, unsigned short *out)
+{
+   const long long v = v0 + threadIdx.x;
+   const size_t base = (size_t)off[v >> 6] * 64 + (v & 63);
+   const int d = deg[v];
+   int n = 0;
+   for(int k = 0; k < d; k++)
+   {
+      const int4 q = tet[ent[base + (size_t)k * 64] >> 4];
+      const int x[4] = {q.x, q.y, q.z, q.w};
+      for(int j = 0; j < 4; j++)
+      {
+         const int l = (x[j] >= v0 && x[j] < v1) ? (int)(x[j] - v0) : TV + lower(U, nh, x[j]);
+         if(l > 4095) return -1;
+         // insertion into the ascending scratch
+         int p = n++;
+         while(p > 0 && R[(p - 1) * kPB] > l) { R[p * kPB] = R[(p - 1) * kPB]; p--; }
+         R[p * kPB] = (unsigned short)l;
+      }
+   }
+   const int self = (int)(v - v0);
+   int e = 0;
+   for(int i = 0; i < n;)
+   {
+      int j = i;
+      while(j < n && R[j * kPB] == R[i * kPB]) j++;
+      int c = j - i;
+      if(R[i * kPB] == self) c = c > d ? c - d : 0;
+      for(; c > 0; c -= min(c, 15))
+      {
+         if(out) out[(size_t)e * 64] = (unsigned short)((R[i * kPB] << 4) | min(c, 15));
+         e++;
+      }
+      i = j;
+   }
+   return e;
+}
+
+__global__ void __launch_bounds__(kPB) k_opplan_count(const int *__restrict__ ent, const unsigned int *__restrict__ off,
+                                                      const int *__restrict__ deg, const int4 *__restrict__ tet,
+                                                      long long nv, int TV, unsigned int *__restrict__ hcnt,
+                                                      unsigned int *__restrict__ rlen, int *flags)
+{
+   extern __shared__ int lds[];
+   int *T = lds, *H = lds + kOpT;
+   const long long v0 = (long long)blockIdx.x * TV, v1 = min(nv, v0 + TV);
+   const int nh = op_halo(ent, off, deg, tet, v0, v1, T, H, flags);
+   if(nh < 0) return;
+   if(threadIdx.x == 0) hcnt[blockIdx.x] = (unsigned int)nh;
+   if((long long)threadIdx.x < v1 - v0)
+   {
+      unsigned short *R = reinterpret_cast<unsigned short *>(H) + threadIdx.x;
+      const int e = op_ring(ent, off, deg, tet, v0, v1, TV, T, nh, R, nullptr);
+      if(e < 0 || e > 65535) atomicOr(flags, 2);
+      rlen[v0 + threadIdx.x] = (unsigned int)max(e, 0);
+   }
+}
+
+__global__ void __launch_bounds__(kPB) k_opplan_fill(const int *__restrict__ ent, const unsigned int *__restrict__ off,
+                                                     const int *__restrict__ deg, const int4 *__restrict__ tet,
+                                                     long long nv, int TV, const unsigned int *__restrict__ vofs,
+                                                     int *__restrict__ vlist, const unsigned int *__restrict__ noff,
+                                                     unsigned short *__restrict__ nbr, int *flags)
+{
+   extern __shared__ int lds[];
+   int *T = lds, *H = lds + kOpT;
+   const long long v0 = (long long)blockIdx.x * TV, v1 = min(nv, v0 + TV);
+   const int nh = op_halo(ent, off, deg, tet, v0, v1, T, H, flags);
+   if(nh < 0) return;
+   const unsigned int o = vofs[blockIdx.x];
+   for(int i = threadIdx.x; i < nh; i += kPB) vlist[o + i] = T[i];
+   if((long long)threadIdx.x < v1 - v0)
+   {
+      const long long v = v0 + threadIdx.x;
+      unsigned short *R = reinterpret_cast<unsigned short *>(H) + threadIdx.x;
+      op_ring(ent, off, deg, tet, v0, v1, TV, T, nh, R, nbr + (size_t)noff[v >> 6] * 64 + (v & 63));
+   }
+}
+
+// per 64-vertex slab: the longest ring (wcnt[c]) and the ring total (ssum[c])
+__global__ void k_opplan_slabs(const unsigned int *__restrict__ rlen, long long nv, long long nchunk,
+                               unsigned int *__restrict__ wcnt, unsigned int *__restrict__ ssum)
+{
+   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(c >= nchunk) return;
+   unsigned int w = 0, t = 0;
+   for(long long v = c * 64; v < min(nv, c * 64 + 64); v++) { w = max(w, rlen[v]); t += rlen[v]; }
+   wcnt[c] = w;
+   ssum[c] = t;
+}
+
 bool ok_(hipError_t e) { return e == hipSuccess; }
 
 // tile offsets from the per-tile counts cnt[0..ntile) (cnt[ntile] = 0): the
@@ -355,6 +539,81 @@ int GpuStagePlan(GmlCtx *g, const int *ent, const unsigned int *off, const int *
       return -1;
    }
    return 1;
+}
+
+// Operator-form sweep plan on the GPU (tiles of TV <= 256 vertices).
+// Returns 1 with vofs / vlist / noff / nbr allocated and the counts set, 0 if
+// a tile exceeds the GPU builder's limits (the host builds the plan), -1 if
+// the plan cannot exist (a tile-local index over 4095, a ring over 65535
+// entries) or the GPU work failed.
+int GpuSweepOpPlan(GmlCtx *g, const int *ent, const unsigned int *off, const int *deg, const int *tet, long long nv,
+                   int TV, unsigned int **vofs, int **vlist, unsigned int **noff, unsigned short **nbr, long long *nslab,
+                   long long *sum_halo, int *max_halo, long long *sum_ring)
+{
+   *vofs = nullptr; *vlist = nullptr; *noff = nullptr; *nbr = nullptr;
+   if(nv < 1 || TV < 64 || TV > kPB || (TV & 63)) return 0;
+   static bool attr = false;
+   if(!attr)
+   {
+      if(hipFuncSetAttribute((const void *)k_opplan_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOpLds) != hipSuccess
+      || hipFuncSetAttribute((const void *)k_opplan_fill, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOpLds) != hipSuccess)
+      {
+         (void)hipGetLastError();
+         return 0;
+      }
+      attr = true;
+   }
+   const long long ntile = (nv + TV - 1) / TV, nchunk = (nv + 63) / 64;
+   hipStream_t s = g->stream;
+   unsigned int *hcnt = (unsigned int *)DevAlloc(g, (size_t)(ntile + 1) * 4);
+   unsigned int *rlen = (unsigned int *)DevAlloc(g, (size_t)nv * 4);
+   unsigned int *wcnt = (unsigned int *)DevAlloc(g, (size_t)(nchunk + 1) * 4);
+   unsigned int *ssum = (unsigned int *)DevAlloc(g, (size_t)(nchunk + 1) * 4);
+   unsigned int *soff = (unsigned int *)DevAlloc(g, (size_t)(nchunk + 1) * 4);
+   int *flags = (int *)DevAlloc(g, 4);
+   *vofs = (unsigned int *)DevAlloc(g, (size_t)(ntile + 1) * 4);
+   *noff = (unsigned int *)DevAlloc(g, (size_t)(nchunk + 1) * 4);
+   int f = 0, rc = -1;
+   int wmax = 0;
+   long long rs = 0;
+   bool ok = hcnt && rlen && wcnt && ssum && soff && flags && *vofs && *noff
+          && ok_(hipMemsetAsync(hcnt, 0, (size_t)(ntile + 1) * 4, s)) && ok_(hipMemsetAsync(wcnt, 0, (size_t)(nchunk + 1) * 4, s))
+          && ok_(hipMemsetAsync(ssum, 0, (size_t)(nchunk + 1) * 4, s)) && ok_(hipMemsetAsync(flags, 0, 4, s));
+   if(ok)
+   {
+      hipLaunchKernelGGL(k_opplan_count, dim3((unsigned int)ntile), dim3(kPB), kOpLds, s, ent, off, deg, (const int4 *)tet,
+                         nv, TV, hcnt, rlen, flags);
+      ok = ok_(hipGetLastError()) && DevCopy(g, &f, flags, 4, 2);
+   }
+   if(ok && f == 0)
+   {
+      hipLaunchKernelGGL(k_opplan_slabs, dim3((unsigned int)((nchunk + 255) / 256)), dim3(256), 0, s, rlen, nv, nchunk, wcnt, ssum);
+      ok = ok_(hipGetLastError()) && ScanCounts(g, hcnt, ntile, *vofs, sum_halo, max_halo)
+        && ScanCounts(g, wcnt, nchunk, *noff, nslab, &wmax) && ScanCounts(g, ssum, nchunk, soff, &rs, &wmax);
+      *sum_ring = rs;
+      if(ok)
+      {
+         *vlist = (int *)DevAlloc(g, (size_t)*sum_halo * 4 + 4);
+         *nbr = (unsigned short *)DevAlloc(g, (size_t)*nslab * 128 + 128);
+         ok = *vlist && *nbr && ok_(hipMemsetAsync(*nbr, 0, (size_t)*nslab * 128 + 128, s));
+      }
+      if(ok)
+      {
+         hipLaunchKernelGGL(k_opplan_fill, dim3((unsigned int)ntile), dim3(kPB), kOpLds, s, ent, off, deg, (const int4 *)tet,
+                            nv, TV, *vofs, *vlist, *noff, *nbr, flags);
+         ok = ok_(hipGetLastError()) && ok_(hipStreamSynchronize(s));
+      }
+      rc = ok ? 1 : -1;
+   }
+   else if(ok)
+      rc = (f & 2) ? -1 : 0;
+   DevFree(g, hcnt); DevFree(g, rlen); DevFree(g, wcnt); DevFree(g, ssum); DevFree(g, soff); DevFree(g, flags);
+   if(rc != 1)
+   {
+      DevFree(g, *vofs); DevFree(g, *vlist); DevFree(g, *noff); DevFree(g, *nbr);
+      *vofs = nullptr; *vlist = nullptr; *noff = nullptr; *nbr = nullptr;
+   }
+   return rc;
 }
 
 } // namespace gml
