@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 namespace gml {
@@ -784,6 +785,174 @@ __global__ void __launch_bounds__(256) k_flag_emit(const uint4 *__restrict__ fla
       if((w[i >> 2] >> (8 * (i & 3))) & 0xff) sel[at++] = p0 + i;
 }
 
+// First appearances through the vertex -> tet ball, for a mesh whose only
+// element kind is the tet and that has no table of the extracted kind yet:
+// item j of tet e (an edge or a face) appears first at position e * ni + j
+// iff no tet of smaller index holds all its nodes.  Every tet holding the
+// item holds its lowest node a, so the ball of a holds them all.  The thread
+// of vertex a walks its ball once in ascending tet index (the ball is stored
+// in descending order) and keeps the items whose lowest node is a that it has
+// met, as keys in registers (the other node of an edge; the two other nodes
+// of a face, lower one in the high word): an item is first iff its key is
+// new (the list is searched 8 keys at a time, up to the keys in use).
+// Every position is written once, 0 or 1.  The low four bits of a ball
+// entry are a's local index in the tet, so the candidates of a tet are found
+// without a search: the edges {a, y} and the faces opposite each other node
+// y, item numbers from map (edges: nibble la * 4 + o for locals la, o; faces:
+// nibble o for the face without local o).  A vertex with more than S
+// distinct items splits them into P classes (key hash mod P) and walks its
+// ball once per class, P doubling until each class fits.  A tet that repeats
+// a vertex, or a ball entry that does not name a, sets *fallback (the
+// general path then runs).
+constexpr int kBallMaxDeg = 256;   // wider balls take the general path
+constexpr int kBallMaxPass = 64;
+
+template <typename T, int S>
+__device__ __forceinline__ bool seen_has(const T (&seen)[S], int cnt, T key)
+{
+   bool found = false;
+#pragma unroll
+   for(int c0 = 0; c0 < S; c0 += 8)
+      if(c0 < cnt)
+#pragma unroll
+         for(int s = c0; s < c0 + 8; s++) found |= seen[s] == key;
+   return found;
+}
+
+// insert at the front (cnt < S entries in use; entries from cnt on hold ~0,
+// which no key equals): every index is a constant, so the list stays in
+// registers; the chunks above the ones in use are not moved
+template <typename T, int S>
+__device__ __forceinline__ void seen_put(T (&seen)[S], int cnt, T key)
+{
+#pragma unroll
+   for(int c0 = S - 8; c0 >= 0; c0 -= 8)
+      if(c0 <= cnt)
+#pragma unroll
+         for(int s = c0 + 7; s >= c0; s--) seen[s] = s == 0 ? key : seen[s - 1];
+}
+
+template <int KD, int S>
+__global__ void __launch_bounds__(256) k_item_first_ball(long long nv, const int *__restrict__ deg,
+                                                         const unsigned int *__restrict__ off,
+                                                         const int *__restrict__ ent, const int4 *__restrict__ tet,
+                                                         int ni, unsigned long long map,
+                                                         unsigned char *__restrict__ flag, int *fallback)
+{
+   const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if(a >= nv) return;
+   const int d = deg[a], ia = (int)a;
+   const size_t base = (size_t)off[a >> 6] * 64 + (a & 63);
+   for(int P = 1; P <= kBallMaxPass; P <<= 1)
+   {
+      bool over = false;
+      for(int p = 0; p < P && !over; p++)
+      {
+         typedef typename std::conditional<KD == 2, unsigned int, unsigned long long>::type Key;
+         Key seen[S];
+#pragma unroll
+         for(int s = 0; s < S; s++) seen[s] = ~(Key)0;
+         int cnt = 0;
+         int wn = d > 0 ? ent[base + (size_t)(d - 1) * 64] : 0;
+         int4 qn = d > 0 ? tet[wn >> 4] : make_int4(0, 0, 0, 0);
+         for(int k = d - 1; k >= 0 && !over; k--)
+         {
+            const int w = wn;
+            const int4 q = qn;
+            if(k > 0)
+            {
+               wn = ent[base + (size_t)(k - 1) * 64];
+               qn = tet[wn >> 4];
+            }
+            const long long e = w >> 4;
+            const int la = w & 15;
+            const int xa = la == 0 ? q.x : la == 1 ? q.y : la == 2 ? q.z : q.w;
+            if(la > 3 || xa != ia || q.x == q.y || q.x == q.z || q.x == q.w || q.y == q.z || q.y == q.w || q.z == q.w)
+            {
+               atomicOr(fallback, 1);
+               return;
+            }
+            // the other three nodes in local order, and their local indices
+            const int y[3] = {la == 0 ? q.y : q.x, la <= 1 ? q.z : q.y, la <= 2 ? q.w : q.z};
+            const int o[3] = {la == 0 ? 1 : 0, la <= 1 ? 2 : 1, la <= 2 ? 3 : 2};
+#pragma unroll
+            for(int i = 0; i < 3; i++)
+            {
+               Key key;
+               int j;
+               bool mine;
+               if constexpr(KD == 2)
+               {
+                  key = (unsigned int)y[i];
+                  j = (int)((map >> (4 * (la * 4 + o[i]))) & 15);
+                  mine = y[i] > ia && ((unsigned int)y[i] & (unsigned int)(P - 1)) == (unsigned int)p;
+               }
+               else
+               {
+                  const int b = y[(i + 1) % 3], c = y[(i + 2) % 3];
+                  const int lo = min(b, c), hi = max(b, c);
+                  key = ((unsigned long long)(unsigned int)lo << 32) | (unsigned int)hi;
+                  j = (int)((map >> (4 * o[i])) & 15);
+                  mine = lo > ia && (((unsigned int)lo + (unsigned int)hi) & (unsigned int)(P - 1)) == (unsigned int)p;
+               }
+               if(mine && !over)
+               {
+                  const bool found = seen_has<Key, S>(seen, cnt, key);
+                  if(!found)
+                  {
+                     if(cnt == S)
+                        over = true;
+                     else
+                        seen_put<Key, S>(seen, cnt++, key);
+                  }
+                  if(!over) flag[e * ni + j] = found ? 0 : 1;
+               }
+            }
+         }
+      }
+      if(!over) return;
+   }
+   atomicOr(fallback, 1);
+}
+
+// item numbers of the tet's local edges (deg 2: nibble la * 4 + o for each
+// pair of distinct locals) or faces (deg 3: nibble o for the face without
+// local o) from its local item table; false when the table is not the six
+// edges or four triangles of a tet
+static bool BallItemMap(const LocalItems &li, int deg, unsigned long long *map)
+{
+   unsigned long long m = 0;
+   unsigned int set = 0;
+   if(deg == 2 && li.n == 6)
+      for(int j = 0; j < 6; j++)
+      {
+         const int p = li.nod[j][0], q = li.nod[j][1];
+         if(p < 0 || p > 3 || q < 0 || q > 3 || p == q) return false;
+         for(int t : {p * 4 + q, q * 4 + p})
+         {
+            if(set & (1u << t)) return false;
+            set |= 1u << t;
+            m |= (unsigned long long)j << (4 * t);
+         }
+      }
+   else if(deg == 3 && li.n == 4)
+      for(int j = 0; j < 4; j++)
+      {
+         unsigned int has = 0;
+         for(int i = 0; i < 3; i++)
+            if(li.nod[j][i] >= 0 && li.nod[j][i] <= 3) has |= 1u << li.nod[j][i];
+         if(__builtin_popcount(has) != 3) return false;
+         const int r = __builtin_ctz(~has & 15u);
+         if(set & (1u << r)) return false;
+         set |= 1u << r;
+         m |= (unsigned long long)j << (4 * r);
+      }
+   else
+      return false;
+   *map = m;
+   return true;
+}
+
 // the sources of an extraction and the first item position of each
 constexpr int kMaxSources = 8;
 struct SourceSet {
@@ -904,12 +1073,14 @@ __global__ void k_neighbours(long long n, const unsigned long long *__restrict__
 int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
 {
    std::vector<Source> src;
+   std::vector<int> kinds;   // element kind of each source (-1: the existing table)
    long long total = 0, nold = 0;
    const int old = g->TypIdx[typ], oref = g->RefIdx[typ];
    if(old)
    {
       const Data *d = &g->dat[old];
       src.push_back({(const int *)d->dev, d->NmbLin, d->ItmLen, Itself(deg)});
+      kinds.push_back(-1);
       nold = d->NmbLin;
    }
    const int t0 = deg == 2 ? GmlEdges + 1 : GmlTetrahedra, t1 = deg == 2 ? GmlMaxEleTyp - 1 : GmlHexahedra;
@@ -918,7 +1089,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       if(!g->TypIdx[t] || !g->NmbEle[t]) continue;
       const Data *d = &g->dat[g->TypIdx[t]];
       const Source s = {(const int *)d->dev, d->NmbLin, d->ItmLen, ItemsOf(t, deg)};
-      if(s.li.n) src.push_back(s);
+      if(s.li.n) { src.push_back(s); kinds.push_back(t); }
    }
    for(const Source &s : src) total += s.ne * s.li.n;
    if(total >= INT_MAX)
@@ -956,10 +1127,74 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    unsigned char *head = nullptr, *flag = nullptr, *defer = nullptr;
    void *xtmp = nullptr, *brec = nullptr;
    int rc = -1, hbad = 0, m = 0;
+   const long long nchunk = (total + kFlagChunk - 1) / kFlagChunk;
+   // 0. a mesh of tets only and no table of the kind yet: first appearances
+   // through the vertex -> tet ball (built if needed), flagged per position
+   // by k_item_first_ball; the general path below otherwise, and when the
+   // ball cannot be built, is wider than kBallMaxDeg or a tet repeats a vertex
+   bool viaball = false;
+   unsigned long long bmap = 0;
+   if(!old && src.size() == 1 && kinds[0] == GmlTetrahedra && src[0].len == 4 && BallItemMap(src[0].li, deg, &bmap)
+   && !g->host_only
+   && BuildUplink(g, GmlVertices, GmlTetrahedra) && g->up[GmlVertices][GmlTetrahedra].MaxDeg <= kBallMaxDeg)
+   {
+      const Uplink &u = g->up[GmlVertices][GmlTetrahedra];
+      int *fb = nullptr, hfb = 1;
+      size_t t3 = 0;
+      bool fok = DAlloc(g, &fb, 4) && ok_(hipMemsetAsync(fb, 0, 4, s)) && DAlloc(g, &flag, nchunk * kFlagChunk)
+              && ok_(hipMemsetAsync(flag + total, 0, nchunk * kFlagChunk - total, s))
+              && DAlloc(g, &ccnt, (nchunk + 1) * 4) && DAlloc(g, &coff, (nchunk + 1) * 4)
+              && ok_(hipMemsetAsync(ccnt + nchunk, 0, 4, s))
+              && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, ccnt, coff, (int)(nchunk + 1), s))
+              && DAlloc(g, &xtmp, t3);
+      tim[1] = clock();
+      if(fok)
+      {
+         const dim3 grid((unsigned int)((nv + 255) / 256));
+         const int *bdeg = (const int *)g->dat[u.DegIdx].dev, *bent = (const int *)g->dat[u.BalIdx].dev;
+         const unsigned int *boff = (const unsigned int *)g->dat[u.OffIdx].dev;
+         if(deg == 2)
+            hipLaunchKernelGGL((k_item_first_ball<2, 16>), grid, dim3(256), 0, s, nv, bdeg, boff, bent,
+                               (const int4 *)src[0].ele, src[0].li.n, bmap, flag, fb);
+         else
+            hipLaunchKernelGGL((k_item_first_ball<3, 32>), grid, dim3(256), 0, s, nv, bdeg, boff, bent,
+                               (const int4 *)src[0].ele, src[0].li.n, bmap, flag, fb);
+         fok = ok_(hipGetLastError()) && ok_(hipMemcpyAsync(&hfb, fb, 4, hipMemcpyDeviceToHost, s))
+            && ok_(hipStreamSynchronize(s));
+      }
+      tim[2] = clock();
+      if(fok && !hfb)
+      {
+         hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, ccnt);
+         size_t t = t3;
+         fok = ok_(hipGetLastError()) && ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, ccnt, coff, (int)(nchunk + 1), s))
+            && ok_(hipMemcpyAsync(&m, coff + nchunk, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s))
+            && DAlloc(g, &sel, (size_t)m * 4 + 4);
+         if(fok)
+         {
+            hipLaunchKernelGGL(k_flag_emit, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, coff, sel);
+            fok = ok_(hipGetLastError());
+         }
+         viaball = fok;
+      }
+      ItemFree(g, fb);
+      if(!viaball)
+      {
+         (void)hipGetLastError();
+         for(void *q : {(void *)flag, (void *)ccnt, (void *)coff, xtmp, (void *)sel}) ItemFree(g, q);
+         flag = nullptr; ccnt = nullptr; coff = nullptr; xtmp = nullptr; sel = nullptr;
+         m = 0;
+      }
+   }
+   bool bucketed = false;
+   bool ok = true;
+   long long n2 = 0;
+   if(!viaball)
+   {
    // 1. block-local first appearances of every source's items
    long long maxgrid = 1;
    for(const Source &sr : src) maxgrid = std::max(maxgrid, (sr.ne + kDedupThreads - 1) / kDedupThreads);
-   bool ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
+   ok = DAlloc(g, &cnt, 16) && ok_(hipMemsetAsync(cnt, 0, 16, s)) && DAlloc(g, &nsel, 4)
           && AllocKeys(g, total, kf, &it) && DAlloc(g, &pos, total * 4) && DAlloc(g, &defer, maxgrid);
    tim[1] = clock();
    long long at = 0;
@@ -999,13 +1234,11 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       fprintf(stderr, "gml: extraction: an element references a vertex outside 0..%lld\n", nv - 1);
       ok = false;
    }
-   const long long n2 = (long long)hc[0];
+   n2 = (long long)hc[0];
    // 2. bucketed path: the block-distinct items by lowest node, each bucket
    // scanned for its keys' smallest positions, those positions selected in
    // order; the table rows are rebuilt from the elements at them
-   bool bucketed = false;
    size_t tb = 0;
-   const long long nchunk = (total + kFlagChunk - 1) / kFlagChunk;
    if(ok && nbk < INT_MAX)
    {
       // one temporary for the bucket reduce and the two scans
@@ -1085,6 +1318,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
          ok = ok_(hipcub::DeviceRadixSort::SortPairs(it.tmp, t, hsel, p2, usel, u2, m, 0, bits_for(total), s));
       }
    }
+   }
    tim[3] = clock();
    // 3. the new table (created before the old one is dropped: the old refs are read)
    if(ok)
@@ -1095,7 +1329,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
       {
          const Data *d = &g->dat[nidx];
          const int *orf = oref ? (const int *)g->dat[oref].dev : nullptr;
-         if(bucketed)
+         if(bucketed || viaball)
             hipLaunchKernelGGL(k_item_write_pos, blocks(m), dim3(kB), 0, s, (long long)m, sel, ss, kf, nv, nold, orf,
                                d->ItmLen, (int *)d->dev, (int *)g->dat[g->RefIdx[typ]].dev);
          else
@@ -1120,7 +1354,7 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    tim[4] = clock();
    DbgPrint(g, "Extraction of %lld items (%lld after the block pass, %d distinct, %s path): allocation %.1f ms, "
             "block pass %.1f ms, first appearances %.1f ms, table and frees %.1f ms\n", total, n2, m,
-            bucketed ? "bucketed" : "sorted", 1e3 * (tim[1] - tim[0]), 1e3 * (tim[2] - tim[1]),
+            viaball ? "vertex-ball" : (bucketed ? "bucketed" : "sorted"), 1e3 * (tim[1] - tim[0]), 1e3 * (tim[2] - tim[1]),
             1e3 * (tim[3] - tim[2]), 1e3 * (tim[4] - tim[3]));
    return rc;
 }

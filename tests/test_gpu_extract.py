@@ -1,5 +1,7 @@
-"""GmlExtractEdges / GmlExtractFaces on a device handle (radix sorts of
-packed node keys on the GPU) against the same calls on a host-only handle
+"""GmlExtractEdges / GmlExtractFaces on a device handle (a mesh of tets only
+through the vertex -> tet ball; otherwise packed node keys, deduplicated per
+work-group, then bucketed by lowest node or sorted) against the same calls on
+a host-only handle
 (the ordered-map form, itself checked against a brute-force restatement of
 sources/gmlib3.c:3719-4171 in test_extract.py) and against that restatement
 directly: bit-identical tables, refs and counts.  Covers the one-word key
@@ -264,16 +266,20 @@ def extract_logged(g, what, capfd):
     n = g.L.GmlExtractEdges(g.h) if what == "edges" else g.L.GmlExtractFaces(g.h)
     g.L.GmlDebugOff(g.h)
     out = capfd.readouterr().out
-    path = "bucketed" if "bucketed path" in out else "sorted" if "sorted path" in out else None
+    path = ("vertex-ball" if "vertex-ball path" in out else "bucketed" if "bucketed path" in out
+            else "sorted" if "sorted path" in out else None)
     return n, path
 
 
+@pytest.mark.parametrize("old", [False, True])
 @pytest.mark.parametrize("shuffle", [False, True])
-def test_gpu_extract_cube_tables_against_oracle(capfd, shuffle):
+def test_gpu_extract_cube_tables_against_oracle(capfd, shuffle, old):
     # a 40^3 cube (384,000 tets; 2.3M edge items, 1.5M face items), in its
     # Hilbert numbering and with vertices and tets shuffled (buckets then
     # spread over the whole vertex range): tables and order bit-identical to
-    # the oracle's restatement of the reference hash
+    # the oracle's restatement of the reference hash.  A mesh of tets only
+    # takes the vertex-ball path; with a two-row table of the kind already
+    # there (kept first, with its ref) the bucketed path runs
     crd, ref, tet = G.cube_mesh(40, 0.2, 17)
     if shuffle:
         rng = np.random.default_rng(3)
@@ -282,15 +288,23 @@ def test_gpu_extract_cube_tables_against_oracle(capfd, shuffle):
         inv[pv] = np.arange(len(crd))
         crd, ref = np.ascontiguousarray(crd[pv]), np.ascontiguousarray(ref[pv])
         tet = np.ascontiguousarray(inv[tet[rng.permutation(len(tet))]].astype(np.int32))
-    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
-    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    oe = np.array([[tet[7, 2], tet[7, 0]], [tet[11, 1], tet[11, 3]]], np.int32) if old else None
+    of = np.array([[tet[9, 3], tet[9, 1], tet[9, 2]], [0, 1, 2]], np.int32) if old else None
+    oref = np.array([31, 32], np.int32) if old else None
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)], oe, oref)
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)], of, oref)
     with G.Gml() as g:
         g.set_mesh(crd, ref, tet)
+        if old:
+            set_block(g, G.GmlEdges, oe, oref)
         ne, pe = extract_logged(g, "edges", capfd)
+        if old:
+            set_block(g, G.GmlTriangles, of, oref)
         nf, pf = extract_logged(g, "faces", capfd)
         e = get_block(g, G.GmlEdges, ne, 2)
         f = get_block(g, G.GmlTriangles, nf, 3)
-    assert pe == "bucketed" and pf == "bucketed"
+    path = "bucketed" if old else "vertex-ball"
+    assert pe == path and pf == path
     assert ne == len(want_e[0]) and nf == len(want_f[0])
     assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
     assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
@@ -321,16 +335,60 @@ def test_gpu_extract_two_word_keys_against_oracle(capfd):
     # the 40^3 cube's node ids spread 64 apart (4.4M vertex slots, 23-bit ids):
     # triangle keys take two words; a work-group's nodes still span a narrow
     # range, so the block pass packs them relative to its lowest node (groups
-    # whose range is too wide take the round-based pass); tables bit-identical
-    # to the oracle
+    # whose range is too wide take the round-based pass); a two-row triangle
+    # table already there keeps the call off the vertex-ball path; tables
+    # bit-identical to the oracle
     crd, ref, tet = G.cube_mesh(40, 0.2, 21)
     tet = np.ascontiguousarray(tet * 64)
     nv = len(crd) * 64
-    want = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    of = np.array([[tet[3, 0], tet[3, 2], tet[3, 1]], [tet[8, 1], tet[8, 0], tet[8, 3]]], np.int32)
+    oref = np.array([9, 10], np.int32)
+    want = O.extract_items(3, [(G.GmlTetrahedra, tet)], of, oref)
     with G.Gml() as g:
         set_vertices(g, nv)
         set_block(g, G.GmlTetrahedra, tet, np.zeros(len(tet), np.int32))
+        set_block(g, G.GmlTriangles, of, oref)
         nf, pf = extract_logged(g, "faces", capfd)
         f = get_block(g, G.GmlTriangles, nf, 3)
     assert pf == "bucketed" and nf == len(want[0])
     assert np.array_equal(f[0], want[0]) and np.array_equal(f[1], want[1])
+
+
+@pytest.mark.parametrize("nt", [100, 250])
+def test_gpu_extract_wide_ball_fan(capfd, nt):
+    # a fan of nt tets (0, i, i+1, i+2): vertex 0's ball is wider than the
+    # rows the vertex-ball kernel keeps in LDS (the rest are read from the
+    # table) and still within its degree bound; tables equal the oracle's
+    tet = np.array([[0, i, i + 1, i + 2] for i in range(1, nt + 1)], np.int32)
+    nv = nt + 3
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        set_vertices(g, nv)
+        set_block(g, G.GmlTetrahedra, tet, np.zeros(nt, np.int32))
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe == "vertex-ball" and pf == "vertex-ball"
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+def test_gpu_extract_degenerate_tet_leaves_ball_path(capfd):
+    # a 6^3 cube with one tet that repeats a vertex: the vertex-ball kernel
+    # reports it and the general path builds the tables, equal to the oracle's
+    crd, ref, tet = G.cube_mesh(6, 0.1, 5)
+    tet = tet.copy()
+    tet[len(tet) // 2, 3] = tet[len(tet) // 2, 1]
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe in ("bucketed", "sorted") and pf in ("bucketed", "sorted")
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
