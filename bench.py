@@ -1327,15 +1327,15 @@ def run_extract(args):
         "tets_per_s": round(T / dt, 1),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole extraction (key, radix sort, select, write kernels)",
+                     "kernel": "whole extraction (vertex-ball first appearances, flag compaction, row writes)",
                      "bytes_per_launch": alg, "avg_us": round(dt * 1e6, 1)},
         "cpu_baseline": None,
     }
     # bytes moved per step by all the step's kernels, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
-    tf = os.path.join(ROOT, "profiles", "traffic_extract256.json")
-    if n == 256 and os.path.exists(tf):
-        out["roofline"]["traffic"] = json.load(open(tf)).get("extract_step")
+    trf = os.path.join(ROOT, "profiles", "traffic_extract256.json")
+    if n == 256 and os.path.exists(trf):
+        out["roofline"]["traffic"] = json.load(open(trf)).get("extract_step")
     g.close()
     if args.cpu_seconds > 0:
         from tests import oracle_lib as O

@@ -241,6 +241,7 @@ struct GmlCtx {
 
 int  NewDatIdx(GmlCtx *g);
 bool AllocData(GmlCtx *g, Data *d, bool zero = true);
+int NewMeshTable(GmlCtx *g, int MshTyp, int NmbLin, bool zero);
 void FreeDataSlot(GmlCtx *g, int idx);
 bool EnsureHost(Data *d);
 size_t Upload(GmlCtx *g, int idx);

@@ -787,23 +787,27 @@ __global__ void __launch_bounds__(256) k_flag_emit(const uint4 *__restrict__ fla
 
 // First appearances through the vertex -> tet ball, for a mesh whose only
 // element kind is the tet and that has no table of the extracted kind yet:
-// item j of tet e (an edge or a face) appears first at position e * ni + j
+// item j of tet e (an edge or a face) appears first at position e * n + j
 // iff no tet of smaller index holds all its nodes.  Every tet holding the
 // item holds its lowest node a, so the ball of a holds them all.  The thread
 // of vertex a walks its ball once in ascending tet index (the ball is stored
 // in descending order) and keeps the items whose lowest node is a that it has
 // met, as keys in registers (the other node of an edge; the two other nodes
 // of a face, lower one in the high word): an item is first iff its key is
-// new (the list is searched 8 keys at a time, up to the keys in use).
-// Every position is written once, 0 or 1.  The low four bits of a ball
-// entry are a's local index in the tet, so the candidates of a tet are found
-// without a search: the edges {a, y} and the faces opposite each other node
-// y, item numbers from map (edges: nibble la * 4 + o for locals la, o; faces:
-// nibble o for the face without local o).  A vertex with more than S
-// distinct items splits them into P classes (key hash mod P) and walks its
-// ball once per class, P doubling until each class fits.  A tet that repeats
-// a vertex, or a ball entry that does not name a, sets *fallback (the
-// general path then runs).
+// new (the list is searched 8 keys at a time, up to the keys in use).  The
+// low four bits of a ball entry are a's local index la in the tet, so the
+// candidates of a tet are found without a search: the edges {a, y} and the
+// faces opposite each other node y, item numbers from map (edges: nibble
+// la * 4 + o for locals la, o; faces: nibble o for the face without local o).
+// Output: byte e * 4 + la of mark (zeroed by the caller) gets bit j for each
+// first item j of tet e whose lowest node is a, written only when some bit
+// is set (at most one store per first item, where a byte per item and
+// position costs a scattered store per item); every item has one lowest
+// node, so the OR of a tet's four bytes is the set of its first items.  A
+// vertex with more than S distinct items splits them into P classes (key
+// hash mod P) and walks its ball once per class, P doubling until each class
+// fits.  A tet that repeats a vertex, or a ball entry that does not name a,
+// sets *fallback (the general path then runs).
 constexpr int kBallMaxDeg = 256;   // wider balls take the general path
 constexpr int kBallMaxPass = 64;
 
@@ -836,10 +840,14 @@ template <int KD, int S>
 __global__ void __launch_bounds__(256) k_item_first_ball(long long nv, const int *__restrict__ deg,
                                                          const unsigned int *__restrict__ off,
                                                          const int *__restrict__ ent, const int4 *__restrict__ tet,
-                                                         int ni, unsigned long long map,
-                                                         unsigned char *__restrict__ flag, int *fallback)
+                                                         unsigned long long map, unsigned char *__restrict__ mark,
+                                                         int *fallback)
 {
-   const long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   // work-groups go to the eight XCDs in turn: each XCD takes one contiguous
+   // eighth of the vertices (the grid is a multiple of 8), so the mark lines
+   // of a range of tets are written in one L2
+   const unsigned int per = gridDim.x / 8, vb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+   const long long a = (long long)vb * blockDim.x + threadIdx.x;
    if(a >= nv) return;
    const int d = deg[a], ia = (int)a;
    const size_t base = (size_t)off[a >> 6] * 64 + (a & 63);
@@ -875,6 +883,7 @@ __global__ void __launch_bounds__(256) k_item_first_ball(long long nv, const int
             // the other three nodes in local order, and their local indices
             const int y[3] = {la == 0 ? q.y : q.x, la <= 1 ? q.z : q.y, la <= 2 ? q.w : q.z};
             const int o[3] = {la == 0 ? 1 : 0, la <= 1 ? 2 : 1, la <= 2 ? 3 : 2};
+            unsigned int bits = 0;
 #pragma unroll
             for(int i = 0; i < 3; i++)
             {
@@ -897,16 +906,25 @@ __global__ void __launch_bounds__(256) k_item_first_ball(long long nv, const int
                }
                if(mine && !over)
                {
-                  const bool found = seen_has<Key, S>(seen, cnt, key);
-                  if(!found)
+                  if(!seen_has<Key, S>(seen, cnt, key))
                   {
                      if(cnt == S)
                         over = true;
                      else
+                     {
                         seen_put<Key, S>(seen, cnt++, key);
+                        bits |= 1u << j;
+                     }
                   }
-                  if(!over) flag[e * ni + j] = found ? 0 : 1;
                }
+            }
+            // a set bit is a first item whatever happens later in the walk;
+            // a walk split into classes adds its bits to the earlier ones
+            // (the byte has one writer: the thread of a)
+            if(bits)
+            {
+               unsigned char *mb = &mark[e * 4 + la];
+               *mb = (unsigned char)(P == 1 ? bits : *mb | bits);
             }
          }
       }
@@ -951,6 +969,74 @@ static bool BallItemMap(const LocalItems &li, int deg, unsigned long long *map)
       return false;
    *map = m;
    return true;
+}
+
+// Compaction of the first items marked by k_item_first_ball, in position
+// order: kTetChunk tets per work-group, two per thread (one 8-byte load of
+// their mark words; the marks past the last tet are zero).  A count pass, a
+// scan of the per-chunk counts, then the write pass: each first item's row,
+// its nodes sorted ascending (the row k_item_write_pos decodes from the
+// key), W columns with zeros past the item's nodes, ref 0.  The write pass
+// builds the work-group's rows in LDS (at most NI per tet) and stores them
+// as whole consecutive rows.
+constexpr int kTetChunk = 512;
+
+__device__ __forceinline__ unsigned int tet_marks(unsigned int w)
+{
+   return (w | w >> 8 | w >> 16 | w >> 24) & 0xffu;
+}
+
+__device__ __forceinline__ int tet_node(const int4 &q, int i)
+{
+   return i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w;
+}
+
+__global__ void __launch_bounds__(256) k_tmark_count(const uint2 *__restrict__ mark, int *__restrict__ ccnt)
+{
+   typedef hipcub::BlockReduce<int, 256> Red;
+   __shared__ typename Red::TempStorage tmp;
+   const uint2 v = mark[(size_t)blockIdx.x * 256 + threadIdx.x];
+   const int t = Red(tmp).Sum(__popc(tet_marks(v.x)) + __popc(tet_marks(v.y)));
+   if(threadIdx.x == 0) ccnt[blockIdx.x] = t;
+}
+
+template <int W, int NI>
+__global__ void __launch_bounds__(256) k_tmark_write(const uint2 *__restrict__ mark, const int *__restrict__ coff,
+                                                     const int4 *__restrict__ tet, LocalItems li, int deg,
+                                                     int *__restrict__ tab, int *__restrict__ ref)
+{
+   typedef hipcub::BlockScan<int, 256> Scan;
+   typedef typename std::conditional<W == 4, int4, int2>::type Row;
+   __shared__ typename Scan::TempStorage tmp;
+   __shared__ Row stage[kTetChunk * NI];
+   const uint2 v = mark[(size_t)blockIdx.x * 256 + threadIdx.x];
+   const unsigned int m[2] = {tet_marks(v.x), tet_marks(v.y)};
+   int at, tot;
+   Scan(tmp).ExclusiveSum(__popc(m[0]) + __popc(m[1]), at, tot);
+   const long long e0 = (long long)blockIdx.x * kTetChunk + threadIdx.x * 2;
+   int *st = reinterpret_cast<int *>(stage);
+   for(int t = 0; t < 2; t++)
+   {
+      if(!m[t]) continue;
+      const int4 q = tet[e0 + t];
+      for(int j = 0; j < NI; j++)
+      {
+         if(!((m[t] >> j) & 1u)) continue;
+         int a[4] = {0, 0, 0, 0};
+         for(int k = 0; k < deg; k++) a[k] = tet_node(q, li.nod[j][k]);
+         sort_nodes(a, deg);
+         for(int c = 0; c < W; c++) st[at * W + c] = c < deg ? a[c] : 0;
+         at++;
+      }
+   }
+   __syncthreads();
+   const size_t base = (size_t)coff[blockIdx.x];
+   Row *out = reinterpret_cast<Row *>(tab) + base;
+   for(int r = threadIdx.x; r < tot; r += 256)
+   {
+      out[r] = stage[r];
+      ref[base + r] = 0;
+   }
 }
 
 // the sources of an extraction and the first item position of each
@@ -1129,60 +1215,80 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    int rc = -1, hbad = 0, m = 0;
    const long long nchunk = (total + kFlagChunk - 1) / kFlagChunk;
    // 0. a mesh of tets only and no table of the kind yet: first appearances
-   // through the vertex -> tet ball (built if needed), flagged per position
-   // by k_item_first_ball; the general path below otherwise, and when the
-   // ball cannot be built, is wider than kBallMaxDeg or a tet repeats a vertex
+   // through the vertex -> tet ball (built if needed), marked per tet by
+   // k_item_first_ball, counted, and the new table written from the marks;
+   // the general path below otherwise, and when the ball cannot be built, is
+   // wider than kBallMaxDeg or a tet repeats a vertex
    bool viaball = false;
+   int ballidx = 0;
    unsigned long long bmap = 0;
    if(!old && src.size() == 1 && kinds[0] == GmlTetrahedra && src[0].len == 4 && BallItemMap(src[0].li, deg, &bmap)
    && !g->host_only
    && BuildUplink(g, GmlVertices, GmlTetrahedra) && g->up[GmlVertices][GmlTetrahedra].MaxDeg <= kBallMaxDeg)
    {
       const Uplink &u = g->up[GmlVertices][GmlTetrahedra];
+      const long long ntc = (src[0].ne + kTetChunk - 1) / kTetChunk;
+      const size_t mbytes = (size_t)ntc * kTetChunk * 4;
       int *fb = nullptr, hfb = 1;
+      unsigned char *mark = nullptr;
       size_t t3 = 0;
-      bool fok = DAlloc(g, &fb, 4) && ok_(hipMemsetAsync(fb, 0, 4, s)) && DAlloc(g, &flag, nchunk * kFlagChunk)
-              && ok_(hipMemsetAsync(flag + total, 0, nchunk * kFlagChunk - total, s))
-              && DAlloc(g, &ccnt, (nchunk + 1) * 4) && DAlloc(g, &coff, (nchunk + 1) * 4)
-              && ok_(hipMemsetAsync(ccnt + nchunk, 0, 4, s))
-              && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, ccnt, coff, (int)(nchunk + 1), s))
+      bool fok = ntc > 0 && DAlloc(g, &fb, 4) && ok_(hipMemsetAsync(fb, 0, 4, s)) && DAlloc(g, &mark, mbytes)
+              && ok_(hipMemsetAsync(mark, 0, mbytes, s))
+              && DAlloc(g, &ccnt, (ntc + 1) * 4) && DAlloc(g, &coff, (ntc + 1) * 4)
+              && ok_(hipMemsetAsync(ccnt + ntc, 0, 4, s))
+              && ok_(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, ccnt, coff, (int)(ntc + 1), s))
               && DAlloc(g, &xtmp, t3);
       tim[1] = clock();
       if(fok)
       {
-         const dim3 grid((unsigned int)((nv + 255) / 256));
+         const dim3 grid((unsigned int)((nv + 255) / 256 + 7) / 8 * 8);
          const int *bdeg = (const int *)g->dat[u.DegIdx].dev, *bent = (const int *)g->dat[u.BalIdx].dev;
          const unsigned int *boff = (const unsigned int *)g->dat[u.OffIdx].dev;
          if(deg == 2)
             hipLaunchKernelGGL((k_item_first_ball<2, 16>), grid, dim3(256), 0, s, nv, bdeg, boff, bent,
-                               (const int4 *)src[0].ele, src[0].li.n, bmap, flag, fb);
+                               (const int4 *)src[0].ele, bmap, mark, fb);
          else
             hipLaunchKernelGGL((k_item_first_ball<3, 32>), grid, dim3(256), 0, s, nv, bdeg, boff, bent,
-                               (const int4 *)src[0].ele, src[0].li.n, bmap, flag, fb);
+                               (const int4 *)src[0].ele, bmap, mark, fb);
          fok = ok_(hipGetLastError()) && ok_(hipMemcpyAsync(&hfb, fb, 4, hipMemcpyDeviceToHost, s))
             && ok_(hipStreamSynchronize(s));
       }
       tim[2] = clock();
       if(fok && !hfb)
       {
-         hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, ccnt);
+         hipLaunchKernelGGL(k_tmark_count, dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark, ccnt);
          size_t t = t3;
-         fok = ok_(hipGetLastError()) && ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, ccnt, coff, (int)(nchunk + 1), s))
-            && ok_(hipMemcpyAsync(&m, coff + nchunk, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s))
-            && DAlloc(g, &sel, (size_t)m * 4 + 4);
-         if(fok)
+         fok = ok_(hipGetLastError()) && ok_(hipcub::DeviceScan::ExclusiveSum(xtmp, t, ccnt, coff, (int)(ntc + 1), s))
+            && ok_(hipMemcpyAsync(&m, coff + ntc, 4, hipMemcpyDeviceToHost, s)) && ok_(hipStreamSynchronize(s));
+         ballidx = fok ? NewMeshTable(g, typ, m, false) : 0;
+         if(ballidx)
          {
-            hipLaunchKernelGGL(k_flag_emit, dim3((unsigned)nchunk), dim3(256), 0, s, (const uint4 *)flag, coff, sel);
-            fok = ok_(hipGetLastError());
+            const Data *d = &g->dat[ballidx];
+            int *tab = (int *)d->dev, *ref = (int *)g->dat[g->RefIdx[typ]].dev;
+            if(deg == 2 && d->ItmLen == 2)
+               hipLaunchKernelGGL((k_tmark_write<2, 6>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
+                                  coff, (const int4 *)src[0].ele, src[0].li, deg, tab, ref);
+            else if(deg == 3 && d->ItmLen == 4)
+               hipLaunchKernelGGL((k_tmark_write<4, 4>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
+                                  coff, (const int4 *)src[0].ele, src[0].li, deg, tab, ref);
+            else
+               fok = false;
+            fok = fok && ok_(hipGetLastError()) && ok_(hipStreamSynchronize(s));
+            if(!fok)
+            {
+               GmlFreeData(GmlIdx, ballidx);
+               ballidx = 0;
+            }
          }
-         viaball = fok;
+         viaball = ballidx != 0;
       }
       ItemFree(g, fb);
+      ItemFree(g, mark);
       if(!viaball)
       {
          (void)hipGetLastError();
-         for(void *q : {(void *)flag, (void *)ccnt, (void *)coff, xtmp, (void *)sel}) ItemFree(g, q);
-         flag = nullptr; ccnt = nullptr; coff = nullptr; xtmp = nullptr; sel = nullptr;
+         for(void *q : {(void *)ccnt, (void *)coff, xtmp}) ItemFree(g, q);
+         ccnt = nullptr; coff = nullptr; xtmp = nullptr;
          m = 0;
       }
    }
@@ -1320,16 +1426,20 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    }
    }
    tim[3] = clock();
-   // 3. the new table (created before the old one is dropped: the old refs are read)
-   if(ok)
+   // 3. the new table (created before the old one is dropped: the old refs
+   // are read), not zero-filled: the row kernels write every row and ref;
+   // the vertex-ball path has written its table already
+   if(ok && viaball)
+      rc = m;
+   else if(ok)
    {
-      const int nidx = GmlNewMeshData(GmlIdx, typ, m);
+      const int nidx = NewMeshTable(g, typ, m, false);
       if(!nidx) ok = false;
       else
       {
          const Data *d = &g->dat[nidx];
          const int *orf = oref ? (const int *)g->dat[oref].dev : nullptr;
-         if(bucketed || viaball)
+         if(bucketed)
             hipLaunchKernelGGL(k_item_write_pos, blocks(m), dim3(kB), 0, s, (long long)m, sel, ss, kf, nv, nold, orf,
                                d->ItmLen, (int *)d->dev, (int *)g->dat[g->RefIdx[typ]].dev);
          else

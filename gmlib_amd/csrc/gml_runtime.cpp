@@ -232,6 +232,58 @@ int GetCntVec(int siz, int *cnt, int *vec, int *typ)
    return p;
 }
 
+// a mesh table and its refs (zero-filled unless the caller writes every row
+// and ref itself); the element data index, 0 on failure
+int NewMeshTable(GmlCtx *g, int MshTyp, int NmbLin, bool zero)
+{
+   if(MshTyp < 0 || MshTyp >= GmlMaxEleTyp || NmbLin < 0) return 0;
+   int EleIdx = NewDatIdx(g);
+   if(!EleIdx) return 0;
+   Data *e = &g->dat[EleIdx];
+   e->kind = DkEle;
+   e->MshTyp = MshTyp;
+   e->ItmTyp = MshItmTyp[MshTyp];
+   e->NmbItm = 1;
+   e->ItmSiz = OclTypSiz[e->ItmTyp];
+   e->ItmLen = TypVecSiz[e->ItmTyp];
+   e->NmbLin = NmbLin;
+   e->LinSiz = (size_t)e->ItmSiz;
+   e->MemSiz = (size_t)NmbLin * e->LinSiz;
+   e->nam = MshTypStr[MshTyp];
+   if(!AllocData(g, e, zero)) return 0;
+
+   int RefIdx = NewDatIdx(g);
+   if(!RefIdx)
+   {
+      FreeDataSlot(g, EleIdx);
+      return 0;
+   }
+   Data *r = &g->dat[RefIdx];
+   r->kind = DkRef;
+   r->MshTyp = MshTyp;
+   r->ItmTyp = GmlInt;
+   r->NmbItm = 1;
+   r->ItmSiz = 4;
+   r->ItmLen = 0;
+   r->NmbLin = NmbLin;
+   r->LinSiz = 4;
+   r->MemSiz = (size_t)NmbLin * 4;
+   r->nam = MshRefStr[MshTyp];
+   if(!AllocData(g, r, zero))
+   {
+      FreeDataSlot(g, EleIdx);
+      return 0;
+   }
+
+   g->NmbEle[MshTyp] = NmbLin;
+   g->TypIdx[MshTyp] = EleIdx;
+   g->LnkMat[MshTyp][GmlVertices] = EleIdx;   // the element table is the downlink to vertices
+   g->RefIdx[MshTyp] = RefIdx;
+   DbgPrint(g, "Allocated mesh data: index=%2d, size=%zu bytes\nAllocated ref  data: index=%2d, size=%zu bytes\n",
+            EleIdx, e->MemSiz, RefIdx, r->MemSiz);
+   return EleIdx;
+}
+
 } // namespace gml
 
 using namespace gml;
@@ -381,44 +433,7 @@ void *GmlNewParameters(size_t GmlIdx, int siz, char *src)
 int GmlNewMeshData(size_t GmlIdx, int MshTyp, int NmbLin)
 {
    GML_CTX(g, GmlIdx);
-   if(MshTyp < 0 || MshTyp >= GmlMaxEleTyp || NmbLin < 0) return 0;
-   int EleIdx = NewDatIdx(g);
-   if(!EleIdx) return 0;
-   Data *e = &g->dat[EleIdx];
-   e->kind = DkEle;
-   e->MshTyp = MshTyp;
-   e->ItmTyp = MshItmTyp[MshTyp];
-   e->NmbItm = 1;
-   e->ItmSiz = OclTypSiz[e->ItmTyp];
-   e->ItmLen = TypVecSiz[e->ItmTyp];
-   e->NmbLin = NmbLin;
-   e->LinSiz = (size_t)e->ItmSiz;
-   e->MemSiz = (size_t)NmbLin * e->LinSiz;
-   e->nam = MshTypStr[MshTyp];
-   if(!AllocData(g, e)) return 0;
-
-   int RefIdx = NewDatIdx(g);
-   if(!RefIdx) return 0;
-   Data *r = &g->dat[RefIdx];
-   r->kind = DkRef;
-   r->MshTyp = MshTyp;
-   r->ItmTyp = GmlInt;
-   r->NmbItm = 1;
-   r->ItmSiz = 4;
-   r->ItmLen = 0;
-   r->NmbLin = NmbLin;
-   r->LinSiz = 4;
-   r->MemSiz = (size_t)NmbLin * 4;
-   r->nam = MshRefStr[MshTyp];
-   if(!AllocData(g, r)) return 0;
-
-   g->NmbEle[MshTyp] = NmbLin;
-   g->TypIdx[MshTyp] = EleIdx;
-   g->LnkMat[MshTyp][GmlVertices] = EleIdx;   // the element table is the downlink to vertices
-   g->RefIdx[MshTyp] = RefIdx;
-   DbgPrint(g, "Allocated mesh data: index=%2d, size=%zu bytes\nAllocated ref  data: index=%2d, size=%zu bytes\n",
-            EleIdx, e->MemSiz, RefIdx, r->MemSiz);
-   return EleIdx;
+   return NewMeshTable(g, MshTyp, NmbLin, true);
 }
 
 int GmlNewSolutionData(size_t GmlIdx, int MshTyp, int NmbDat, int ItmTyp, char *nam)
