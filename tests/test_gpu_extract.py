@@ -356,9 +356,10 @@ def test_gpu_extract_two_word_keys_against_oracle(capfd):
 
 @pytest.mark.parametrize("nt", [100, 250])
 def test_gpu_extract_wide_ball_fan(capfd, nt):
-    # a fan of nt tets (0, i, i+1, i+2): vertex 0's ball is wider than the
-    # rows the vertex-ball kernel keeps in LDS (the rest are read from the
-    # table) and still within its degree bound; tables equal the oracle's
+    # a fan of nt tets (0, i, i+1, i+2): vertex 0 is the lowest node of
+    # nt + 2 edges and 2 nt + 1 faces, more than the vertex-ball walk's key
+    # list holds, so its walk splits them into classes; its ball is within the
+    # walk's degree bound; tables equal the oracle's
     tet = np.array([[0, i, i + 1, i + 2] for i in range(1, nt + 1)], np.int32)
     nv = nt + 3
     want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
