@@ -393,3 +393,51 @@ def test_gpu_extract_degenerate_tet_leaves_ball_path(capfd):
     assert pe in ("bucketed", "sorted") and pf in ("bucketed", "sorted")
     assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
     assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+def test_gpu_extract_ball_path_sparse_ids(capfd):
+    # the 40^3 cube's node ids spread 64 apart: 63 of every 64 vertices are in
+    # no tet (empty balls); tet-only, so the vertex-ball path runs; tables
+    # equal the oracle's
+    crd, ref, tet = G.cube_mesh(40, 0.2, 23)
+    tet = np.ascontiguousarray(tet * 64)
+    nv = len(crd) * 64
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        set_vertices(g, nv)
+        set_block(g, G.GmlTetrahedra, tet, np.zeros(len(tet), np.int32))
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe == "vertex-ball" and pf == "vertex-ball"
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+@pytest.mark.parametrize("nv,nt", [(100, 2000), (3000, 20000)])
+def test_gpu_extract_ball_path_random_soup(capfd, nv, nt):
+    # random tets (four distinct vertices each) over few vertices, plus a
+    # "book" of 50 tets around the face (0, 1, 2), in shuffled order: faces
+    # and edges shared by any number of tets, vertices owning up to hundreds
+    # of distinct items (the walk splits them into classes); tables equal the
+    # oracle's
+    rng = np.random.default_rng(nv)
+    soup = [rng.choice(nv, 4, replace=False) for _ in range(nt)]
+    book = [[0, 1, 2, 3 + i] for i in range(50)]
+    tet = np.array(soup + book, np.int32)[rng.permutation(nt + 50)]
+    tet = np.ascontiguousarray(tet)
+    nt = len(tet)
+    want_e = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want_f = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+    with G.Gml() as g:
+        set_vertices(g, nv)
+        set_block(g, G.GmlTetrahedra, tet, np.zeros(nt, np.int32))
+        ne, pe = extract_logged(g, "edges", capfd)
+        nf, pf = extract_logged(g, "faces", capfd)
+        e = get_block(g, G.GmlEdges, ne, 2)
+        f = get_block(g, G.GmlTriangles, nf, 3)
+    assert pe == "vertex-ball" and pf == "vertex-ball"
+    assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
+    assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
