@@ -81,6 +81,8 @@ struct Uplink {
    int   DegIdx = 0, BalIdx = 0, OffIdx = 0;
    int   BasWid = 0, HghWid = 0, MaxPos = 0, MaxDeg = 0;
    long long nslab = 0;
+   long long SrcCnt = 0;                   // source entities it was built for
+   unsigned long long DstVer = 0;          // version of the element table it was built from
    StagePlan *stage[3] = {nullptr, nullptr, nullptr};   // work-group size 64, 128, 256
 };
 

@@ -374,6 +374,8 @@ static int RegisterUplink(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int
    u->BasWid = LenMatBas[Src][Dst];
    u->MaxDeg = maxdeg;
    u->nslab = nslab;
+   u->SrcCnt = nsrc;
+   u->DstVer = g->dat[g->TypIdx[Dst]].ver;
    if(firsthigh >= 0 && maxdeg > u->BasWid)
    {
       int w = 1;

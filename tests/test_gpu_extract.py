@@ -441,3 +441,55 @@ def test_gpu_extract_ball_path_random_soup(capfd, nv, nt):
     assert pe == "vertex-ball" and pf == "vertex-ball"
     assert np.array_equal(e[0], want_e[0]) and np.array_equal(e[1], want_e[1])
     assert np.array_equal(f[0], want_f[0]) and np.array_equal(f[1], want_f[1])
+
+
+def _drop(g, typ):
+    idx = C.c_int(0)
+    if g.L.GmlGetMeshInfo(g.h, typ, None, C.byref(idx)):
+        assert g.L.GmlFreeData(g.h, idx.value)
+
+
+def _rewrite_tets(g, tet):
+    # new contents for the existing tet table (same count)
+    nt = len(tet)
+    tref = np.zeros(nt, np.int32)
+    assert g.L.GmlSetDataBlock(g.h, G.GmlTetrahedra, 0, nt - 1, G.ptr(tet), C.c_void_p(tet.ctypes.data + (nt - 1) * 16),
+                               G.ptr(tref), C.c_void_p(tref.ctypes.data + (nt - 1) * 4))
+
+
+def test_gpu_extract_ball_follows_table_changes(capfd):
+    # the vertex -> tet ball kept by a first extraction is not walked once the
+    # tet table has new contents or the vertex count has changed: the general
+    # path runs and the tables equal the oracle's for the current mesh
+    crd, ref, tet = G.cube_mesh(12, 0.2, 5)
+    rng = np.random.default_rng(8)
+    pv = rng.permutation(len(crd))
+    inv = np.empty_like(pv)
+    inv[pv] = np.arange(len(crd))
+    tet2 = np.ascontiguousarray(inv[tet[rng.permutation(len(tet))]].astype(np.int32))
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        ne, pe = extract_logged(g, "edges", capfd)
+        want = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+        assert pe == "vertex-ball" and np.array_equal(get_block(g, G.GmlEdges, ne, 2)[0], want[0])
+        # same table, new contents
+        _rewrite_tets(g, tet2)
+        _drop(g, G.GmlEdges)
+        ne, pe = extract_logged(g, "edges", capfd)
+        want = O.extract_items(2, [(G.GmlTetrahedra, tet2)])
+        assert pe in ("bucketed", "sorted")
+        e = get_block(g, G.GmlEdges, ne, 2)
+        assert np.array_equal(e[0], want[0]) and np.array_equal(e[1], want[1])
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        nf, pf = extract_logged(g, "faces", capfd)
+        assert pf == "vertex-ball"
+        # a larger vertex table (the tets unchanged): the ball's rows are for
+        # the old count
+        set_vertices(g, 2 * len(crd))
+        _drop(g, G.GmlTriangles)
+        nf, pf = extract_logged(g, "faces", capfd)
+        want = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+        assert pf in ("bucketed", "sorted")
+        f = get_block(g, G.GmlTriangles, nf, 3)
+        assert np.array_equal(f[0], want[0]) and np.array_equal(f[1], want[1])
