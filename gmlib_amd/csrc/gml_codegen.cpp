@@ -1273,6 +1273,7 @@ static int CompileKernelTab(GmlCtx *g, const char *KrnSrc, const char *PrcNam, i
       else if(MshTypDim[MshTyp] < MshTypDim[Dst])
       {
          if(!g->up[MshTyp][Dst].BalIdx && !BuildUplink(g, MshTyp, Dst)) return 0;
+         g->up[MshTyp][Dst].pinned = true;   // the kernel holds its slots
          LnkTab[i] = g->up[MshTyp][Dst].BalIdx;
          CntTab[i] = g->up[MshTyp][Dst].DegIdx;
          UpSrc[i] = MshTyp; UpDst[i] = Dst;

@@ -83,6 +83,8 @@ struct Uplink {
    long long nslab = 0;
    long long SrcCnt = 0;                   // source entities it was built for
    unsigned long long DstVer = 0;          // version of the element table it was built from
+   unsigned long long SrcVer = 0;          // version of the source table (0 for vertices)
+   bool  pinned = false;                   // held by a compiled kernel or handed to the caller
    StagePlan *stage[3] = {nullptr, nullptr, nullptr};   // work-group size 64, 128, 256
 };
 
@@ -215,6 +217,7 @@ struct GmlCtx {
    SweepPlan   *plan = nullptr;             // exact fused form
    SweepPlan   *oplan = nullptr;            // operator form
    void        *xalt = nullptr;
+   long long    xalt_n = 0;                 // vertices xalt was allocated for
    // batches of meshes processed together (gml_batch.hip)
    Batch       *bat[GmlxMaxBatch + 1] = {nullptr};
    // host-only handle (GmlxInitHostOnly): data in host memory, kernels are
@@ -256,6 +259,12 @@ void DbgPrint(GmlCtx *g, const char *fmt, ...);
 
 // topology (gml_topology.cpp / gml_topology_kernels.hip)
 int  BuildUplink(GmlCtx *g, int SrcTyp, int DstTyp);
+// the uplink of the current tables: built if absent; one built from other
+// table contents or another source count is dropped and built again, unless
+// it is pinned (then null)
+Uplink *CurrentUplink(GmlCtx *g, int SrcTyp, int DstTyp);
+// free an uplink's data slots and the plans built on it, and forget it
+void DropUplink(GmlCtx *g, int SrcTyp, int DstTyp);
 int  BuildDownlink(GmlCtx *g, int SrcTyp, int DstTyp);
 int  gpu_vertex_ball(hipStream_t s, const int *ele, int stride, int nnod, long long nele,
                      long long nver, int *deg, unsigned int **off_out, int **ent_out,

@@ -1219,17 +1219,15 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
    // k_item_first_ball, counted, and the new table written from the marks;
    // the general path below otherwise, and when the ball cannot be built, is
    // wider than kBallMaxDeg or a tet repeats a vertex.  A ball kept from an
-   // earlier call (or built for the user's kernels) is walked only while the
-   // tet table is the one it was built from and the vertex count unchanged;
-   // a stale ball is left as it is (the user's kernels may hold it) and the
-   // general path runs
+   // earlier call is rebuilt when the tet table or the vertex count has
+   // changed since; one that a compiled kernel holds is left as it is and
+   // the general path runs
    bool viaball = false;
    int ballidx = 0;
    unsigned long long bmap = 0;
    if(!old && src.size() == 1 && kinds[0] == GmlTetrahedra && src[0].len == 4 && BallItemMap(src[0].li, deg, &bmap)
-   && !g->host_only && BuildUplink(g, GmlVertices, GmlTetrahedra)
-   && g->up[GmlVertices][GmlTetrahedra].MaxDeg <= kBallMaxDeg && g->up[GmlVertices][GmlTetrahedra].SrcCnt == nv
-   && g->up[GmlVertices][GmlTetrahedra].DstVer == g->dat[g->TypIdx[GmlTetrahedra]].ver)
+   && !g->host_only && CurrentUplink(g, GmlVertices, GmlTetrahedra)
+   && g->up[GmlVertices][GmlTetrahedra].MaxDeg <= kBallMaxDeg)
    {
       const Uplink &u = g->up[GmlVertices][GmlTetrahedra];
       const long long ntc = (src[0].ne + kTetChunk - 1) / kTetChunk;

@@ -408,8 +408,8 @@ int GmlxBallMin(size_t GmlIdx, int TetValIdx, int VerOutIdx)
    if(!a || !o || !a->use || !o->use || a->ItmTyp != GmlFlt || o->ItmTyp != GmlFlt || a->NmbItm != 1 || o->NmbItm != 1
    || a->NmbLin != g->NmbEle[GmlTetrahedra] || o->NmbLin != g->NmbEle[GmlVertices])
       return -2;
-   if(!g->up[GmlVertices][GmlTetrahedra].BalIdx && !BuildUplink(g, GmlVertices, GmlTetrahedra)) return -3;
-   Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
+   Uplink *u = CurrentUplink(g, GmlVertices, GmlTetrahedra);
+   if(!u) return -3;
    long long nv = g->NmbEle[GmlVertices];
    const Shape sh = g->shape[GmlxBuiltinBallMin];
    o->host_valid = false;
@@ -429,9 +429,15 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
    Data *b = (BaryIdx >= 1 && BaryIdx <= GmlMaxDat) ? &g->dat[BaryIdx] : nullptr;
    if(!b || !b->use || b->ItmTyp != GmlFlt4 || b->NmbItm != 1 || b->NmbLin != g->NmbEle[GmlTetrahedra])
       return -2;
-   if(!g->up[GmlVertices][GmlTetrahedra].BalIdx && !BuildUplink(g, GmlVertices, GmlTetrahedra)) return -3;
-   Uplink *u = &g->up[GmlVertices][GmlTetrahedra];
+   Uplink *u = CurrentUplink(g, GmlVertices, GmlTetrahedra);
+   if(!u) return -3;
    long long nv = g->NmbEle[GmlVertices], nt = g->NmbEle[GmlTetrahedra];
+   // the alternate coordinate buffer follows the vertex count
+   if(g->xalt && g->xalt_n != nv)
+   {
+      DevFree(g, g->xalt);
+      g->xalt = nullptr;
+   }
    Data *vd = &g->dat[g->TypIdx[GmlVertices]];
    const v4i *tet = (const v4i *)g->dat[g->TypIdx[GmlTetrahedra]].dev;
    const int *ref = (const int *)g->dat[g->RefIdx[GmlVertices]].dev;
@@ -445,6 +451,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
       if(g->oplan && g->oplan->TV != sf.grp) { FreeSweepPlan(g, g->oplan); g->oplan = nullptr; }
       if(!g->oplan && !(g->oplan = NewSweepPlan(g, sf.grp, 1))) return -4;
       if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
+      g->xalt_n = nv;
       SweepPlan *p = g->oplan;
       size_t lds = (size_t)p->max_nv * 16;
       if(lds > 160 * 1024) return -4;
@@ -471,6 +478,7 @@ int GmlxSmoothSweeps(size_t GmlIdx, int BaryIdx, int nsweep, double omega)
       if(g->plan && g->plan->TV != sf.grp) { FreeSweepPlan(g, g->plan); g->plan = nullptr; }
       if(!g->plan && !(g->plan = NewSweepPlan(g, sf.grp, 0))) return -4;
       if(!g->xalt && !(g->xalt = DevAlloc(g, (size_t)nv * 16))) return -5;
+      g->xalt_n = nv;
       SweepPlan *p = g->plan;
       size_t lds = (size_t)(p->max_nv + p->max_nt) * 16;
       if(lds > 160 * 1024) return -4;

@@ -490,6 +490,17 @@ int GmlFreeData(size_t GmlIdx, int idx)
    GML_CTX(g, GmlIdx);
    if(idx < 1 || idx > GmlMaxDat || !g->dat[idx].use || !g->dat[idx].dev)
       return 0;
+   // a slot of an uplink takes the whole uplink (its other slots and plans)
+   for(int s = 0; s < GmlMaxEleTyp; s++)
+      for(int t = 0; t < GmlMaxEleTyp; t++)
+      {
+         const Uplink &u = g->up[s][t];
+         if(u.BalIdx && (idx == u.DegIdx || idx == u.BalIdx || idx == u.OffIdx))
+         {
+            DropUplink(g, s, t);
+            return 1;
+         }
+      }
    Data *d = &g->dat[idx];
    if(d->kind == DkEle)
    {

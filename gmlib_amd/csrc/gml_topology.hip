@@ -376,6 +376,7 @@ static int RegisterUplink(GmlCtx *g, int Src, int Dst, int *degdev, unsigned int
    u->nslab = nslab;
    u->SrcCnt = nsrc;
    u->DstVer = g->dat[g->TypIdx[Dst]].ver;
+   u->SrcVer = Src == GmlVertices ? 0 : g->dat[g->TypIdx[Src]].ver;
    if(firsthigh >= 0 && maxdeg > u->BasWid)
    {
       int w = 1;
@@ -591,6 +592,41 @@ int BuildDownlink(GmlCtx *g, int Src, int Dst)
    return 1;
 }
 
+void DropUplink(GmlCtx *g, int Src, int Dst)
+{
+   Uplink *u = &g->up[Src][Dst];
+   for(StagePlan *&p : u->stage)
+   {
+      FreeStagePlan(g, p);
+      p = nullptr;
+   }
+   for(int idx : {u->DegIdx, u->BalIdx, u->OffIdx})
+      if(idx && g->dat[idx].use) FreeDataSlot(g, idx);
+   if(Src == GmlVertices && Dst == GmlTetrahedra)
+   {
+      // the smoothing sweep plans are built on the vertex -> tet ball
+      FreeSweepPlan(g, g->plan);
+      FreeSweepPlan(g, g->oplan);
+      g->plan = g->oplan = nullptr;
+   }
+   *u = Uplink();
+}
+
+Uplink *CurrentUplink(GmlCtx *g, int Src, int Dst)
+{
+   Uplink *u = &g->up[Src][Dst];
+   const int di = g->TypIdx[Dst], si = Src == GmlVertices ? 0 : g->TypIdx[Src];
+   if(!di || (Src != GmlVertices && !si)) return nullptr;
+   if(u->BalIdx
+   && (u->SrcCnt != g->NmbEle[Src] || u->DstVer != g->dat[di].ver || (si && u->SrcVer != g->dat[si].ver)))
+   {
+      if(u->pinned) return nullptr;
+      DropUplink(g, Src, Dst);
+   }
+   if(!u->BalIdx && !BuildUplink(g, Src, Dst)) return nullptr;
+   return u;
+}
+
 } // namespace gml
 
 using namespace gml;
@@ -601,7 +637,12 @@ int GmlxBuildLink(size_t GmlIdx, int Src, int Dst)
 {
    GML_CTX(g, GmlIdx);
    if(Src < 0 || Src >= GmlMaxEleTyp || Dst < 0 || Dst >= GmlMaxEleTyp) return 0;
-   if(MshTypDim[Src] < MshTypDim[Dst]) return BuildUplink(g, Src, Dst);
+   if(MshTypDim[Src] < MshTypDim[Dst])
+   {
+      if(!BuildUplink(g, Src, Dst)) return 0;
+      g->up[Src][Dst].pinned = true;
+      return 1;
+   }
    if(MshTypDim[Src] > MshTypDim[Dst]) return Dst == GmlVertices ? (g->TypIdx[Src] ? 1 : 0) : BuildDownlink(g, Src, Dst);
    return 0;
 }
@@ -618,7 +659,7 @@ int GmlxGetUplinkTable(size_t GmlIdx, int Src, int Dst, int *deg, int *ball, int
    GML_CTX(g, GmlIdx);
    if(Src < 0 || Src >= GmlMaxEleTyp || Dst < 0 || Dst >= GmlMaxEleTyp) return 0;
    Uplink *u = &g->up[Src][Dst];
-   if(!u->BalIdx || width < u->MaxDeg) return 0;
+   if(!u->BalIdx || width < u->MaxDeg || u->SrcCnt != g->NmbEle[Src]) return 0;
    long long n = g->NmbEle[Src], nchunk = (n + 63) / 64;
    std::vector<unsigned int> off((size_t)nchunk + 1);
    std::vector<int> ent((size_t)u->nslab * 64 + 64);

@@ -108,3 +108,31 @@ def test_operator_sweep_degree_from_counts(degenerate):
         assert deg[a] == np.count_nonzero(tet == a)            # the repeated vertex's ball holds the tet twice
     c64 = O.smooth64(tet, ref, deg, ball, crd, 2, omega)
     assert np.abs(c[:, :3] - c64).max() <= 2e-6
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_builtin_sweeps_follow_a_new_mesh(mode):
+    # GmlxSmoothSweeps and GmlxBallMin on one handle, then again after a
+    # larger mesh is loaded into the same handle: the vertex ball, the sweep
+    # plans and the alternate coordinate buffer follow the new mesh;
+    # coordinates within 1e-5 of the oracle's FP32 sweeps, ball minimum
+    # bit-exact on the GPU's own qualities
+    omega = 0.2
+    with G.Gml() as g:
+        assert g.L.GmlxSetSmoothMode(g.h, mode)
+        for n in (6, 11):
+            crd, ref, tet = G.cube_mesh(n, 0.2, 4 + n)
+            V, T = len(crd), len(tet)
+            g.set_mesh(crd, ref, tet)
+            b = g.solution(G.GmlTetrahedra, 1, G.GmlFlt4, "Bary")
+            q = g.solution(G.GmlTetrahedra, 1, G.GmlFlt, "qal")
+            vq = g.solution(G.GmlVertices, 1, G.GmlFlt, "vq")
+            assert g.L.GmlxSmoothSweeps(g.h, b, 3, omega) == 1
+            c, _ = g.vertices(V)
+            deg, ball, _ = O.vertex_ball(tet, V, 32)
+            want = O.smooth32(tet, ref, deg, ball, crd, 3, omega)[:, :3]
+            assert float(np.abs(c[:, :3] - want).max()) <= 1e-5
+            assert g.L.GmlxTetQuality(g.h, q) == 1
+            assert g.L.GmlxBallMin(g.h, q, vq) == 1
+            qv = g.download(q, np.float32, (T,))
+            assert np.array_equal(g.download(vq, np.float32, (V,)), O.ball_min(deg, ball, qv))

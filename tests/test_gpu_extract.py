@@ -458,38 +458,50 @@ def _rewrite_tets(g, tet):
 
 
 def test_gpu_extract_ball_follows_table_changes(capfd):
-    # the vertex -> tet ball kept by a first extraction is not walked once the
-    # tet table has new contents or the vertex count has changed: the general
-    # path runs and the tables equal the oracle's for the current mesh
+    # the vertex -> tet ball kept by a first extraction is rebuilt once the
+    # tet table has new contents or the vertex count has changed; a ball that
+    # a caller holds (GmlxBuildLink) is left as it is and the general path
+    # runs; every table equals the oracle's for the current mesh
     crd, ref, tet = G.cube_mesh(12, 0.2, 5)
     rng = np.random.default_rng(8)
     pv = rng.permutation(len(crd))
     inv = np.empty_like(pv)
     inv[pv] = np.arange(len(crd))
     tet2 = np.ascontiguousarray(inv[tet[rng.permutation(len(tet))]].astype(np.int32))
-    with G.Gml() as g:
-        g.set_mesh(crd, ref, tet)
-        ne, pe = extract_logged(g, "edges", capfd)
-        want = O.extract_items(2, [(G.GmlTetrahedra, tet)])
-        assert pe == "vertex-ball" and np.array_equal(get_block(g, G.GmlEdges, ne, 2)[0], want[0])
-        # same table, new contents
-        _rewrite_tets(g, tet2)
+    want1 = O.extract_items(2, [(G.GmlTetrahedra, tet)])
+    want2 = O.extract_items(2, [(G.GmlTetrahedra, tet2)])
+    wantf = O.extract_items(3, [(G.GmlTetrahedra, tet)])
+
+    def edges(g, want, path):
         _drop(g, G.GmlEdges)
         ne, pe = extract_logged(g, "edges", capfd)
-        want = O.extract_items(2, [(G.GmlTetrahedra, tet2)])
-        assert pe in ("bucketed", "sorted")
         e = get_block(g, G.GmlEdges, ne, 2)
+        assert pe in path
         assert np.array_equal(e[0], want[0]) and np.array_equal(e[1], want[1])
+
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        edges(g, want1, ("vertex-ball",))
+        _rewrite_tets(g, tet2)                     # same table, new contents
+        edges(g, want2, ("vertex-ball",))
     with G.Gml() as g:
         g.set_mesh(crd, ref, tet)
         nf, pf = extract_logged(g, "faces", capfd)
         assert pf == "vertex-ball"
-        # a larger vertex table (the tets unchanged): the ball's rows are for
-        # the old count
-        set_vertices(g, 2 * len(crd))
+        set_vertices(g, 2 * len(crd))              # a larger vertex table
         _drop(g, G.GmlTriangles)
         nf, pf = extract_logged(g, "faces", capfd)
-        want = O.extract_items(3, [(G.GmlTetrahedra, tet)])
-        assert pf in ("bucketed", "sorted")
         f = get_block(g, G.GmlTriangles, nf, 3)
-        assert np.array_equal(f[0], want[0]) and np.array_equal(f[1], want[1])
+        assert pf == "vertex-ball"
+        assert np.array_equal(f[0], wantf[0]) and np.array_equal(f[1], wantf[1])
+    with G.Gml() as g:
+        g.set_mesh(crd, ref, tet)
+        assert g.L.GmlxBuildLink(g.h, G.GmlVertices, G.GmlTetrahedra)   # held by the caller
+        edges(g, want1, ("vertex-ball",))
+        _rewrite_tets(g, tet2)
+        edges(g, want2, ("bucketed", "sorted"))
+        # the built-in ball minimum refuses the stale ball it would read
+        q = g.L.GmlNewSolutionData(g.h, G.GmlTetrahedra, 1, G.GmlFlt, b"q")
+        o = g.L.GmlNewSolutionData(g.h, G.GmlVertices, 1, G.GmlFlt, b"o")
+        assert q and o
+        assert g.L.GmlxBallMin(g.h, q, o) == -3
