@@ -977,8 +977,8 @@ static bool BallItemMap(const LocalItems &li, int deg, unsigned long long *map)
 // scan of the per-chunk counts, then the write pass: each first item's row,
 // its nodes sorted ascending (the row k_item_write_pos decodes from the
 // key), W columns with zeros past the item's nodes, ref 0.  The write pass
-// builds the work-group's rows in LDS (at most NI per tet) and stores them
-// as whole consecutive rows.
+// builds the work-group's rows in LDS and stores them as whole consecutive
+// rows.
 constexpr int kTetChunk = 512;
 
 __device__ __forceinline__ unsigned int tet_marks(unsigned int w)
@@ -1000,7 +1000,7 @@ __global__ void __launch_bounds__(256) k_tmark_count(const uint2 *__restrict__ m
    if(threadIdx.x == 0) ccnt[blockIdx.x] = t;
 }
 
-template <int W, int NI>
+template <int W, int NI, int CAP>
 __global__ void __launch_bounds__(256) k_tmark_write(const uint2 *__restrict__ mark, const int *__restrict__ coff,
                                                      const int4 *__restrict__ tet, LocalItems li, int deg,
                                                      int *__restrict__ tab, int *__restrict__ ref)
@@ -1008,13 +1008,17 @@ __global__ void __launch_bounds__(256) k_tmark_write(const uint2 *__restrict__ m
    typedef hipcub::BlockScan<int, 256> Scan;
    typedef typename std::conditional<W == 4, int4, int2>::type Row;
    __shared__ typename Scan::TempStorage tmp;
-   __shared__ Row stage[kTetChunk * NI];
+   __shared__ Row stage[CAP];
    const uint2 v = mark[(size_t)blockIdx.x * 256 + threadIdx.x];
    const unsigned int m[2] = {tet_marks(v.x), tet_marks(v.y)};
    int at, tot;
    Scan(tmp).ExclusiveSum(__popc(m[0]) + __popc(m[1]), at, tot);
    const long long e0 = (long long)blockIdx.x * kTetChunk + threadIdx.x * 2;
-   int *st = reinterpret_cast<int *>(stage);
+   const size_t base = (size_t)coff[blockIdx.x];
+   Row *out = reinterpret_cast<Row *>(tab) + base;
+   // the stage holds CAP rows (a tet chunk has about two first faces or 1.2
+   // first edges per tet); a chunk with more writes its rows straight out
+   const bool staged = tot <= CAP;
    for(int t = 0; t < 2; t++)
    {
       if(!m[t]) continue;
@@ -1025,18 +1029,24 @@ __global__ void __launch_bounds__(256) k_tmark_write(const uint2 *__restrict__ m
          int a[4] = {0, 0, 0, 0};
          for(int k = 0; k < deg; k++) a[k] = tet_node(q, li.nod[j][k]);
          sort_nodes(a, deg);
-         for(int c = 0; c < W; c++) st[at * W + c] = c < deg ? a[c] : 0;
+         Row r;
+         if constexpr(W == 4)
+            r = make_int4(a[0], a[1], deg > 2 ? a[2] : 0, deg > 3 ? a[3] : 0);
+         else
+            r = make_int2(a[0], a[1]);
+         if(staged)
+            stage[at] = r;
+         else
+            out[at] = r;
          at++;
       }
    }
-   __syncthreads();
-   const size_t base = (size_t)coff[blockIdx.x];
-   Row *out = reinterpret_cast<Row *>(tab) + base;
-   for(int r = threadIdx.x; r < tot; r += 256)
+   if(staged)
    {
-      out[r] = stage[r];
-      ref[base + r] = 0;
+      __syncthreads();
+      for(int r = threadIdx.x; r < tot; r += 256) out[r] = stage[r];
    }
+   for(int r = threadIdx.x; r < tot; r += 256) ref[base + r] = 0;
 }
 
 // the sources of an extraction and the first item position of each
@@ -1269,10 +1279,10 @@ int GpuExtractItems(size_t GmlIdx, GmlCtx *g, int typ, int deg)
             const Data *d = &g->dat[ballidx];
             int *tab = (int *)d->dev, *ref = (int *)g->dat[g->RefIdx[typ]].dev;
             if(deg == 2 && d->ItmLen == 2)
-               hipLaunchKernelGGL((k_tmark_write<2, 6>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
+               hipLaunchKernelGGL((k_tmark_write<2, 6, 1024>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
                                   coff, (const int4 *)src[0].ele, src[0].li, deg, tab, ref);
             else if(deg == 3 && d->ItmLen == 4)
-               hipLaunchKernelGGL((k_tmark_write<4, 4>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
+               hipLaunchKernelGGL((k_tmark_write<4, 4, 1536>), dim3((unsigned)ntc), dim3(256), 0, s, (const uint2 *)mark,
                                   coff, (const int4 *)src[0].ele, src[0].li, deg, tab, ref);
             else
                fok = false;
