@@ -806,8 +806,9 @@ __global__ void __launch_bounds__(256) k_flag_emit(const uint4 *__restrict__ fla
 // node, so the OR of a tet's four bytes is the set of its first items.  A
 // vertex with more than S distinct items splits them into P classes (key
 // hash mod P) and walks its ball once per class, P doubling until each class
-// fits.  A tet that repeats a vertex, or a ball entry that does not name a,
-// sets *fallback (the general path then runs).
+// fits.  A tet that repeats a vertex, a ball entry that does not name a, or
+// a class still too large at kBallMaxPass classes sets *fallback (the
+// general path then runs).
 constexpr int kBallMaxDeg = 256;   // wider balls take the general path
 constexpr int kBallMaxPass = 64;
 
